@@ -1,0 +1,84 @@
+// engine.h -- device-visible descriptors shared by the kernels and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "geom.h"
+
+namespace pasta {
+
+constexpr int TILE = 1792;        // synthetic pixels per model-kernel workgroup
+constexpr int NT = 256;           // threads per workgroup (4 wavefronts)
+constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
+constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)
+constexpr int CH_MAX = 32;        // max chunks
+constexpr int P1_MAX = 32;        // max continuum parameters per chunk / linear columns
+constexpr int BLK = 2048;         // observed pixels per reduction block
+
+enum DispKind : int { DISP_NONE = 0, DISP_FIXED = 1, DISP_VARIABLE = 2 };
+enum ContKind : int { CONT_CHUNK = 0, CONT_LINEAR = 1, CONT_FIXED = 2, CONT_CONSTANT = 3 };
+
+struct DevGrid {
+    const uint16_t* spectra;  // n_models rows of `pitch` uint16 (pitch % 8 == 0)
+    const float* factors;     // per-model f32 factor (max/65535) or nullptr
+    const GridGeom* geom;
+    long long pitch;
+    int W, Wq;                // pixels; first pixel of the n % 128 tail
+    double sg_first, sg_step; // log10 of the first wavelength, log10 step
+    double synth_first_wl, synth_last_wl;  // 10^first, 10^(first + step*W)
+};
+
+struct DevStar {
+    int N, disp_kind, Kd, cont_kind;
+    int n_chunks, p1, ignore, n_params;
+    int n_mblocks, n_cblocks, n_tiles, pad0;
+    double chi2_div, wl0, wlN;
+    const float* flux;
+    const float* var;
+    const double* fidx;       // float index of each observed pixel in the synthetic grid
+    const double* wl;         // observed wavelengths
+    const float* dkern;       // fixed: Kd taps; variable: W rows of Kd taps
+    const double* chunk_lohi; // per chunk: wl[start], wl[stop-1]
+    const int* startstop;     // per chunk: start, stop
+    const double* chol;       // per chunk (or linear): p1 x p1 lower Cholesky factor of AᵀA
+    const float* fac;         // per observed pixel: overlap blend factor
+    const float* design;      // linear: p1 x N (column k contiguous)
+    const float* fixedc;      // fixed continuum
+    const int4* mblocks;      // moment blocks (chunk, i0, i1, 0)
+    const int* mb_off;        // chunk c owns mblocks [mb_off[c], mb_off[c+1])
+    const int4* cblocks;      // chi2 blocks (kind, chunk, i0, i1); kind 1 = overlap
+};
+
+// Per-evaluation record, written by k_prep.
+struct DevEval {
+    int star;
+    int status;
+    int krot;        // rotation taps; 0 = pass-through
+    int tile_base;   // first synthetic tile touched
+    int cmax;        // exclusive upper bound of readable model-segment indices
+    int active;      // 0 = skip (finished / failed star in the batched path)
+    double shift_log;
+    long long m_off;  // offset of this evaluation's resampled model in the model buffer
+    long long mp_off; // offset of its continuum-moment partials (doubles)
+    long long cp_off; // offset of its chi2 partials (doubles)
+    long long pf_off; // offset of its continuum parameters (floats)
+    int idx[64];
+    float w[64];
+    float f[64];
+};
+
+// Batched PSO state per star (device).
+struct DevPso {
+    Bounds b;
+    PsoParams s;
+    uint64_t key_iter;
+    int P;            // particles
+    int first_eval;   // index of particle 0's evaluation
+    int iter;         // next iteration number (1-based)
+    int max_iters;
+    int status;       // 0 running, ST_* error
+    int done;
+    double gcost;
+    double gbest[5];
+};
+
+}  // namespace pasta
