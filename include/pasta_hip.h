@@ -1,0 +1,222 @@
+/*
+ * pasta_hip.h -- C ABI of libpasta_hip.so, the MI355X (gfx950) model-evaluation
+ * path of PASTA.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status:
+ * PASTA_OK (0) on success, a negative PASTA_ERR_* otherwise, with a message in
+ * pasta_last_error() (thread-local).  Handles are immutable after creation and
+ * may be shared between threads; evaluation calls on one handle serialise on
+ * an internal lock.
+ *
+ * Each entry point names the reference (SimonVanSchuylenbergh/PASTA) interface
+ * it replaces.  Labels are always (Teff, [M/H], logg, vsini, RV), f64.
+ */
+#ifndef PASTA_HIP_H
+#define PASTA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PASTA_OK 0
+#define PASTA_ERR_ARG -1      /* bad argument / length mismatch */
+#define PASTA_ERR_HIP -2      /* HIP runtime error */
+#define PASTA_ERR_NOMEM -3    /* device allocation failed / admission refused */
+#define PASTA_ERR_MODEL -4    /* model evaluation failed (per-item status holds the reason) */
+#define PASTA_ERR_FIT -5      /* a fit failed (particle out of bounds, evaluation error) */
+#define PASTA_ERR_STATE -6    /* request unknown / not finished / cancelled */
+
+/* Per-evaluation status codes (status_out arrays). */
+#define PASTA_ST_OK 0
+#define PASTA_ST_BOUNDS 1     /* src/interpolate.rs:425-430 (labels outside the grid) */
+#define PASTA_ST_NEIGHBORS 2  /* src/cubic.rs:155 (not enough neighbours) */
+#define PASTA_ST_VSINI 3      /* src/convolve_rv.rs:489-491 (vsini < 0) */
+#define PASTA_ST_KERNEL 4     /* src/convolve_rv.rs:512-519 (rotation kernel > 2048 taps) */
+#define PASTA_ST_RANGE 5      /* src/convolve_rv.rs:549-567 (RV shift outside synthetic range) */
+#define PASTA_ST_INDEX 6      /* resample reads past the model segment (the reference panics) */
+#define PASTA_ST_PSO 7        /* src/particleswarm.rs:267-275 (particle left the bounds) */
+#define PASTA_ST_SOLVE 8      /* continuum normal equations not positive definite */
+
+const char *pasta_last_error(void);
+int pasta_version(void);
+/* Free device memory in bytes (for admission decisions). */
+int pasta_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* ------------------------------------------------------------------ grid --
+ * Replaces InMemFetcher::new + GridInterpolator::new (src/model_fetchers.rs:136-166,
+ * src/interpolate.rs:267-331,905-910; Python InMemInterpolator, src/lib.rs:1888-1894).
+ * labels: n_models x 3 (Teff, [M/H], logg) of each row of `spectra` (any order);
+ * the grid geometry is built from them like Grid::new and the rows are stored
+ * in HBM in list_gridpoints order (src/interpolate.rs:334-348).
+ * spectra: n_models x n_pix uint16, or NULL to allocate only (see
+ * pasta_grid_device_rows).  factors: per-row f32 max/65535 (includes_factor=True,
+ * src/model_fetchers.rs:53-63) or NULL (1/65535).
+ * The synthetic wavelength grid is WlGrid::Logspace(log10_first, log10_step, n_pix).
+ */
+typedef struct pasta_grid pasta_grid;
+int pasta_grid_create(const uint16_t *spectra, const float *factors, const double *labels,
+                      int n_models, int n_pix, double log10_first, double log10_step,
+                      pasta_grid **out);
+int pasta_grid_destroy(pasta_grid *g);
+/* n_teff, n_m, n_logg, n_models, n_pix */
+int pasta_grid_shape(const pasta_grid *g, int out[5]);
+/* axes (Python teffs/ms/loggs/logg_limits/cumulative_grid_size, src/lib.rs:1436-1457) */
+int pasta_grid_axes(const pasta_grid *g, double *teff, double *m, double *logg, int *logg_lo,
+                    int *logg_hi, int *cum);
+/* Device rows in storage order (list_gridpoints): pointer to row 0 and the row
+ * pitch in elements, so a device-side producer can fill the grid in place. */
+int pasta_grid_device_rows(const pasta_grid *g, uint16_t **rows, int64_t *pitch);
+/* Upload per-row factors after creation (storage order), or NULL to clear. */
+int pasta_grid_set_factors(pasta_grid *g, const float *factors_storage_order);
+/* get_local_grid + calculate_interpolation_coefficients (src/interpolate.rs:424-533,
+ * src/cubic.rs:232-243): storage row of each of the 64 neighbours (-1 absent) and
+ * the f32 weights, in the reference's neighbour order. */
+int pasta_grid_local_weights(const pasta_grid *g, double teff, double m, double logg,
+                             int idx[64], float w[64]);
+/* Grid::clamp_1d (src/interpolate.rs:625-628) */
+int pasta_grid_clamp_1d(const pasta_grid *g, double teff, double m, double logg, int index,
+                        double *out);
+int pasta_grid_is_teff_logg_between_bounds(const pasta_grid *g, double teff, double logg);
+
+/* ------------------------------------------------------------ dispersion --
+ * Replaces NoConvolutionDispersion / FixedResolutionDispersion (log-spaced) /
+ * VariableResolutionDispersion (src/lib.rs:762-797; src/convolve_rv.rs:77-86,
+ * 168-196, 346-402).  wl: observed wavelengths (n).  The synthetic grid is the
+ * grid's WlGrid::Logspace. */
+typedef struct pasta_disp pasta_disp;
+int pasta_disp_create_none(const pasta_grid *g, const double *wl, int n, pasta_disp **out);
+int pasta_disp_create_fixed(const pasta_grid *g, const double *wl, int n, double resolution,
+                            pasta_disp **out);
+int pasta_disp_create_variable(const pasta_grid *g, const double *wl, int n, const float *disp,
+                               pasta_disp **out);
+int pasta_disp_destroy(pasta_disp *d);
+/* kernel length (0 for none) and, if out != NULL, the taps (fixed) or the
+ * K x n_pix kernel matrix, row k = tap k (variable, get_kernels src/lib.rs:1043-1060) */
+int pasta_disp_kernel(const pasta_disp *d, float *out, int64_t cap, int *k_len);
+
+/* ------------------------------------------------------------- continuum --
+ * Replaces ChunkContinuumFitter / LinearModelContinuumFitter / FixedContinuum /
+ * ConstantContinuum (src/lib.rs:944-983; src/continuum_fitting.rs:46-107,127-423,
+ * 427-505).  design: n x ncols row-major. */
+typedef struct pasta_cont pasta_cont;
+int pasta_cont_create_chunk(const double *wl, int n, int n_chunks, int p_order, double overlap,
+                            pasta_cont **out);
+int pasta_cont_create_linear(const float *design, int n, int ncols, pasta_cont **out);
+int pasta_cont_create_fixed(const float *continuum, int n, int ignore_first_and_last,
+                            pasta_cont **out);
+int pasta_cont_create_constant(int n, pasta_cont **out);
+int pasta_cont_destroy(pasta_cont *c);
+int pasta_cont_n_params(const pasta_cont *c);
+/* chunk start/stop indices (n_chunks x 2) */
+int pasta_cont_chunks(const pasta_cont *c, int *startstop, int cap);
+/* build_continuum (src/continuum_fitting.rs:228-278,398-413) on the host */
+int pasta_cont_build(const pasta_cont *c, const float *params, int n_params, float *out,
+                     int64_t cap);
+
+/* ------------------------------------------------------------------ star --
+ * An observed spectrum bound to a grid, dispersion and continuum model, resident
+ * in HBM (ObservedSpectrum + CostFunction fields, src/fitting.rs:22-38,299-305).
+ * flux/var: n (= dispersion and continuum length). */
+typedef struct pasta_star pasta_star;
+int pasta_star_create(pasta_grid *g, const pasta_disp *d, const pasta_cont *c, const float *flux,
+                      const float *var, int n, pasta_star **out);
+int pasta_star_destroy(pasta_star *s);
+/* bytes of HBM one evaluation of this star needs (model buffer + partials) */
+int pasta_star_eval_bytes(const pasta_star *s, int64_t *bytes);
+
+/* ----------------------------------------------------------- evaluation --
+ * Batched CostFunction::cost (src/fitting.rs:313-326) = SingleFitter::chi2 /
+ * Python chi2 (src/lib.rs:1603-1627): labels P x 5 -> chi2 (NaN + status on
+ * failure, the allow_nan behaviour).  params (nullable): P x n_params continuum
+ * parameters.  Returns PASTA_ERR_MODEL if any status is non-zero. */
+int pasta_eval_chi2(pasta_star *s, const double *labels, int P, double *chi2, int *status,
+                    float *params);
+/* produce_model / produce_model_bulk (src/interpolate.rs:1068-1090, src/lib.rs:993-1011,
+ * 1064-1089): labels P x 5 -> out P x n_obs */
+int pasta_produce_model(pasta_grid *g, const pasta_disp *d, const double *labels, int P,
+                        float *out, int *status);
+/* interpolate (src/interpolate.rs:930-983, src/lib.rs:1111-1123): labels P x 3 -> P x n_pix */
+int pasta_interpolate(pasta_grid *g, const double *labels, int P, float *out, int *status);
+/* interpolate_and_convolve (src/interpolate.rs:1030-1047, src/lib.rs:1141-1157):
+ * labels P x 4 -> P x n_pix */
+int pasta_interpolate_and_convolve(pasta_grid *g, const pasta_disp *d, const double *labels,
+                                   int P, float *out, int *status);
+/* get_vsini_kernel (src/lib.rs:1962-1969): returns the tap count in *len */
+int pasta_vsini_kernel(const pasta_grid *g, double vsini, float *out, int cap, int *len);
+
+/* ------------------------------------------------------------------ fit --
+ * SingleFitter::fit (src/fitting.rs:399-458) with the PSO of src/particleswarm.rs
+ * and the bounds of src/bounds.rs.  The reference seeds its RNG from entropy
+ * (src/particleswarm.rs:65); here `seed` selects a counter-based SplitMix64
+ * stream, so a fit is reproducible.  constraint kinds: 0 none, 1 fixed (a),
+ * 2 range [a, b] (FixConstraint / RangeConstraint, src/lib.rs:817-835). */
+typedef struct pasta_fit_settings {
+    int num_particles;
+    int max_iters;
+    double inertia_factor;    /* default 0.7213475204 (src/lib.rs:66) */
+    double cognitive_factor;  /* default 1.1931471806 */
+    double social_factor;     /* default 1.1931471806 */
+    double delta;             /* default 1e-7 (stored, unused, as in the reference) */
+    double vsini_range[2];
+    double rv_range[2];
+    int constraint_kind[5];
+    double constraint_a[5];
+    double constraint_b[5];
+} pasta_fit_settings;
+
+typedef struct pasta_fit_result {
+    double label[5];
+    double chi2;
+    int iterations;
+    int status;               /* PASTA_ST_* of the failure, 0 on success */
+    double time_s;            /* wall time of the fit */
+    int n_params;
+    float continuum_params[512];
+} pasta_fit_result;
+
+/* The plain path: one star, one PSO iteration per host round trip.
+ * init_pos: optional P x 5 starting positions (population injection,
+ * src/particleswarm.rs:186-195).  trace: optional (max_iters+1) x 6 of
+ * (global-best cost, position) per iteration. */
+int pasta_fit(pasta_star *s, const pasta_fit_settings *st, uint64_t seed, const double *init_pos,
+              pasta_fit_result *out, double *trace);
+
+/* The batched path (replaces fit_bulk, src/lib.rs:1511-1540): every particle of
+ * every star in one device-driven, hipGraph-captured PSO iteration. */
+int pasta_fit_bulk(pasta_star *const *stars, int n_stars, const pasta_fit_settings *st,
+                   const uint64_t *seeds, pasta_fit_result *out, double *trace);
+
+/* ---------------------------------------------------------------- server --
+ * Continuous batching of fit requests over a resident grid: requests are
+ * admitted into free slots (and free HBM) between PSO iterations, every
+ * admitted star's particles share each iteration's launches, and a finished
+ * star leaves its slot at once.  Results do not depend on the mix: each star
+ * has its own RNG stream and no reduction crosses stars. */
+typedef struct pasta_server pasta_server;
+typedef struct pasta_server_config {
+    int max_stars;            /* concurrent stars (slots) */
+    int max_particles;        /* particles per star (upper bound) */
+    int iters_per_step;       /* PSO iterations per pasta_server_step */
+    int64_t hbm_budget_bytes; /* 0 = use free memory minus 4 GiB */
+} pasta_server_config;
+int pasta_server_create(pasta_grid *g, const pasta_server_config *cfg, pasta_server **out);
+int pasta_server_destroy(pasta_server *sv);
+/* The server takes ownership of `s` (destroyed when the request is reaped). */
+int pasta_server_submit(pasta_server *sv, pasta_star *s, const pasta_fit_settings *st,
+                        uint64_t seed, uint64_t *request_id);
+/* Admit waiting requests, run iters_per_step iterations, retire finished stars.
+ * *active = stars in flight after the step, *waiting = queued. */
+int pasta_server_step(pasta_server *sv, int *active, int *waiting);
+/* done: 1 finished (result filled), 0 pending; PASTA_ERR_STATE if unknown. */
+int pasta_server_poll(pasta_server *sv, uint64_t request_id, int *done, pasta_fit_result *out);
+int pasta_server_cancel(pasta_server *sv, uint64_t request_id);
+/* counters: [0] evals done, [1] requests finished, [2] iterations run,
+ * [3] HBM bytes in use by in-flight stars */
+int pasta_server_stats(const pasta_server *sv, int64_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

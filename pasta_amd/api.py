@@ -1,0 +1,580 @@
+"""Python mirror of the reference's PyO3 module `pasta` (src/lib.rs) over libpasta_hip.so.
+
+Same names, argument meaning and error behaviour as the reference's Python API
+for the model-evaluation path:
+
+    WlGrid, InMemInterpolator, PSOSettings, Label, OptimizationResult,
+    NoConvolutionDispersion, FixedResolutionDispersion, VariableResolutionDispersion,
+    ChunkContinuumFitter, LinearModelContinuumFitter, FixedContinuum, ConstantContinuum,
+    FixConstraint, RangeConstraint, get_vsini_kernel,
+    InMemInterpolator.{produce_model, produce_model_bulk, interpolate,
+        interpolate_and_convolve, get_fitter, calculate_interpolation_coefficients,
+        get_neighbor_indices, teffs, ms, loggs, logg_limits, cumulative_grid_size,
+        list_gridpoints, is_teff_logg_between_bounds, clamp_1d}
+    InMemSingleFitter.{fit, fit_bulk, chi2}
+
+Every computation runs in the HIP library; there is no CPU fallback.  Errors
+raise ValueError like PyO3's anyhow conversion.
+"""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, f32, f64, lib, ptr
+
+__all__ = [
+    "WlGrid", "InMemInterpolator", "PSOSettings", "Label", "OptimizationResult",
+    "NoConvolutionDispersion", "FixedResolutionDispersion", "VariableResolutionDispersion",
+    "ChunkContinuumFitter", "LinearModelContinuumFitter", "FixedContinuum", "ConstantContinuum",
+    "FixConstraint", "RangeConstraint", "get_vsini_kernel", "InMemSingleFitter",
+]
+
+
+def _raise(rc, what):
+    if rc < 0:
+        raise ValueError(f"{what}: {_lib.last_error()}")
+    return rc
+
+
+class WlGrid:
+    """WlGrid(min, step, len, log=False) (src/lib.rs:630-653).  Only log-spaced
+    synthetic grids are supported by the HIP path (the north-star configs are
+    log-uniform); Linspace grids raise on use."""
+
+    def __init__(self, min, step, len, log=None):
+        self.min, self.step, self.len, self.log = float(min), float(step), int(len), bool(log)
+
+    def get_array(self):
+        i = np.arange(self.len)
+        return 10.0 ** (self.min + self.step * i) if self.log else self.min + self.step * i
+
+
+class PSOSettings:
+    """PSOSettings(num_particles, max_iters, inertia, cognitive, social, delta) (src/lib.rs:42-72)."""
+
+    def __init__(self, num_particles, max_iters, inertia_factor=None, cognitive_factor=None,
+                 social_factor=None, delta=None):
+        self.num_particles = int(num_particles)
+        self.max_iters = int(max_iters)
+        self.inertia_factor = 0.7213475204 if inertia_factor is None else float(inertia_factor)
+        self.cognitive_factor = 1.1931471806 if cognitive_factor is None else float(cognitive_factor)
+        self.social_factor = 1.1931471806 if social_factor is None else float(social_factor)
+        self.delta = 1e-7 if delta is None else float(delta)
+
+
+class Label:
+    """Fitted labels (src/lib.rs:81-166)."""
+
+    def __init__(self, teff, m, logg, vsini, rv):
+        self.teff, self.m, self.logg, self.vsini, self.rv = map(float, (teff, m, logg, vsini, rv))
+
+    def as_list(self):
+        return [self.teff, self.m, self.logg, self.vsini, self.rv]
+
+    def to_dict(self):
+        return dict(teff=self.teff, m=self.m, logg=self.logg, vsini=self.vsini, rv=self.rv)
+
+    @staticmethod
+    def from_dict(d):
+        return Label(d["teff"], d["m"], d["logg"], d["vsini"], d["rv"])
+
+
+class OptimizationResult:
+    """Output of a fit (src/lib.rs:212-294)."""
+
+    def __init__(self, label, continuum_params, chi2, iterations, time):
+        self.label, self.continuum_params = label, list(continuum_params)
+        self.chi2, self.iterations, self.time = float(chi2), int(iterations), float(time)
+
+    def to_dict(self):
+        return dict(label=self.label.to_dict(), continuum_params=self.continuum_params,
+                    chi2=self.chi2, iterations=self.iterations, time=self.time)
+
+    def to_json(self):
+        return json.dumps(self.to_dict())
+
+    @staticmethod
+    def from_dict(d):
+        return OptimizationResult(Label.from_dict(d["label"]), d["continuum_params"], d["chi2"],
+                                  d["iterations"], d["time"])
+
+    @staticmethod
+    def _from_c(r):
+        lab = Label(*r.label[:5])
+        return OptimizationResult(lab, r.continuum_params[: r.n_params], r.chi2, r.iterations,
+                                  r.time_s)
+
+
+class _Constraint:
+    def __init__(self, parameter, kind, a, b):
+        self.parameter, self.kind, self.a, self.b = int(parameter), kind, float(a), float(b)
+
+
+def FixConstraint(parameter, value):
+    """Fix one parameter during the fit (src/lib.rs:817-823)."""
+    return _Constraint(parameter, 1, value, value)
+
+
+def RangeConstraint(parameter, lower, upper):
+    """Bound one parameter during the fit (src/lib.rs:829-835)."""
+    return _Constraint(parameter, 2, lower, upper)
+
+
+# ---------------------------------------------------------------- dispersion
+class PyWavelengthDispersion:
+    """Deferred dispersion description, bound to a grid's synthetic sampling on use
+    (the reference binds it to `synth_wl` at construction, src/lib.rs:762-797)."""
+
+    def __init__(self, kind, wl, synth_wl, resolution=None, disp=None):
+        self.kind, self.wl, self.synth_wl = kind, f64(wl), synth_wl
+        self.resolution, self.disp = resolution, None if disp is None else f32(disp)
+        self._handles = {}
+
+    def wavelength(self):
+        return self.wl.tolist()
+
+    def handle(self, grid):
+        key = id(grid)
+        if key not in self._handles:
+            if not self.synth_wl.log:
+                raise ValueError("only log-spaced synthetic grids are supported")
+            h = C.c_void_p()
+            n = len(self.wl)
+            if self.kind == "none":
+                rc = lib.pasta_disp_create_none(grid._h, ptr(self.wl, C.c_double), n, C.byref(h))
+            elif self.kind == "fixed":
+                rc = lib.pasta_disp_create_fixed(grid._h, ptr(self.wl, C.c_double), n,
+                                                 float(self.resolution), C.byref(h))
+            else:
+                rc = lib.pasta_disp_create_variable(grid._h, ptr(self.wl, C.c_double), n,
+                                                    ptr(self.disp, C.c_float), C.byref(h))
+            _raise(rc, "dispersion")
+            self._handles[key] = (grid, h)
+        return self._handles[key][1]
+
+    def __del__(self):
+        for _, h in getattr(self, "_handles", {}).values():
+            lib.pasta_disp_destroy(h)
+
+
+def NoConvolutionDispersion(wl, wl_grid):
+    return PyWavelengthDispersion("none", wl, wl_grid)
+
+
+def FixedResolutionDispersion(wl, resolution, synth_wl):
+    if resolution <= 0:
+        raise ValueError("Resolution must be positive")
+    return PyWavelengthDispersion("fixed", wl, synth_wl, resolution=resolution)
+
+
+def VariableResolutionDispersion(wl, disp, synth_wl):
+    return PyWavelengthDispersion("variable", wl, synth_wl, disp=disp)
+
+
+# ----------------------------------------------------------------- continuum
+class PyContinuumFitter:
+    def __init__(self, kind, **kw):
+        self.kind = kind
+        h = C.c_void_p()
+        if kind == "chunk":
+            wl = f64(kw["wl"])
+            self.n = len(wl)
+            rc = lib.pasta_cont_create_chunk(ptr(wl, C.c_double), len(wl), kw["n_chunks"],
+                                             kw["p_order"], kw["overlap"], C.byref(h))
+        elif kind == "linear":
+            dm = f32(kw["design_matrix"])
+            self.n = dm.shape[0]
+            rc = lib.pasta_cont_create_linear(ptr(dm, C.c_float), dm.shape[0], dm.shape[1],
+                                              C.byref(h))
+        elif kind == "fixed":
+            cont = f32(kw["continuum"])
+            self.n = len(cont)
+            rc = lib.pasta_cont_create_fixed(ptr(cont, C.c_float), len(cont), kw["ignore"],
+                                             C.byref(h))
+        else:
+            self.n = None
+            rc = 0
+            h = None
+        _raise(rc, "continuum")
+        self._h = h
+        self._const = {}
+
+    def handle(self, n):
+        if self.kind != "constant":
+            if n != self.n:
+                raise ValueError("Length of observed and synthetic spectrum are different")
+            return self._h
+        if n not in self._const:
+            h = C.c_void_p()
+            _raise(lib.pasta_cont_create_constant(n, C.byref(h)), "continuum")
+            self._const[n] = h
+        return self._const[n]
+
+    def n_params(self, n):
+        return lib.pasta_cont_n_params(self.handle(n))
+
+    def build_continuum(self, params):
+        params = f32(params)
+        n = self.n or 1
+        out = np.zeros(max(n, 1), np.float32)
+        k = _raise(lib.pasta_cont_build(self.handle(n), ptr(params, C.c_float), len(params),
+                                        ptr(out, C.c_float), len(out)), "build_continuum")
+        return out[:k]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.pasta_cont_destroy(self._h)
+        for h in getattr(self, "_const", {}).values():
+            lib.pasta_cont_destroy(h)
+
+
+def ChunkContinuumFitter(wl, n_chunks=None, p_order=None, overlap=None):
+    """Chunked polynomials with sin^2 blending (src/lib.rs:944-960)."""
+    return PyContinuumFitter("chunk", wl=wl, n_chunks=5 if n_chunks is None else n_chunks,
+                             p_order=8 if p_order is None else p_order,
+                             overlap=0.2 if overlap is None else overlap)
+
+
+def LinearModelContinuumFitter(design_matrix):
+    return PyContinuumFitter("linear", design_matrix=design_matrix)
+
+
+def FixedContinuum(continuum, ignore_first_and_last=None):
+    return PyContinuumFitter("fixed", continuum=continuum,
+                             ignore=0 if ignore_first_and_last is None else ignore_first_and_last)
+
+
+def ConstantContinuum():
+    return PyContinuumFitter("constant")
+
+
+# ---------------------------------------------------------------- interpolator
+_FNAME = re.compile(r"^l([mp])(\d{4})_(\d{5})_(\d{4})\.npy$")
+
+
+def _read_npy_u16(path):
+    a = np.load(path, allow_pickle=False)
+    if a.dtype != np.uint16:
+        raise ValueError(f"{path}: expected uint16")
+    return a.ravel()
+
+
+class InMemInterpolator:
+    """Grid resident in HBM (InMemInterpolator, src/lib.rs:1881-1908).
+
+    InMemInterpolator(dir, includes_factor, wavelength) reads `l{m|p}MMMM_TTTTT_GGGG.npy`
+    files like InMemFetcher::new (src/model_fetchers.rs:23-38,48-67,136-166);
+    InMemInterpolator.from_arrays(spectra, labels, wavelength, factors) takes arrays.
+    """
+
+    def __init__(self, dir=None, includes_factor=False, wavelength=None, *, _arrays=None):
+        if _arrays is None:
+            labels, spectra, factors = [], [], []
+            for fn in sorted(os.listdir(dir)):
+                mt = _FNAME.match(fn)
+                if not mt:
+                    raise ValueError(f"Invalid filename: {fn}")
+                sign = -1.0 if mt.group(1) == "m" else 1.0
+                labels.append((float(mt.group(3)), sign * float(mt.group(2)) / 100.0,
+                               float(mt.group(4)) / 100.0))
+                a = _read_npy_u16(os.path.join(dir, fn))
+                if includes_factor:
+                    fac = np.frombuffer(a[:2].tobytes(), dtype="<f4")[0]
+                    if fac < 0:
+                        raise ValueError(f"Negative factor: {fac}")
+                    factors.append(np.float32(fac) / np.float32(65535.0))
+                    a = a[2:]
+                spectra.append(a)
+            spectra = np.stack(spectra)
+            labels = np.array(labels)
+            factors = np.array(factors, np.float32) if includes_factor else None
+        else:
+            spectra, labels, factors = _arrays
+        if not wavelength.log:
+            raise ValueError("only log-spaced synthetic grids are supported")
+        spectra = np.ascontiguousarray(spectra, dtype=np.uint16)
+        labels = f64(labels).reshape(-1, 3)
+        if spectra.shape[1] != wavelength.len:
+            raise ValueError("Length of model and wavelength grid don't match")
+        self.wavelength = wavelength
+        h = C.c_void_p()
+        fac = None if factors is None else f32(factors)
+        rc = lib.pasta_grid_create(ptr(spectra, C.c_uint16),
+                                   None if fac is None else ptr(fac, C.c_float),
+                                   ptr(labels, C.c_double), spectra.shape[0], spectra.shape[1],
+                                   wavelength.min, wavelength.step, C.byref(h))
+        _raise(rc, "grid")
+        self._h = h
+        shape = (C.c_int * 5)()
+        lib.pasta_grid_shape(self._h, shape)
+        self.n_teff, self.n_m, self.n_logg, self.n_models, self.n_pix = list(shape)
+
+    @staticmethod
+    def from_arrays(spectra, labels, wavelength, factors=None):
+        return InMemInterpolator(wavelength=wavelength, _arrays=(spectra, labels, factors))
+
+    @staticmethod
+    def empty(labels, wavelength):
+        """Allocate the grid in HBM without data (filled in place by a device producer)."""
+        obj = InMemInterpolator.__new__(InMemInterpolator)
+        labels = f64(labels).reshape(-1, 3)
+        h = C.c_void_p()
+        _raise(lib.pasta_grid_create(None, None, ptr(labels, C.c_double), len(labels),
+                                     wavelength.len, wavelength.min, wavelength.step, C.byref(h)),
+               "grid")
+        obj._h, obj.wavelength = h, wavelength
+        shape = (C.c_int * 5)()
+        lib.pasta_grid_shape(h, shape)
+        obj.n_teff, obj.n_m, obj.n_logg, obj.n_models, obj.n_pix = list(shape)
+        return obj
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.pasta_grid_destroy(self._h)
+            self._h = None
+
+    # -- geometry accessors (src/lib.rs:1435-1465)
+    def _axes(self):
+        t = np.zeros(self.n_teff); m = np.zeros(self.n_m); g = np.zeros(self.n_logg)
+        lo = np.zeros(self.n_teff, np.int32); hi = np.zeros(self.n_teff, np.int32)
+        cum = np.zeros(self.n_teff, np.int32)
+        lib.pasta_grid_axes(self._h, ptr(t, C.c_double), ptr(m, C.c_double), ptr(g, C.c_double),
+                            ptr(lo, C.c_int), ptr(hi, C.c_int), ptr(cum, C.c_int))
+        return t, m, g, lo, hi, cum
+
+    def teffs(self):
+        return self._axes()[0].tolist()
+
+    def ms(self):
+        return self._axes()[1].tolist()
+
+    def loggs(self):
+        return self._axes()[2].tolist()
+
+    def logg_limits(self):
+        _, _, _, lo, hi, _ = self._axes()
+        return list(zip(lo.tolist(), hi.tolist()))
+
+    def cumulative_grid_size(self):
+        return self._axes()[5].tolist()
+
+    def list_gridpoints(self):
+        t, m, g, lo, hi, _ = self._axes()
+        return [[t[i], mm, g[k]] for i in range(len(t)) for k in range(lo[i], hi[i] + 1) for mm in m]
+
+    def is_teff_logg_between_bounds(self, teff, logg):
+        return bool(lib.pasta_grid_is_teff_logg_between_bounds(self._h, teff, logg))
+
+    def clamp_1d(self, teff, m, logg, index):
+        out = C.c_double()
+        _raise(lib.pasta_grid_clamp_1d(self._h, teff, m, logg, index, C.byref(out)), "clamp_1d")
+        return out.value
+
+    def _weights(self, teff, m, logg):
+        idx = np.zeros(64, np.int32)
+        w = np.zeros(64, np.float32)
+        _raise(lib.pasta_grid_local_weights(self._h, teff, m, logg, ptr(idx, C.c_int),
+                                            ptr(w, C.c_float)), "local grid")
+        return idx, w
+
+    def calculate_interpolation_coefficients(self, teff, m, logg):
+        return self._weights(teff, m, logg)[1].tolist()
+
+    def get_neighbor_indices(self, teff, m, logg):
+        """(teff index, m index, logg index) per neighbour, (0,0,0) if absent (src/lib.rs:1414-1433)."""
+        idx, _ = self._weights(teff, m, logg)
+        t, mm, g, lo, hi, cum = self._axes()
+        out = []
+        for r in idx:
+            if r < 0:
+                out.append((0, 0, 0))
+                continue
+            j = r % len(mm)
+            pair = r // len(mm)
+            i = int(np.searchsorted(cum, pair, side="right") - 1)
+            out.append((i, j, int(pair - cum[i] + lo[i])))
+        return out
+
+    # -- model production
+    def interpolate(self, teff, m, logg):
+        lab = f64([[teff, m, logg]])
+        out = np.zeros(self.n_pix, np.float32)
+        _raise(lib.pasta_interpolate(self._h, ptr(lab, C.c_double), 1, ptr(out, C.c_float), None),
+               "Error while interpolating")
+        return out
+
+    def interpolate_bulk(self, labels):
+        lab = f64(labels).reshape(-1, 3)
+        out = np.zeros((len(lab), self.n_pix), np.float32)
+        st = np.zeros(len(lab), np.int32)
+        lib.pasta_interpolate(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_float),
+                              ptr(st, C.c_int))
+        return out, st
+
+    def interpolate_and_convolve(self, dispersion, teff, m, logg, vsini):
+        lab = f64([[teff, m, logg, vsini]])
+        out = np.zeros(self.n_pix, np.float32)
+        _raise(lib.pasta_interpolate_and_convolve(self._h, dispersion.handle(self),
+                                                  ptr(lab, C.c_double), 1, ptr(out, C.c_float),
+                                                  None), "interpolate_and_convolve")
+        return out
+
+    def produce_model(self, dispersion, teff, m, logg, vsini, rv):
+        out, st = self._produce(dispersion, [[teff, m, logg, vsini, rv]])
+        if st[0]:
+            raise ValueError(f"produce_model: {_lib.STATUS_TEXT.get(int(st[0]))}")
+        return out[0]
+
+    def produce_model_bulk(self, dispersion, labels, progress=None):
+        out, st = self._produce(dispersion, labels)
+        if st.any():
+            raise ValueError(f"produce_model: {_lib.STATUS_TEXT.get(int(st[st != 0][0]))}")
+        return out
+
+    def _produce(self, dispersion, labels):
+        lab = f64(labels).reshape(-1, 5)
+        n = len(dispersion.wl)
+        out = np.zeros((len(lab), n), np.float32)
+        st = np.zeros(len(lab), np.int32)
+        lib.pasta_produce_model(self._h, dispersion.handle(self), ptr(lab, C.c_double), len(lab),
+                                ptr(out, C.c_float), ptr(st, C.c_int))
+        return out, st
+
+    def get_fitter(self, dispersion, continuum_fitter, settings, vsini_range, rv_range):
+        return InMemSingleFitter(self, dispersion, continuum_fitter, settings, vsini_range, rv_range)
+
+
+def get_vsini_kernel(vsini, synth_wl):
+    """Rotation kernel (src/lib.rs:1962-1969)."""
+    if not synth_wl.log:
+        raise ValueError("Only logspace is supported")
+    tmp = InMemInterpolator.__new__(InMemInterpolator)
+    # the kernel depends only on the log step: compute it with a 2-pixel dummy grid
+    lab = f64([[1.0, 0.0, 0.0]])
+    h = C.c_void_p()
+    spectra = np.zeros((1, 2), np.uint16)
+    _raise(lib.pasta_grid_create(ptr(spectra, C.c_uint16), None, ptr(lab, C.c_double), 1, 2,
+                                 synth_wl.min, synth_wl.step, C.byref(h)), "grid")
+    try:
+        n = C.c_int()
+        lib.pasta_vsini_kernel(h, vsini, None, 0, C.byref(n))
+        out = np.zeros(max(n.value, 1), np.float32)
+        _raise(lib.pasta_vsini_kernel(h, vsini, ptr(out, C.c_float), len(out), C.byref(n)), "kernel")
+        return out[: n.value].tolist()
+    finally:
+        lib.pasta_grid_destroy(h)
+        del tmp
+
+
+def _settings_c(settings, vsini_range, rv_range, constraints):
+    s = _lib.FitSettings()
+    s.num_particles = settings.num_particles
+    s.max_iters = settings.max_iters
+    s.inertia_factor = settings.inertia_factor
+    s.cognitive_factor = settings.cognitive_factor
+    s.social_factor = settings.social_factor
+    s.delta = settings.delta
+    s.vsini_range[0], s.vsini_range[1] = vsini_range
+    s.rv_range[0], s.rv_range[1] = rv_range
+    for c in constraints or []:
+        s.constraint_kind[c.parameter] = c.kind
+        s.constraint_a[c.parameter] = c.a
+        s.constraint_b[c.parameter] = c.b
+    return s
+
+
+class Star:
+    """An observed spectrum resident in HBM, bound to a grid/dispersion/continuum."""
+
+    def __init__(self, interpolator, dispersion, continuum, flux, var, _own=True):
+        flux, var = f32(flux), f32(var)
+        if len(flux) != len(var):
+            raise ValueError("Flux and variance vectors must have the same length")
+        h = C.c_void_p()
+        _raise(lib.pasta_star_create(interpolator._h, dispersion.handle(interpolator),
+                                     continuum.handle(len(flux)), ptr(flux, C.c_float),
+                                     ptr(var, C.c_float), len(flux), C.byref(h)), "star")
+        self._h, self._own, self.n = h, _own, len(flux)
+        self.n_params = continuum.n_params(len(flux))
+        self._refs = (interpolator, dispersion, continuum)
+
+    def release(self):
+        """Hand ownership to a server (the server destroys it)."""
+        self._own = False
+        return self._h
+
+    def __del__(self):
+        if getattr(self, "_own", False) and self._h:
+            lib.pasta_star_destroy(self._h)
+            self._h = None
+
+    def chi2(self, labels):
+        lab = f64(labels).reshape(-1, 5)
+        out = np.zeros(len(lab))
+        st = np.zeros(len(lab), np.int32)
+        params = np.zeros((len(lab), max(1, self.n_params)), np.float32)
+        lib.pasta_eval_chi2(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_double),
+                            ptr(st, C.c_int), ptr(params, C.c_float))
+        return out, st, params
+
+
+class InMemSingleFitter:
+    """SingleFitter for the resident grid (src/lib.rs:1468-1628)."""
+
+    def __init__(self, interpolator, dispersion, continuum_fitter, settings, vsini_range, rv_range):
+        self.interpolator, self.dispersion = interpolator, dispersion
+        self.continuum_fitter, self.settings = continuum_fitter, settings
+        self.vsini_range, self.rv_range = tuple(vsini_range), tuple(rv_range)
+
+    def _star(self, interpolator, flux, var):
+        if len(flux) != len(self.dispersion.wl):
+            raise ValueError("Observed spectrum and target dispersion must have the same length")
+        return Star(interpolator, self.dispersion, self.continuum_fitter, flux, var)
+
+    def fit(self, interpolator, observed_flux, observed_var, trace_directory=None,
+            best_particle_file=None, parallelize=None, constraints=None, seed=0, init_pos=None):
+        """PSO fit of one star (src/lib.rs:1478-1507).  `seed` selects the random
+        stream (the reference draws it from entropy); `parallelize` is accepted
+        for compatibility (all particles always run on the GPU)."""
+        if trace_directory is not None and best_particle_file is not None:
+            raise ValueError("Cannot specify both trace_directory and best_particle_file")
+        star = self._star(interpolator, observed_flux, observed_var)
+        s = _settings_c(self.settings, self.vsini_range, self.rv_range, constraints)
+        res = _lib.FitResult()
+        trace = np.zeros((self.settings.max_iters + 1, 6))
+        ip = None if init_pos is None else ptr(f64(init_pos), C.c_double)
+        keep = None if init_pos is None else f64(init_pos)
+        if keep is not None:
+            ip = ptr(keep, C.c_double)
+        rc = lib.pasta_fit(star._h, C.byref(s), int(seed), ip, C.byref(res), ptr(trace, C.c_double))
+        _raise(rc, "fit")
+        out = OptimizationResult._from_c(res)
+        if best_particle_file is not None:
+            with open(best_particle_file, "w") as f:
+                json.dump([{"position": t[1:].tolist(), "cost": t[0]} for t in trace], f)
+        out._trace = trace
+        return out
+
+    def fit_bulk(self, interpolator, observed_fluxes, observed_vars, constraints=None, seeds=None):
+        """Many stars in one device-driven batched PSO (src/lib.rs:1511-1540)."""
+        stars = [self._star(interpolator, f, v) for f, v in zip(observed_fluxes, observed_vars)]
+        n = len(stars)
+        s = _settings_c(self.settings, self.vsini_range, self.rv_range, constraints)
+        seeds = np.arange(n, dtype=np.uint64) if seeds is None else np.asarray(seeds, np.uint64)
+        arr = (C.c_void_p * n)(*[st._h for st in stars])
+        res = (_lib.FitResult * n)()
+        rc = lib.pasta_fit_bulk(arr, n, C.byref(s), ptr(seeds, C.c_uint64), res, None)
+        _raise(rc, "fit_bulk")
+        return [OptimizationResult._from_c(r) for r in res]
+
+    def chi2(self, interpolator, observed_flux, observed_var, labels, allow_nan=None):
+        """chi2 at each label vector (src/lib.rs:1603-1627): NaN on failure unless allow_nan=False."""
+        star = self._star(interpolator, observed_flux, observed_var)
+        out, st, _ = star.chi2(labels)
+        if allow_nan is False and st.any():
+            raise ValueError(_lib.STATUS_TEXT.get(int(st[st != 0][0])))
+        return out.tolist()

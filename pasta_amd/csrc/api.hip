@@ -1,0 +1,1621 @@
+// api.hip -- host driver behind the C ABI in include/pasta_hip.h.
+//
+// Host-side set-up restated from the reference (SimonVanSchuylenbergh/PASTA):
+//   Grid::new / list_gridpoints              src/interpolate.rs:267-348
+//   WlGrid::get_float_index_of_wl (Logspace)  src/interpolate.rs:705-708
+//   FixedTargetDispersionLogSpace::new        src/convolve_rv.rs:169-195
+//   VariableTargetDispersion::new             src/convolve_rv.rs:303-331,350-402
+//   ChunkFitter::new                          src/continuum_fitting.rs:140-214
+//   PSO init / loop / best update             src/particleswarm.rs:111-150,180-307
+//   SingleFitter::fit                         src/fitting.rs:399-458
+// The least-squares solve of each chunk (an f32 SVD in the reference,
+// src/continuum_fitting.rs:181,223) is done as FP64 normal equations: the Gram
+// matrix AᵀA of the f32 design values is fixed per star, so its Cholesky factor
+// is computed here once and the device only forms Aᵀy per evaluation.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "../../include/pasta_hip.h"
+
+namespace pasta {
+hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
+                       const double* labels, float* rot, const int2* mblk_model, int n_model,
+                       const int2* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
+                       const int2* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
+                       double* cpart, double* chi2, int* status, int pmax, int mode,
+                       hipStream_t st);
+hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, int n, double* pos,
+                             double* vel, const double* bpos, hipStream_t st);
+hipError_t launch_pso_best(DevPso* pso, int n_stars, const DevEval* evals, const double* pos,
+                           double* bpos, double* bcost, const double* cost, double* trace,
+                           hipStream_t st);
+hipError_t launch_pso_init(DevPso* pso, int n_stars, const DevEval* evals, double* pos,
+                           double* vel, double* bpos, double* bcost, double* cost, double* trace,
+                           hipStream_t st);
+}  // namespace pasta
+
+using namespace pasta;
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? PASTA_ERR_NOMEM : PASTA_ERR_HIP,      \
+                        std::string(#x) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+#define CHK(x)                  \
+    do {                        \
+        int r_ = (x);           \
+        if (r_) return r_;      \
+    } while (0)
+
+static const double LN_10 = 2.302585092994045684017991454684364208;
+static const double C_KMS = 299792.0;
+
+// ---------------------------------------------------------------------------
+// device buffer
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        if (count <= n && p) return PASTA_OK;
+        release();
+        size_t c = count ? count : 1;
+        HIPCHK(hipMalloc((void**)&p, c * sizeof(T)));
+        n = c;
+        return PASTA_OK;
+    }
+    int upload(const T* h, size_t count) {
+        CHK(alloc(count));
+        if (count) HIPCHK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+        return PASTA_OK;
+    }
+    int upload(const std::vector<T>& v) { return upload(v.data(), v.size()); }
+};
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct pasta_grid {
+    GridGeom geom{};
+    DBuf<GridGeom> d_geom;
+    DBuf<uint16_t> d_spectra;
+    DBuf<float> d_factors;
+    bool has_factors = false;
+    long long pitch = 0;
+    int W = 0;
+    double first = 0, step = 0, first_wl = 0, last_wl = 0;
+    std::mutex mu;
+    DevGrid dev() const {
+        DevGrid g;
+        g.spectra = d_spectra.p;
+        g.factors = has_factors ? d_factors.p : nullptr;
+        g.geom = d_geom.p;
+        g.pitch = pitch;
+        g.W = W;
+        g.Wq = (W / 128) * 128;  // BATCH_SIZE (src/interpolate.rs:917,969)
+        g.sg_first = first;
+        g.sg_step = step;
+        g.synth_first_wl = first_wl;
+        g.synth_last_wl = last_wl;
+        return g;
+    }
+};
+
+struct pasta_disp {
+    int kind = DISP_NONE;
+    std::vector<double> wl, fidx;
+    std::vector<float> kern;  // fixed: K; variable: W x K
+    int K = 0;
+    int W = 0;
+    double first = 0, step = 0;
+};
+
+struct pasta_cont {
+    int kind = CONT_CONSTANT;
+    int n = 0;
+    int n_chunks = 0, p1 = 1, ignore = 0, n_params = 1;
+    std::vector<int> startstop;
+    std::vector<double> lohi, chol, wl;
+    std::vector<float> fac, design_cm, design_rm, fixedc;
+    std::vector<int4> mblocks, cblocks;
+    std::vector<int> mb_off;
+    double chi2_div = 1.0;
+};
+
+struct Batch;
+
+struct pasta_star {
+    pasta_grid* grid = nullptr;
+    int N = 0;
+    int n_params = 1, p1 = 1, cont_kind = CONT_CONSTANT, n_chunks = 0;
+    DevStar dev{};
+    std::vector<void*> allocs;
+    std::unique_ptr<Batch> cache;
+    int cache_P = -1;
+    std::mutex mu;
+    ~pasta_star();
+};
+
+const char* pasta_last_error(void) { return g_err.c_str(); }
+int pasta_version(void) { return 100; }
+
+int pasta_device_mem_info(uint64_t* free_b, uint64_t* total_b) {
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_b) *free_b = f;
+    if (total_b) *total_b = t;
+    return PASTA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// grid
+// ---------------------------------------------------------------------------
+static int insert_unique_sorted(std::vector<double>& v, double x) {
+    auto it = std::lower_bound(v.begin(), v.end(), x);
+    if (it != v.end() && *it == x) return 0;
+    v.insert(it, x);
+    return 1;
+}
+
+int pasta_grid_create(const uint16_t* spectra, const float* factors, const double* labels,
+                      int n_models, int n_pix, double log10_first, double log10_step,
+                      pasta_grid** out) {
+    if (!labels || n_models <= 0 || n_pix <= 1 || !out || !(log10_step > 0))
+        return fail(PASTA_ERR_ARG, "pasta_grid_create: bad arguments");
+    std::vector<double> t, m, l;
+    for (int i = 0; i < n_models; i++) {
+        insert_unique_sorted(t, labels[3 * i]);
+        insert_unique_sorted(m, labels[3 * i + 1]);
+        insert_unique_sorted(l, labels[3 * i + 2]);
+    }
+    if ((int)t.size() > PASTA_MAX_AXIS || (int)m.size() > PASTA_MAX_AXIS ||
+        (int)l.size() > PASTA_MAX_AXIS)
+        return fail(PASTA_ERR_ARG, "grid axis longer than 128 values");
+    auto g = std::make_unique<pasta_grid>();
+    GridGeom& G = g->geom;
+    G.n_teff = (int)t.size(); G.n_m = (int)m.size(); G.n_logg = (int)l.size();
+    std::copy(t.begin(), t.end(), G.teff);
+    std::copy(m.begin(), m.end(), G.m);
+    std::copy(l.begin(), l.end(), G.logg);
+    for (int i = 0; i < G.n_teff; i++) { G.logg_lo[i] = G.n_logg - 1; G.logg_hi[i] = 0; }
+    for (int q = 0; q < n_models; q++) {
+        int i = right_index(G.teff, G.n_teff, labels[3 * q]);
+        int k = right_index(G.logg, G.n_logg, labels[3 * q + 2]);
+        G.logg_lo[i] = std::min(G.logg_lo[i], k);
+        G.logg_hi[i] = std::max(G.logg_hi[i], k);
+    }
+    int acc = 0;
+    for (int i = 0; i < G.n_teff; i++) {
+        if (G.logg_lo[i] > G.logg_hi[i]) return fail(PASTA_ERR_ARG, "Invalid logg limits");
+        G.cum[i] = acc;
+        acc += G.logg_hi[i] - G.logg_lo[i] + 1;
+    }
+    G.n_models = acc * G.n_m;
+    if (G.n_models != n_models)
+        return fail(PASTA_ERR_ARG, "grid labels do not form a staircase grid (" +
+                                       std::to_string(n_models) + " rows, " +
+                                       std::to_string(G.n_models) + " grid points)");
+    // storage row of each input row; every grid point exactly once
+    std::vector<int> row_of(n_models), seen(n_models, 0);
+    for (int q = 0; q < n_models; q++) {
+        int i = right_index(G.teff, G.n_teff, labels[3 * q]);
+        int j = right_index(G.m, G.n_m, labels[3 * q + 1]);
+        int k = right_index(G.logg, G.n_logg, labels[3 * q + 2]);
+        int r = model_index(G, i, j, k);
+        if (r < 0 || r >= n_models || seen[r]) return fail(PASTA_ERR_ARG, "duplicate grid label");
+        seen[r] = 1;
+        row_of[q] = r;
+    }
+    g->W = n_pix;
+    g->pitch = ((long long)n_pix + 7) & ~7LL;
+    g->first = log10_first;
+    g->step = log10_step;
+    // WlGrid::get_first_and_last (src/interpolate.rs:695-698): one step past the last pixel
+    g->first_wl = std::pow(10.0, log10_first);
+    g->last_wl = std::pow(10.0, log10_first + log10_step * (double)n_pix);
+    CHK(g->d_geom.upload(&G, 1));
+    CHK(g->d_spectra.alloc((size_t)n_models * g->pitch));
+    HIPCHK(hipMemset(g->d_spectra.p, 0, (size_t)n_models * g->pitch * sizeof(uint16_t)));
+    if (spectra) {
+        for (int q = 0; q < n_models; q++)
+            HIPCHK(hipMemcpy(g->d_spectra.p + (size_t)row_of[q] * g->pitch,
+                             spectra + (size_t)q * n_pix, (size_t)n_pix * sizeof(uint16_t),
+                             hipMemcpyHostToDevice));
+    }
+    if (factors) {
+        std::vector<float> f(n_models);
+        for (int q = 0; q < n_models; q++) f[row_of[q]] = factors[q];
+        CHK(g->d_factors.upload(f));
+        g->has_factors = true;
+    }
+    HIPCHK(hipDeviceSynchronize());
+    *out = g.release();
+    return PASTA_OK;
+}
+
+int pasta_grid_destroy(pasta_grid* g) {
+    delete g;
+    return PASTA_OK;
+}
+
+int pasta_grid_shape(const pasta_grid* g, int out[5]) {
+    if (!g) return fail(PASTA_ERR_ARG, "null grid");
+    out[0] = g->geom.n_teff; out[1] = g->geom.n_m; out[2] = g->geom.n_logg;
+    out[3] = g->geom.n_models; out[4] = g->W;
+    return PASTA_OK;
+}
+
+int pasta_grid_axes(const pasta_grid* g, double* teff, double* m, double* logg, int* lo, int* hi,
+                    int* cum) {
+    if (!g) return fail(PASTA_ERR_ARG, "null grid");
+    const GridGeom& G = g->geom;
+    for (int i = 0; i < G.n_teff; i++) {
+        if (teff) teff[i] = G.teff[i];
+        if (lo) lo[i] = G.logg_lo[i];
+        if (hi) hi[i] = G.logg_hi[i];
+        if (cum) cum[i] = G.cum[i];
+    }
+    for (int i = 0; i < G.n_m; i++) if (m) m[i] = G.m[i];
+    for (int i = 0; i < G.n_logg; i++) if (logg) logg[i] = G.logg[i];
+    return PASTA_OK;
+}
+
+int pasta_grid_device_rows(const pasta_grid* g, uint16_t** rows, int64_t* pitch) {
+    if (!g) return fail(PASTA_ERR_ARG, "null grid");
+    *rows = g->d_spectra.p;
+    *pitch = g->pitch;
+    return PASTA_OK;
+}
+
+int pasta_grid_set_factors(pasta_grid* g, const float* f) {
+    if (!g) return fail(PASTA_ERR_ARG, "null grid");
+    if (!f) { g->has_factors = false; return PASTA_OK; }
+    CHK(g->d_factors.upload(f, g->geom.n_models));
+    g->has_factors = true;
+    return PASTA_OK;
+}
+
+int pasta_grid_local_weights(const pasta_grid* g, double teff, double m, double logg, int idx[64],
+                             float w[64]) {
+    if (!g) return fail(PASTA_ERR_ARG, "null grid");
+    int rc = local_weights(g->geom, teff, m, logg, idx, w);
+    if (rc) return fail(PASTA_ERR_MODEL, "labels outside the grid / not enough neighbours");
+    return PASTA_OK;
+}
+
+int pasta_grid_clamp_1d(const pasta_grid* g, double teff, double m, double logg, int index,
+                        double* out) {
+    if (!g || index < 0 || index > 2) return fail(PASTA_ERR_ARG, "bad clamp_1d arguments");
+    Bounds b{};
+    b.vsini_lo = 0; b.vsini_hi = 0; b.rv_lo = 0; b.rv_hi = 0;
+    double p[5] = {teff, m, logg, 0, 0};
+    if (bounds_clamp(g->geom, b, p, index, out)) return fail(PASTA_ERR_MODEL, "Cannot clamp");
+    return PASTA_OK;
+}
+
+int pasta_grid_is_teff_logg_between_bounds(const pasta_grid* g, double teff, double logg) {
+    return g && teff_logg_between_bounds(g->geom, teff, logg) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// dispersion
+// ---------------------------------------------------------------------------
+// make_gaussian (src/convolve_rv.rs:107-118)
+static void make_gaussian(float* arr, int n, float sigma) {
+    int mid = (n - 1) / 2;
+    for (int i = 0; i < n; i++) {
+        float x = ((float)i - (float)mid) / sigma;
+        arr[i] = expf(-0.5f * x * x);
+    }
+    float s = 0.0f;
+    for (int i = 0; i < n; i++) s = s + arr[i];
+    for (int i = 0; i < n; i++) arr[i] = arr[i] / s;
+}
+
+static int disp_common(const pasta_grid* g, const double* wl, int n, pasta_disp* d) {
+    if (!g || !wl || n < 2) return fail(PASTA_ERR_ARG, "dispersion: bad arguments");
+    d->wl.assign(wl, wl + n);
+    d->fidx.resize(n);
+    for (int i = 0; i < n; i++) d->fidx[i] = (std::log10(wl[i]) - g->first) / g->step;
+    for (int i = 1; i < n; i++)
+        if (!(d->fidx[i] >= d->fidx[i - 1]))
+            return fail(PASTA_ERR_ARG, "observed wavelengths must be sorted ascending");
+    d->W = g->W;
+    d->first = g->first;
+    d->step = g->step;
+    return PASTA_OK;
+}
+
+int pasta_disp_create_none(const pasta_grid* g, const double* wl, int n, pasta_disp** out) {
+    auto d = std::make_unique<pasta_disp>();
+    CHK(disp_common(g, wl, n, d.get()));
+    d->kind = DISP_NONE;
+    *out = d.release();
+    return PASTA_OK;
+}
+
+int pasta_disp_create_fixed(const pasta_grid* g, const double* wl, int n, double resolution,
+                            pasta_disp** out) {
+    if (!(resolution > 0.0)) return fail(PASTA_ERR_ARG, "Resolution must be positive");
+    auto d = std::make_unique<pasta_disp>();
+    CHK(disp_common(g, wl, n, d.get()));
+    d->kind = DISP_FIXED;
+    double sigma = 1.0 / (LN_10 * 2.355 * resolution * g->step);
+    long long nme = (long long)std::ceil(6.0 * sigma);
+    long long K = nme + 1 - nme % 2;
+    if (K > 2 * HD_MAX + 1)
+        return fail(PASTA_ERR_ARG, "instrumental kernel longer than 255 taps (resolution too low)");
+    d->K = (int)K;
+    d->kern.resize(K);
+    make_gaussian(d->kern.data(), (int)K, (float)sigma);
+    *out = d.release();
+    return PASTA_OK;
+}
+
+int pasta_disp_create_variable(const pasta_grid* g, const double* wl, int n, const float* disp,
+                               pasta_disp** out) {
+    if (!disp) return fail(PASTA_ERR_ARG, "null dispersion array");
+    auto d = std::make_unique<pasta_disp>();
+    CHK(disp_common(g, wl, n, d.get()));
+    d->kind = DISP_VARIABLE;
+    const int W = g->W;
+    // resample_observed_to_synth (src/convolve_rv.rs:303-331)
+    std::vector<float> dr(W), sig(W);
+    int j = 0;
+    for (int i = 0; i < W; i++) {
+        double x = std::pow(10.0, g->first + g->step * (double)i);
+        while (j < n - 1 && wl[j + 1] < x) j++;
+        if (j == n - 1) {
+            dr[i] = disp[j];
+        } else {
+            float w = (float)((x - wl[j]) / (wl[j + 1] - wl[j]));
+            dr[i] = (1.0f - w) * disp[j] + w * disp[j + 1];
+        }
+    }
+    float largest = -INFINITY;
+    for (int i = 0; i < W; i++) {
+        double x = std::pow(10.0, g->first + g->step * (double)i);
+        sig[i] = dr[i] / (float)(g->step * LN_10 * x);
+        if (std::isnan(sig[i])) return fail(PASTA_ERR_ARG, "NaN dispersion");
+        largest = std::max(largest, sig[i]);
+    }
+    long long kme = (long long)std::ceil(largest * 6.0f);
+    long long K = kme + 1 - kme % 2;
+    if (K < 1 || K > 2 * HD_MAX + 1)
+        return fail(PASTA_ERR_ARG, "variable instrumental kernel longer than 255 taps");
+    d->K = (int)K;
+    d->kern.resize((size_t)W * K);
+    for (int i = 0; i < W; i++) make_gaussian(d->kern.data() + (size_t)i * K, (int)K, sig[i]);
+    *out = d.release();
+    return PASTA_OK;
+}
+
+int pasta_disp_destroy(pasta_disp* d) {
+    delete d;
+    return PASTA_OK;
+}
+
+int pasta_disp_kernel(const pasta_disp* d, float* out, int64_t cap, int* k_len) {
+    if (!d) return fail(PASTA_ERR_ARG, "null dispersion");
+    if (k_len) *k_len = d->K;
+    if (!out) return PASTA_OK;
+    if (d->kind == DISP_FIXED) {
+        if (cap < d->K) return fail(PASTA_ERR_ARG, "buffer too small");
+        std::copy(d->kern.begin(), d->kern.end(), out);
+    } else if (d->kind == DISP_VARIABLE) {
+        if (cap < (int64_t)d->kern.size()) return fail(PASTA_ERR_ARG, "buffer too small");
+        for (int k = 0; k < d->K; k++)
+            for (int i = 0; i < d->W; i++) out[(size_t)k * d->W + i] = d->kern[(size_t)i * d->K + k];
+    }
+    return PASTA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// continuum
+// ---------------------------------------------------------------------------
+static double map_range_d(double x, double fl, double fh, double tl, double th, bool clamp) {
+    double v = (x - fl) / (fh - fl) * (th - tl) + tl;
+    if (clamp) v = std::min(std::max(v, tl), th);
+    return v;
+}
+static float map_range_f(float x, float fl, float fh, float tl, float th) {
+    return (x - fl) / (fh - fl) * (th - tl) + tl;
+}
+static float poly_blend(float x) {  // src/continuum_fitting.rs:119-123
+    float xc = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+    float s = sinf(xc * (float)M_PI / 2.0f);
+    return s * s;
+}
+
+static int cholesky(std::vector<double>& A, int n) {  // in place, lower
+    for (int j = 0; j < n; j++) {
+        double s = A[j * n + j];
+        for (int k = 0; k < j; k++) s -= A[j * n + k] * A[j * n + k];
+        if (!(s > 0.0)) return -1;
+        double d = std::sqrt(s);
+        A[j * n + j] = d;
+        for (int i = j + 1; i < n; i++) {
+            double t = A[i * n + j];
+            for (int k = 0; k < j; k++) t -= A[i * n + k] * A[j * n + k];
+            A[i * n + j] = t / d;
+        }
+        for (int i = 0; i < j; i++) A[i * n + j] = 0.0;
+    }
+    return 0;
+}
+
+static void push_blocks(std::vector<int4>& out, int kind, int chunk, int i0, int i1) {
+    for (int s = i0; s < i1; s += BLK) out.push_back(make_int4(kind, chunk, s, std::min(s + BLK, i1)));
+}
+
+static void host_design_row(double xh, int p1, float* a) {
+    double sq[5];
+    sq[0] = xh;
+    for (int t = 1; t < 5; t++) sq[t] = sq[t - 1] * sq[t - 1];
+    for (int k = 0; k < p1; k++) {
+        double r = 1.0;
+        for (int t = 0; t < 5; t++)
+            if (k & (1 << t)) r *= sq[t];
+        a[k] = (float)r;
+    }
+}
+
+int pasta_cont_create_chunk(const double* wl, int n, int n_chunks, int p_order, double overlap,
+                            pasta_cont** out) {
+    if (!wl || n < 200 || n_chunks < 2 || n_chunks > CH_MAX || p_order < 0 || p_order + 1 > 31)
+        return fail(PASTA_ERR_ARG, "chunk continuum: bad arguments (n >= 200, 2..32 chunks, "
+                                   "degree <= 30)");
+    auto c = std::make_unique<pasta_cont>();
+    c->kind = CONT_CHUNK;
+    c->n = n;
+    c->n_chunks = n_chunks;
+    c->p1 = p_order + 1;
+    c->n_params = n_chunks * c->p1;
+    c->wl.assign(wl, wl + n);
+    c->startstop.resize(2 * n_chunks);
+    for (int i = 0; i < n_chunks; i++) {
+        double start = map_range_d((double)i - overlap, 0.0, (double)n_chunks, wl[0], wl[n - 1], true);
+        double stop = map_range_d((double)i + 1.0 + overlap, 0.0, (double)n_chunks, wl[0],
+                                  wl[n - 1], true);
+        int a = -1, b = -1;
+        for (int x = 0; x < n; x++) if (wl[x] > start) { a = x; break; }
+        for (int x = n - 1; x >= 0; x--) if (wl[x] < stop) { b = x; break; }
+        if (a < 0 || b < 0) return fail(PASTA_ERR_ARG, "chunk boundary not found");
+        c->startstop[2 * i] = a;
+        c->startstop[2 * i + 1] = b + 1;
+    }
+    c->startstop[0] = 0;
+    c->startstop[2 * n_chunks - 1] = n;
+    const int p1 = c->p1;
+    c->lohi.resize(2 * n_chunks);
+    c->chol.assign((size_t)n_chunks * p1 * p1, 0.0);
+    std::vector<float> a(p1);
+    for (int ch = 0; ch < n_chunks; ch++) {
+        int s = c->startstop[2 * ch], e = c->startstop[2 * ch + 1];
+        if (e - s < p1) return fail(PASTA_ERR_ARG, "chunk shorter than the polynomial order");
+        c->lohi[2 * ch] = wl[s];
+        c->lohi[2 * ch + 1] = wl[e - 1];
+        std::vector<double> G((size_t)p1 * p1, 0.0);
+        for (int i = s; i < e; i++) {
+            host_design_row(chunk_xhat(wl[i], wl[s], wl[e - 1]), p1, a.data());
+            for (int k = 0; k < p1; k++)
+                for (int l = 0; l <= k; l++) G[k * p1 + l] += (double)a[k] * (double)a[l];
+        }
+        for (int k = 0; k < p1; k++)
+            for (int l = k + 1; l < p1; l++) G[k * p1 + l] = G[l * p1 + k];
+        if (cholesky(G, p1)) return fail(PASTA_ERR_ARG, "chunk design matrix is rank deficient");
+        std::copy(G.begin(), G.end(), c->chol.begin() + (size_t)ch * p1 * p1);
+    }
+    c->fac.assign(n, 0.0f);
+    for (int ch = 0; ch + 1 < n_chunks; ch++) {
+        int stop = c->startstop[2 * ch + 1], sn = c->startstop[2 * (ch + 1)];
+        for (int i = sn; i < stop; i++)
+            c->fac[i] = poly_blend(map_range_f((float)wl[i], (float)wl[sn], (float)wl[stop - 1], 1.0f, 0.0f));
+    }
+    // moment blocks per chunk
+    c->mb_off.push_back(0);
+    for (int ch = 0; ch < n_chunks; ch++) {
+        push_blocks(c->mblocks, ch, 0, c->startstop[2 * ch], c->startstop[2 * ch + 1]);
+        c->mb_off.push_back((int)c->mblocks.size());
+    }
+    for (auto& b : c->mblocks) { b.x = b.y; b.y = b.z; b.z = b.w; b.w = 0; }  // (chunk, i0, i1, 0)
+    // chi2 segments (src/continuum_fitting.rs:300-358)
+    const int* ss = c->startstop.data();
+    if (ss[2] < 50) return fail(PASTA_ERR_ARG, "first chunk shorter than the 50-pixel trim");
+    push_blocks(c->cblocks, 0, 0, 50, ss[2]);
+    for (int ch = 1; ch < n_chunks; ch++) {
+        int start = ss[2 * ch], stop_prev = ss[2 * (ch - 1) + 1];
+        int start_next = (ch == n_chunks - 1) ? n - 50 : ss[2 * (ch + 1)];
+        if (stop_prev < start || start_next < stop_prev)
+            return fail(PASTA_ERR_ARG, "chunk overlaps do not form consecutive segments");
+        push_blocks(c->cblocks, 1, ch, start, stop_prev);
+        push_blocks(c->cblocks, 0, ch, stop_prev, start_next);
+    }
+    c->chi2_div = (double)(n - 100);
+    *out = c.release();
+    return PASTA_OK;
+}
+
+int pasta_cont_create_linear(const float* design, int n, int ncols, pasta_cont** out) {
+    if (!design || n < 101 || ncols < 1 || ncols > P1_MAX)
+        return fail(PASTA_ERR_ARG, "linear continuum: bad arguments (1..32 columns)");
+    auto c = std::make_unique<pasta_cont>();
+    c->kind = CONT_LINEAR;
+    c->n = n;
+    c->p1 = ncols;
+    c->n_params = ncols;
+    c->design_rm.assign(design, design + (size_t)n * ncols);
+    c->design_cm.resize((size_t)n * ncols);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < ncols; k++) c->design_cm[(size_t)k * n + i] = design[(size_t)i * ncols + k];
+    std::vector<double> G((size_t)ncols * ncols, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < ncols; k++)
+            for (int l = 0; l <= k; l++)
+                G[k * ncols + l] += (double)design[(size_t)i * ncols + k] * (double)design[(size_t)i * ncols + l];
+    for (int k = 0; k < ncols; k++)
+        for (int l = k + 1; l < ncols; l++) G[k * ncols + l] = G[l * ncols + k];
+    if (cholesky(G, ncols)) return fail(PASTA_ERR_ARG, "design matrix is rank deficient");
+    c->chol = G;
+    push_blocks(c->mblocks, 0, 0, 0, n);
+    for (auto& b : c->mblocks) { b.x = 0; b.y = b.z; b.z = b.w; b.w = 0; }
+    c->mb_off = {0, (int)c->mblocks.size()};
+    push_blocks(c->cblocks, 0, 0, 50, n - 50);
+    c->chi2_div = (double)n;
+    *out = c.release();
+    return PASTA_OK;
+}
+
+int pasta_cont_create_fixed(const float* continuum, int n, int ignore, pasta_cont** out) {
+    if (!continuum || n < 1 || ignore < 0 || 2 * ignore >= n)
+        return fail(PASTA_ERR_ARG, "fixed continuum: bad arguments");
+    auto c = std::make_unique<pasta_cont>();
+    c->kind = CONT_FIXED;
+    c->n = n;
+    c->ignore = ignore;
+    c->fixedc.assign(continuum, continuum + n);
+    push_blocks(c->cblocks, 0, 0, ignore, n - ignore);
+    c->chi2_div = (double)(n - 2 * ignore);
+    *out = c.release();
+    return PASTA_OK;
+}
+
+int pasta_cont_create_constant(int n, pasta_cont** out) {
+    if (n < 1) return fail(PASTA_ERR_ARG, "constant continuum: bad arguments");
+    auto c = std::make_unique<pasta_cont>();
+    c->kind = CONT_CONSTANT;
+    c->n = n;
+    push_blocks(c->mblocks, 0, 0, 0, n);
+    for (auto& b : c->mblocks) { b.x = 0; b.y = b.z; b.z = b.w; b.w = 0; }
+    c->mb_off = {0, (int)c->mblocks.size()};
+    push_blocks(c->cblocks, 0, 0, 0, n);
+    c->chi2_div = (double)n;
+    *out = c.release();
+    return PASTA_OK;
+}
+
+int pasta_cont_destroy(pasta_cont* c) {
+    delete c;
+    return PASTA_OK;
+}
+
+int pasta_cont_n_params(const pasta_cont* c) { return c ? c->n_params : 0; }
+
+int pasta_cont_chunks(const pasta_cont* c, int* ss, int cap) {
+    if (!c || c->kind != CONT_CHUNK) return fail(PASTA_ERR_ARG, "not a chunk continuum");
+    if (cap < 2 * c->n_chunks) return fail(PASTA_ERR_ARG, "buffer too small");
+    std::copy(c->startstop.begin(), c->startstop.end(), ss);
+    return c->n_chunks;
+}
+
+// build_continuum (src/continuum_fitting.rs:97-106,228-278,398-413,476-478,502-504)
+int pasta_cont_build(const pasta_cont* c, const float* params, int n_params, float* out,
+                     int64_t cap) {
+    if (!c) return fail(PASTA_ERR_ARG, "null continuum");
+    const int n = c->n;
+    if (c->kind == CONT_CONSTANT) {
+        if (n_params < 1 || cap < 1) return fail(PASTA_ERR_ARG, "bad parameters");
+        out[0] = params[0];
+        return 1;
+    }
+    if (cap < n) return fail(PASTA_ERR_ARG, "buffer too small");
+    if (c->kind == CONT_FIXED) {
+        std::copy(c->fixedc.begin(), c->fixedc.end(), out);
+        return n;
+    }
+    if (n_params != c->n_params)
+        return fail(PASTA_ERR_ARG, "Incorrect number of continuum parameters");
+    if (c->kind == CONT_LINEAR) {
+        for (int i = 0; i < n; i++) {
+            float y = 0.0f;
+            for (int k = 0; k < c->p1; k++) {
+                float t = c->design_rm[(size_t)i * c->p1 + k] * params[k];
+                y = (k == 0) ? t : t + y;
+            }
+            out[i] = y;
+        }
+        return n;
+    }
+    const int p1 = c->p1;
+    std::vector<float> a(p1);
+    std::fill(out, out + n, 0.0f);
+    for (int ch = 0; ch < c->n_chunks; ch++) {
+        int s = c->startstop[2 * ch], e = c->startstop[2 * ch + 1];
+        for (int i = s; i < e; i++) {
+            host_design_row(chunk_xhat(c->wl[i], c->wl[s], c->wl[e - 1]), p1, a.data());
+            float poly = 0.0f;
+            for (int k = 0; k < p1; k++) {
+                float t = a[k] * params[ch * p1 + k];
+                poly = (k == 0) ? t : t + poly;
+            }
+            if (ch == 0) { out[i] = poly; continue; }
+            int stop_prev = c->startstop[2 * (ch - 1) + 1];
+            float fac = poly_blend(map_range_f((float)c->wl[i], (float)c->wl[stop_prev - 1],
+                                               (float)c->wl[s], 0.0f, 1.0f));
+            out[i] = out[i] * fac + poly * (1.0f - fac);
+        }
+    }
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// star
+// ---------------------------------------------------------------------------
+template <typename T>
+static int dev_copy(pasta_star* s, const T* h, size_t n, const T** out) {
+    if (n == 0) { *out = nullptr; return PASTA_OK; }
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    s->allocs.push_back(p);
+    HIPCHK(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+    *out = (const T*)p;
+    return PASTA_OK;
+}
+
+pasta_star::~pasta_star() {
+    cache.reset();
+    for (void* p : allocs) (void)hipFree(p);
+}
+
+// Star without continuum (produce_model / interpolate paths)
+static int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, const float* flux,
+                      const float* var, int n, std::unique_ptr<pasta_star>& out) {
+    auto s = std::make_unique<pasta_star>();
+    s->grid = g;
+    s->N = n;
+    DevStar& D = s->dev;
+    D.N = n;
+    D.disp_kind = d ? d->kind : DISP_NONE;
+    D.Kd = d ? d->K : 0;
+    if (d) {
+        CHK(dev_copy(s.get(), d->fidx.data(), d->fidx.size(), &D.fidx));
+        CHK(dev_copy(s.get(), d->wl.data(), d->wl.size(), &D.wl));
+        CHK(dev_copy(s.get(), d->kern.data(), d->kern.size(), &D.dkern));
+        D.wl0 = d->wl.front();
+        D.wlN = d->wl.back();
+        double f0 = d->fidx.front(), fN = d->fidx.back();
+        D.n_tiles = (int)std::floor((fN - f0 + 1.0) / TILE) + 2;
+    } else {
+        D.n_tiles = (g->W + TILE - 1) / TILE;
+    }
+    if (flux) CHK(dev_copy(s.get(), flux, n, &D.flux));
+    if (var) CHK(dev_copy(s.get(), var, n, &D.var));
+    if (c) {
+        D.cont_kind = c->kind;
+        D.n_chunks = c->n_chunks;
+        D.p1 = c->p1;
+        D.ignore = c->ignore;
+        D.n_params = c->n_params;
+        D.n_mblocks = (int)c->mblocks.size();
+        D.n_cblocks = (int)c->cblocks.size();
+        D.chi2_div = c->chi2_div;
+        CHK(dev_copy(s.get(), c->lohi.data(), c->lohi.size(), &D.chunk_lohi));
+        CHK(dev_copy(s.get(), c->startstop.data(), c->startstop.size(), &D.startstop));
+        CHK(dev_copy(s.get(), c->chol.data(), c->chol.size(), &D.chol));
+        CHK(dev_copy(s.get(), c->fac.data(), c->fac.size(), &D.fac));
+        CHK(dev_copy(s.get(), c->design_cm.data(), c->design_cm.size(), &D.design));
+        CHK(dev_copy(s.get(), c->fixedc.data(), c->fixedc.size(), &D.fixedc));
+        CHK(dev_copy(s.get(), c->mblocks.data(), c->mblocks.size(), &D.mblocks));
+        CHK(dev_copy(s.get(), c->mb_off.data(), c->mb_off.size(), &D.mb_off));
+        CHK(dev_copy(s.get(), c->cblocks.data(), c->cblocks.size(), &D.cblocks));
+        s->n_params = c->n_params;
+        s->p1 = c->p1;
+        s->cont_kind = c->kind;
+        s->n_chunks = c->n_chunks;
+    } else {
+        D.cont_kind = CONT_FIXED;
+        D.n_params = 0;
+        D.chi2_div = 1.0;
+        s->n_params = 0;
+        s->cont_kind = -1;
+    }
+    out = std::move(s);
+    return PASTA_OK;
+}
+
+int pasta_star_create(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, const float* flux,
+                      const float* var, int n, pasta_star** out) {
+    if (!g || !d || !c || !flux || !var || !out) return fail(PASTA_ERR_ARG, "null argument");
+    if ((int)d->wl.size() != n)
+        return fail(PASTA_ERR_ARG, "Observed spectrum and target dispersion must have the same length");
+    if (c->n != n)
+        return fail(PASTA_ERR_ARG, "Length of observed and synthetic spectrum are different");
+    if (d->W != g->W || d->step != g->step || d->first != g->first)
+        return fail(PASTA_ERR_ARG, "dispersion was built for another synthetic grid");
+    std::unique_ptr<pasta_star> s;
+    CHK(star_build(g, d, c, flux, var, n, s));
+    *out = s.release();
+    return PASTA_OK;
+}
+
+int pasta_star_destroy(pasta_star* s) {
+    delete s;
+    return PASTA_OK;
+}
+
+static int64_t star_eval_bytes(const pasta_star* s) {
+    const DevStar& D = s->dev;
+    int64_t b = (int64_t)D.N * 4 + sizeof(DevEval) + KROT_MAX * 4 + 5 * 8 * 4;
+    b += (int64_t)D.n_mblocks * std::max(1, D.p1) * 8 + (int64_t)D.n_cblocks * 8 + D.n_params * 4;
+    b += (int64_t)(D.n_tiles + D.n_mblocks + D.n_cblocks + std::max(1, D.n_chunks)) * 8;
+    return b;
+}
+
+int pasta_star_eval_bytes(const pasta_star* s, int64_t* bytes) {
+    if (!s) return fail(PASTA_ERR_ARG, "null star");
+    *bytes = star_eval_bytes(s);
+    return PASTA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batch of evaluations
+// ---------------------------------------------------------------------------
+struct Batch {
+    int mode = 0;  // 0: resample to the observed pixels; 1: synthetic-grid output
+    int n_evals = 0, n_stars = 0, pmax = 1;
+    pasta_grid* grid = nullptr;
+    std::vector<pasta_star*> stars;
+    std::vector<int> first_eval, count;
+    std::vector<DevEval> h_evals;
+    int64_t model_len = 0, mpart_len = 0, cpart_len = 0, pf_len = 0;
+    DBuf<DevStar> d_stars;
+    DBuf<DevEval> d_evals;
+    DBuf<double> d_labels, d_mpart, d_cpart, d_chi2;
+    DBuf<float> d_rot, d_model, d_pf;
+    DBuf<int> d_status;
+    DBuf<int2> d_bm, d_bmom, d_solve, d_bchi2;
+    int n_bm = 0, n_bmom = 0, n_solve = 0, n_bchi2 = 0;
+    // batched PSO state
+    DBuf<DevPso> d_pso;
+    DBuf<double> d_vel, d_bpos, d_bcost, d_trace;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    ~Batch() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
+                       const std::vector<int>& counts, int mode) {
+    B.mode = mode;
+    B.grid = g;
+    B.stars = stars;
+    B.n_stars = (int)stars.size();
+    B.count = counts;
+    B.first_eval.assign(B.n_stars, 0);
+    int ne = 0;
+    B.pmax = 1;
+    for (int s = 0; s < B.n_stars; s++) {
+        B.first_eval[s] = ne;
+        ne += counts[s];
+        B.pmax = std::max(B.pmax, stars[s]->dev.p1);
+    }
+    B.n_evals = ne;
+    B.h_evals.assign(ne, DevEval{});
+    int64_t mo = 0, mp = 0, cp = 0, pf = 0;
+    std::vector<int2> bm, bmom, solve, bchi2;
+    int max_tiles = 0;
+    for (int s = 0; s < B.n_stars; s++) max_tiles = std::max(max_tiles, stars[s]->dev.n_tiles);
+    for (int s = 0; s < B.n_stars; s++) {
+        const DevStar& D = stars[s]->dev;
+        const int outlen = mode == 0 ? D.N : g->W;
+        for (int q = 0; q < counts[s]; q++) {
+            int e = B.first_eval[s] + q;
+            DevEval& ev = B.h_evals[e];
+            ev.star = s;
+            ev.active = 1;
+            ev.m_off = mo;
+            mo += outlen;
+            ev.mp_off = mp;
+            mp += (int64_t)D.n_mblocks * std::max(1, D.p1);
+            ev.cp_off = cp;
+            cp += D.n_cblocks;
+            ev.pf_off = pf;
+            pf += std::max(1, D.n_params);
+            for (int b = 0; b < D.n_mblocks; b++) bmom.push_back(make_int2(e, b));
+            for (int b = 0; b < D.n_cblocks; b++) bchi2.push_back(make_int2(e, b));
+            if (D.cont_kind == CONT_CHUNK)
+                for (int c = 0; c < D.n_chunks; c++) solve.push_back(make_int2(e, c));
+            else if (D.cont_kind == CONT_LINEAR || D.cont_kind == CONT_CONSTANT)
+                solve.push_back(make_int2(e, 0));
+        }
+    }
+    // model blocks tile-major: all evaluations working on one synthetic tile run together
+    for (int t = 0; t < max_tiles; t++)
+        for (int s = 0; s < B.n_stars; s++)
+            if (t < stars[s]->dev.n_tiles)
+                for (int q = 0; q < counts[s]; q++) bm.push_back(make_int2(B.first_eval[s] + q, t));
+    B.model_len = mo; B.mpart_len = mp; B.cpart_len = cp; B.pf_len = pf;
+    B.n_bm = (int)bm.size(); B.n_bmom = (int)bmom.size();
+    B.n_solve = (int)solve.size(); B.n_bchi2 = (int)bchi2.size();
+    std::vector<DevStar> ds(B.n_stars);
+    for (int s = 0; s < B.n_stars; s++) ds[s] = stars[s]->dev;
+    CHK(B.d_stars.upload(ds));
+    CHK(B.d_evals.upload(B.h_evals));
+    CHK(B.d_labels.alloc((size_t)ne * 5));
+    CHK(B.d_rot.alloc((size_t)ne * KROT_MAX));
+    CHK(B.d_model.alloc((size_t)std::max<int64_t>(mo, 1)));
+    CHK(B.d_mpart.alloc((size_t)std::max<int64_t>(mp, 1)));
+    CHK(B.d_cpart.alloc((size_t)std::max<int64_t>(cp, 1)));
+    CHK(B.d_pf.alloc((size_t)std::max<int64_t>(pf, 1)));
+    CHK(B.d_chi2.alloc(ne));
+    CHK(B.d_status.alloc(ne));
+    CHK(B.d_bm.upload(bm));
+    CHK(B.d_bmom.upload(bmom));
+    CHK(B.d_solve.upload(solve));
+    CHK(B.d_bchi2.upload(bchi2));
+    if (!B.stream) HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+    return PASTA_OK;
+}
+
+static int batch_launch(Batch& B) {
+    DevGrid G = B.grid->dev();
+    HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p, B.d_bm.p,
+                       B.n_bm, B.d_bmom.p, B.n_bmom, B.d_solve.p, B.n_solve, B.d_bchi2.p,
+                       B.n_bchi2, B.d_model.p, B.d_mpart.p, B.d_pf.p, B.d_cpart.p, B.d_chi2.p,
+                       B.d_status.p, B.pmax, B.mode, B.stream));
+    return PASTA_OK;
+}
+
+static int batch_eval(Batch& B, const double* labels, double* chi2, int* status) {
+    HIPCHK(hipMemcpyAsync(B.d_labels.p, labels, sizeof(double) * 5 * B.n_evals,
+                          hipMemcpyHostToDevice, B.stream));
+    CHK(batch_launch(B));
+    if (chi2)
+        HIPCHK(hipMemcpyAsync(chi2, B.d_chi2.p, sizeof(double) * B.n_evals, hipMemcpyDeviceToHost,
+                              B.stream));
+    if (status)
+        HIPCHK(hipMemcpyAsync(status, B.d_status.p, sizeof(int) * B.n_evals, hipMemcpyDeviceToHost,
+                              B.stream));
+    HIPCHK(hipStreamSynchronize(B.stream));
+    return PASTA_OK;
+}
+
+static const char* status_text(int st) {
+    switch (st) {
+    case ST_BOUNDS: return "labels out of the grid bounds";
+    case ST_NEIGHBORS: return "not enough neighbours";
+    case ST_VSINI: return "vsini must be positive";
+    case ST_KERNEL: return "rotation kernel larger than FFTSIZE";
+    case ST_RANGE: return "observed wavelength outside the synthetic range after the RV shift";
+    case ST_INDEX: return "resample index past the end of the model";
+    case ST_PSO: return "particle is not within bounds";
+    case ST_SOLVE: return "continuum solve failed";
+    default: return "evaluation failed";
+    }
+}
+
+static int get_cached_batch(pasta_star* s, int P, Batch** out) {
+    if (!s->cache || s->cache_P != P) {
+        s->cache.reset(new Batch());
+        CHK(batch_build(*s->cache, s->grid, {s}, {P}, 0));
+        s->cache_P = P;
+    }
+    *out = s->cache.get();
+    return PASTA_OK;
+}
+
+int pasta_eval_chi2(pasta_star* s, const double* labels, int P, double* chi2, int* status,
+                    float* params) {
+    if (!s || !labels || P <= 0 || !chi2) return fail(PASTA_ERR_ARG, "bad arguments");
+    if (s->cont_kind < 0) return fail(PASTA_ERR_ARG, "star has no continuum model");
+    std::lock_guard<std::mutex> lk(s->mu);
+    Batch* B;
+    CHK(get_cached_batch(s, P, &B));
+    std::vector<int> st(P);
+    CHK(batch_eval(*B, labels, chi2, st.data()));
+    if (params) {
+        const int np = std::max(1, s->n_params);
+        if (s->cont_kind == CONT_FIXED) {
+            for (int i = 0; i < P * np; i++) params[i] = 0.0f;
+        } else {
+            HIPCHK(hipMemcpy(params, B->d_pf.p, sizeof(float) * (size_t)P * np, hipMemcpyDeviceToHost));
+        }
+    }
+    int bad = 0;
+    for (int i = 0; i < P; i++) {
+        if (status) status[i] = st[i];
+        if (st[i] && !bad) bad = st[i];
+    }
+    if (bad) return fail(PASTA_ERR_MODEL, status_text(bad));
+    return PASTA_OK;
+}
+
+static int model_only(pasta_grid* g, const pasta_disp* d, const double* labels5, int P,
+                      float* out, int* status, int mode) {
+    if (!g || !labels5 || P <= 0 || !out) return fail(PASTA_ERR_ARG, "bad arguments");
+    pasta_disp none;
+    if (!d) {  // synthetic-grid output without instrumental broadening
+        none.kind = DISP_NONE;
+        none.W = g->W;
+        none.wl = {std::pow(10.0, g->first), std::pow(10.0, g->first + g->step * (g->W - 1))};
+        none.fidx = {0.0, (double)(g->W - 1)};
+        d = &none;
+    }
+    std::unique_ptr<pasta_star> s;
+    int n = mode == 0 ? (int)d->wl.size() : g->W;
+    CHK(star_build(g, d, nullptr, nullptr, nullptr, n, s));
+    if (mode == 1) s->dev.n_tiles = (g->W + TILE - 1) / TILE;
+    std::lock_guard<std::mutex> lk(g->mu);
+    Batch B;
+    CHK(batch_build(B, g, {s.get()}, {P}, mode));
+    std::vector<int> st(P);
+    CHK(batch_eval(B, labels5, nullptr, st.data()));
+    HIPCHK(hipMemcpy(out, B.d_model.p, sizeof(float) * (size_t)P * n, hipMemcpyDeviceToHost));
+    int bad = 0;
+    for (int i = 0; i < P; i++) {
+        if (status) status[i] = st[i];
+        if (st[i]) {
+            if (!bad) bad = st[i];
+            for (int k = 0; k < n; k++) out[(size_t)i * n + k] = NAN;
+        }
+    }
+    if (bad) return fail(PASTA_ERR_MODEL, status_text(bad));
+    return PASTA_OK;
+}
+
+int pasta_produce_model(pasta_grid* g, const pasta_disp* d, const double* labels, int P, float* out,
+                        int* status) {
+    if (!d) return fail(PASTA_ERR_ARG, "null dispersion");
+    return model_only(g, d, labels, P, out, status, 0);
+}
+
+int pasta_interpolate(pasta_grid* g, const double* labels, int P, float* out, int* status) {
+    if (!labels || P <= 0) return fail(PASTA_ERR_ARG, "bad arguments");
+    std::vector<double> l5((size_t)P * 5, 0.0);
+    for (int i = 0; i < P; i++)
+        for (int d = 0; d < 3; d++) l5[5 * i + d] = labels[3 * i + d];
+    return model_only(g, nullptr, l5.data(), P, out, status, 1);
+}
+
+int pasta_interpolate_and_convolve(pasta_grid* g, const pasta_disp* d, const double* labels, int P,
+                                   float* out, int* status) {
+    if (!labels || P <= 0 || !d) return fail(PASTA_ERR_ARG, "bad arguments");
+    std::vector<double> l5((size_t)P * 5, 0.0);
+    for (int i = 0; i < P; i++)
+        for (int k = 0; k < 4; k++) l5[5 * i + k] = labels[4 * i + k];
+    pasta_disp dd = *d;  // synthetic-grid output ignores the observed sampling
+    return model_only(g, &dd, l5.data(), P, out, status, 1);
+}
+
+int pasta_vsini_kernel(const pasta_grid* g, double vsini, float* out, int cap, int* len) {
+    if (!g) return fail(PASTA_ERR_ARG, "null grid");
+    // build_rotation_kernel (src/convolve_rv.rs:438-475)
+    const double eps = 0.6, dvelo = LN_10 * g->step, vrot = vsini / C_KMS;
+    double a = 4.0 * (eps - 1.0) / (M_PI * eps);
+    double v = std::ceil(2.0 * vrot / dvelo * std::sqrt(1.0 + a * a));
+    long long nme = (v >= 0.0 && v < 1e9) ? (long long)v + 2 : 2;
+    if (nme < 1) nme = 1;
+    int n = (int)(nme + 1 - nme % 2);
+    if (len) *len = n;
+    if (!out) return PASTA_OK;
+    if (cap < n) return fail(PASTA_ERR_ARG, "buffer too small");
+    double mid = ((double)(n - 1) * dvelo) / 2.0;
+    float s = 0.0f;
+    for (int i = 0; i < n; i++) {
+        double vk = (double)i * dvelo - mid;
+        double q = vk / vrot;
+        double y = 1.0 - q * q;
+        float k = 0.0f;
+        if (y > 0.0)
+            k = (float)((2.0 * (1.0 - eps) * std::sqrt(y) + M_PI * eps / 2.0 * y) /
+                        (M_PI * vrot * (1.0 - eps / 3.0)));
+        out[i] = k;
+        s = s + k;
+    }
+    for (int i = 0; i < n; i++) out[i] = out[i] / s;
+    return PASTA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// PSO set-up shared by the plain and batched fits
+// ---------------------------------------------------------------------------
+static int settings_check(const pasta_fit_settings* st) {
+    if (!st || st->num_particles < 1 || st->max_iters < 0)
+        return fail(PASTA_ERR_ARG, "bad PSO settings");
+    if (st->social_factor < 0.0) return fail(PASTA_ERR_ARG, "social factor must be >=0.");
+    if (st->delta < 0.0) return fail(PASTA_ERR_ARG, "delta must be >=0.");
+    return PASTA_OK;
+}
+
+static Bounds make_bounds(const pasta_fit_settings* st) {
+    Bounds b{};
+    b.vsini_lo = st->vsini_range[0]; b.vsini_hi = st->vsini_range[1];
+    b.rv_lo = st->rv_range[0]; b.rv_hi = st->rv_range[1];
+    for (int d = 0; d < 5; d++) {
+        b.ckind[d] = st->constraint_kind[d];
+        b.ca[d] = st->constraint_a[d];
+        b.cb[d] = st->constraint_b[d];
+    }
+    return b;
+}
+
+// initialize_positions_and_velocities with the rejection sampler
+// (src/particleswarm.rs:137-150, src/bounds.rs:21-35)
+static int pso_init_positions(const GridGeom& g, const Bounds& b, int P, uint64_t seed, double* pos,
+                              double* vel) {
+    double lo[5], hi[5];
+    bounds_outer(g, b, lo, hi);
+    const uint64_t kp = rng_u64(seed, KEY_POS), kv = rng_u64(seed, KEY_VEL);
+    uint64_t attempt = 0;
+    int got = 0;
+    while (got < P) {
+        double p[5];
+        for (int d = 0; d < 5; d++) p[d] = rand_range(lo[d], hi[d], rng_uniform(kp, attempt * 5 + d));
+        attempt++;
+        if (attempt > 100000000ULL)
+            return fail(PASTA_ERR_FIT, "cannot draw particles inside the bounds");
+        if (bounds_within(g, b, p)) {
+            std::memcpy(pos + 5 * got, p, sizeof(p));
+            got++;
+        }
+    }
+    for (int q = 0; q < P; q++)
+        for (int d = 0; d < 5; d++) {
+            double delta = hi[d] - lo[d];
+            vel[5 * q + d] = rand_range(-delta, delta, rng_uniform(kv, (uint64_t)q * 5 + d));
+        }
+    return PASTA_OK;
+}
+
+static int final_params(pasta_star* s, const double best[5], pasta_fit_result* out) {
+    // one more evaluation at the best labels for the continuum parameters (src/fitting.rs:436-440)
+    Batch B;
+    CHK(batch_build(B, s->grid, {s}, {1}, 0));
+    double c;
+    int st;
+    CHK(batch_eval(B, best, &c, &st));
+    if (st) return fail(PASTA_ERR_FIT, status_text(st));
+    int np = std::min(512, std::max(1, s->n_params));
+    out->n_params = np;
+    if (s->cont_kind == CONT_FIXED) {
+        for (int k = 0; k < np; k++) out->continuum_params[k] = 0.0f;
+    } else {
+        HIPCHK(hipMemcpy(out->continuum_params, B.d_pf.p, sizeof(float) * np, hipMemcpyDeviceToHost));
+    }
+    return PASTA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plain fit: host PSO, one device round trip per iteration
+// ---------------------------------------------------------------------------
+int pasta_fit(pasta_star* s, const pasta_fit_settings* st, uint64_t seed, const double* init_pos,
+              pasta_fit_result* out, double* trace) {
+    if (!s || !out) return fail(PASTA_ERR_ARG, "null argument");
+    if (s->cont_kind < 0) return fail(PASTA_ERR_ARG, "star has no continuum model");
+    CHK(settings_check(st));
+    std::memset(out, 0, sizeof(*out));
+    auto t_start = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    const GridGeom& g = s->grid->geom;
+    const Bounds b = make_bounds(st);
+    const PsoParams ps{st->inertia_factor, st->cognitive_factor, st->social_factor};
+    const int P = st->num_particles;
+    std::vector<double> pos(5 * P), vel(5 * P), bpos(5 * P), cost(P), bcost(P);
+    std::vector<int> sts(P);
+    CHK(pso_init_positions(g, b, P, seed, pos.data(), vel.data()));
+    if (init_pos) std::memcpy(pos.data(), init_pos, sizeof(double) * 5 * P);
+    Batch* B;
+    CHK(get_cached_batch(s, P, &B));
+    auto evaluate = [&]() -> int {
+        CHK(batch_eval(*B, pos.data(), cost.data(), sts.data()));
+        for (int q = 0; q < P; q++)
+            if (sts[q]) {
+                out->status = sts[q];
+                return fail(PASTA_ERR_FIT, std::string("Error computing cost: ") + status_text(sts[q]));
+            }
+        return PASTA_OK;
+    };
+    CHK(evaluate());
+    // sort by cost, stable (src/particleswarm.rs:127-131)
+    std::vector<int> order(P);
+    for (int q = 0; q < P; q++) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] < cost[c]; });
+    {
+        std::vector<double> p2(5 * P), v2(5 * P), c2(P);
+        for (int q = 0; q < P; q++) {
+            std::memcpy(&p2[5 * q], &pos[5 * order[q]], 40);
+            std::memcpy(&v2[5 * q], &vel[5 * order[q]], 40);
+            c2[q] = cost[order[q]];
+        }
+        pos.swap(p2); vel.swap(v2); cost.swap(c2);
+    }
+    bpos = pos;
+    bcost = cost;
+    double gb[5], gc = cost[0];
+    std::memcpy(gb, pos.data(), sizeof(gb));
+    if (trace) { trace[0] = gc; std::memcpy(trace + 1, gb, sizeof(gb)); }
+    const uint64_t ki = rng_u64(seed, KEY_ITER);
+    for (int it = 1; it <= st->max_iters; it++) {
+        for (int q = 0; q < P; q++) {
+            int rc = pso_update_particle(g, b, ps, ki, it, P, q, &pos[5 * q], &vel[5 * q],
+                                         &bpos[5 * q], gb);
+            if (rc) {
+                out->status = rc;
+                return fail(PASTA_ERR_FIT, "Particle is not within bounds");
+            }
+        }
+        CHK(evaluate());
+        for (int q = 0; q < P; q++) {
+            if (cost[q] < bcost[q]) {
+                std::memcpy(&bpos[5 * q], &pos[5 * q], 40);
+                bcost[q] = cost[q];
+                if (cost[q] < gc) {
+                    std::memcpy(gb, &pos[5 * q], 40);
+                    gc = cost[q];
+                }
+            }
+        }
+        if (trace) { trace[6 * it] = gc; std::memcpy(trace + 6 * it + 1, gb, sizeof(gb)); }
+    }
+    std::memcpy(out->label, gb, sizeof(gb));
+    out->chi2 = gc;
+    out->iterations = st->max_iters;
+    CHK(final_params(s, gb, out));
+    out->time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return PASTA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched fit: device PSO, graph-captured iterations
+// ---------------------------------------------------------------------------
+struct PsoBatch {
+    Batch B;
+    std::vector<DevPso> h_pso;
+};
+
+static int pso_batch_setup(PsoBatch& PB, pasta_grid* g, const std::vector<pasta_star*>& stars,
+                           const std::vector<const pasta_fit_settings*>& sts,
+                           const std::vector<uint64_t>& seeds, bool with_trace, int* max_it) {
+    Batch& B = PB.B;
+    std::vector<int> counts;
+    for (auto* st : sts) counts.push_back(st->num_particles);
+    CHK(batch_build(B, g, stars, counts, 0));
+    const int ns = (int)stars.size();
+    std::vector<double> pos((size_t)B.n_evals * 5), vel((size_t)B.n_evals * 5);
+    PB.h_pso.assign(ns, DevPso{});
+    *max_it = 0;
+    for (int s = 0; s < ns; s++) {
+        const pasta_fit_settings* st = sts[s];
+        DevPso& P = PB.h_pso[s];
+        P.b = make_bounds(st);
+        P.s = PsoParams{st->inertia_factor, st->cognitive_factor, st->social_factor};
+        P.key_iter = rng_u64(seeds[s], KEY_ITER);
+        P.P = st->num_particles;
+        P.first_eval = B.first_eval[s];
+        P.iter = 0;
+        P.max_iters = st->max_iters;
+        *max_it = std::max(*max_it, st->max_iters);
+        CHK(pso_init_positions(g->geom, P.b, P.P, seeds[s], &pos[5 * (size_t)P.first_eval],
+                               &vel[5 * (size_t)P.first_eval]));
+    }
+    CHK(B.d_labels.upload(pos));
+    CHK(B.d_vel.upload(vel));
+    CHK(B.d_bpos.alloc(pos.size()));
+    CHK(B.d_bcost.alloc(B.n_evals));
+    CHK(B.d_pso.upload(PB.h_pso));
+    if (with_trace) CHK(B.d_trace.alloc((size_t)ns * (*max_it + 1) * 6));
+    return PASTA_OK;
+}
+
+static int pso_first_eval(PsoBatch& PB) {
+    Batch& B = PB.B;
+    CHK(batch_launch(B));
+    HIPCHK(launch_pso_init(B.d_pso.p, B.n_stars, B.d_evals.p, B.d_labels.p, B.d_vel.p, B.d_bpos.p,
+                           B.d_bcost.p, B.d_chi2.p, B.d_trace.p, B.stream));
+    return PASTA_OK;
+}
+
+static int pso_iteration_launches(PsoBatch& PB) {
+    Batch& B = PB.B;
+    HIPCHK(launch_pso_update(B.grid->d_geom.p, B.d_pso.p, B.d_evals.p, B.n_evals, B.d_labels.p,
+                             B.d_vel.p, B.d_bpos.p, B.stream));
+    CHK(batch_launch(B));
+    HIPCHK(launch_pso_best(B.d_pso.p, B.n_stars, B.d_evals.p, B.d_labels.p, B.d_bpos.p,
+                           B.d_bcost.p, B.d_chi2.p, B.d_trace.p, B.stream));
+    return PASTA_OK;
+}
+
+static int pso_capture(PsoBatch& PB) {
+    Batch& B = PB.B;
+    hipGraph_t gr;
+    HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+    int rc = pso_iteration_launches(PB);
+    hipError_t e = hipStreamEndCapture(B.stream, &gr);
+    if (rc) return rc;
+    HIPCHK(e);
+    HIPCHK(hipGraphInstantiate(&B.graph, gr, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(gr));
+    return PASTA_OK;
+}
+
+int pasta_fit_bulk(pasta_star* const* stars, int n_stars, const pasta_fit_settings* st,
+                   const uint64_t* seeds, pasta_fit_result* out, double* trace) {
+    if (!stars || n_stars <= 0 || !seeds || !out) return fail(PASTA_ERR_ARG, "bad arguments");
+    CHK(settings_check(st));
+    auto t_start = std::chrono::steady_clock::now();
+    pasta_grid* g = stars[0]->grid;
+    std::vector<pasta_star*> sv(stars, stars + n_stars);
+    for (auto* s : sv) {
+        if (!s || s->grid != g) return fail(PASTA_ERR_ARG, "stars must share one grid");
+        if (s->cont_kind < 0) return fail(PASTA_ERR_ARG, "star has no continuum model");
+    }
+    std::lock_guard<std::mutex> lk(g->mu);
+    PsoBatch PB;
+    int max_it = 0;
+    std::vector<const pasta_fit_settings*> sts(n_stars, st);
+    CHK(pso_batch_setup(PB, g, sv, sts, std::vector<uint64_t>(seeds, seeds + n_stars),
+                        trace != nullptr, &max_it));
+    CHK(pso_first_eval(PB));
+    if (max_it > 0) {
+        CHK(pso_capture(PB));
+        for (int it = 0; it < max_it; it++) HIPCHK(hipGraphLaunch(PB.B.graph, PB.B.stream));
+    }
+    HIPCHK(hipMemcpyAsync(PB.h_pso.data(), PB.B.d_pso.p, sizeof(DevPso) * n_stars,
+                          hipMemcpyDeviceToHost, PB.B.stream));
+    if (trace)
+        HIPCHK(hipMemcpyAsync(trace, PB.B.d_trace.p, sizeof(double) * n_stars * (max_it + 1) * 6,
+                              hipMemcpyDeviceToHost, PB.B.stream));
+    HIPCHK(hipStreamSynchronize(PB.B.stream));
+    double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    int first_bad = 0;
+    for (int s = 0; s < n_stars; s++) {
+        pasta_fit_result& r = out[s];
+        std::memset(&r, 0, sizeof(r));
+        const DevPso& P = PB.h_pso[s];
+        r.status = P.status;
+        r.iterations = st->max_iters;
+        r.time_s = elapsed;
+        std::memcpy(r.label, P.gbest, sizeof(r.label));
+        r.chi2 = P.gcost;
+        if (P.status) {
+            if (!first_bad) first_bad = P.status;
+            continue;
+        }
+    }
+    // continuum parameters at the best labels: one evaluation per star
+    {
+        Batch F;
+        CHK(batch_build(F, g, sv, std::vector<int>(n_stars, 1), 0));
+        std::vector<double> lab((size_t)n_stars * 5);
+        for (int s = 0; s < n_stars; s++) std::memcpy(&lab[5 * s], out[s].label, 40);
+        std::vector<double> c(n_stars);
+        std::vector<int> fs(n_stars);
+        CHK(batch_eval(F, lab.data(), c.data(), fs.data()));
+        std::vector<float> pf((size_t)std::max<int64_t>(F.pf_len, 1));
+        HIPCHK(hipMemcpy(pf.data(), F.d_pf.p, sizeof(float) * pf.size(), hipMemcpyDeviceToHost));
+        for (int s = 0; s < n_stars; s++) {
+            if (out[s].status) continue;
+            int np = std::min(512, std::max(1, sv[s]->n_params));
+            out[s].n_params = np;
+            for (int k = 0; k < np; k++)
+                out[s].continuum_params[k] =
+                    sv[s]->cont_kind == CONT_FIXED ? 0.0f : pf[F.h_evals[s].pf_off + k];
+        }
+    }
+    if (first_bad) return fail(PASTA_ERR_FIT, std::string("a fit failed: ") + status_text(first_bad));
+    return PASTA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// server: continuous batching of fit requests over the resident grid
+// ---------------------------------------------------------------------------
+#include <deque>
+#include <map>
+
+struct Request {
+    uint64_t id = 0;
+    pasta_star* star = nullptr;
+    pasta_fit_settings st{};
+    uint64_t seed = 0;
+    int state = 0;  // 0 waiting, 1 running, 2 finished, 3 cancelled
+    int64_t bytes = 0;
+    std::chrono::steady_clock::time_point t_submit;
+    pasta_fit_result res{};
+    // PSO state carried across batch rebuilds (host copies)
+    bool started = false;
+    DevPso pso{};
+    std::vector<double> pos, vel, bpos, bcost;
+};
+
+struct pasta_server {
+    pasta_grid* g = nullptr;
+    pasta_server_config cfg{};
+    int64_t budget = 0, in_use = 0;
+    uint64_t next_id = 1;
+    std::deque<Request*> waiting;
+    std::vector<Request*> running;
+    std::map<uint64_t, std::unique_ptr<Request>> all;
+    std::unique_ptr<PsoBatch> pb;
+    bool dirty = false;
+    int64_t evals_done = 0, finished = 0, iters_run = 0;
+    std::mutex mu;
+    ~pasta_server() {
+        pb.reset();
+        for (auto& kv : all) delete kv.second->star;
+    }
+};
+
+int pasta_server_create(pasta_grid* g, const pasta_server_config* cfg, pasta_server** out) {
+    if (!g || !cfg || !out || cfg->max_stars < 1 || cfg->max_particles < 1 || cfg->iters_per_step < 1)
+        return fail(PASTA_ERR_ARG, "bad server configuration");
+    auto sv = std::make_unique<pasta_server>();
+    sv->g = g;
+    sv->cfg = *cfg;
+    if (cfg->hbm_budget_bytes > 0) {
+        sv->budget = cfg->hbm_budget_bytes;
+    } else {
+        size_t f = 0, t = 0;
+        HIPCHK(hipMemGetInfo(&f, &t));
+        sv->budget = std::max<int64_t>((int64_t)f - (4LL << 30), 1LL << 28);
+    }
+    *out = sv.release();
+    return PASTA_OK;
+}
+
+int pasta_server_destroy(pasta_server* sv) {
+    delete sv;
+    return PASTA_OK;
+}
+
+int pasta_server_submit(pasta_server* sv, pasta_star* s, const pasta_fit_settings* st, uint64_t seed,
+                        uint64_t* id) {
+    if (!sv || !s || !id) return fail(PASTA_ERR_ARG, "null argument");
+    if (s->grid != sv->g) return fail(PASTA_ERR_ARG, "star belongs to another grid");
+    if (s->cont_kind < 0) return fail(PASTA_ERR_ARG, "star has no continuum model");
+    CHK(settings_check(st));
+    if (st->num_particles > sv->cfg.max_particles)
+        return fail(PASTA_ERR_ARG, "more particles than the server's max_particles");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto r = std::make_unique<Request>();
+    r->id = sv->next_id++;
+    r->star = s;
+    r->st = *st;
+    r->seed = seed;
+    r->bytes = star_eval_bytes(s) * st->num_particles + 4 * 8 * 5 * st->num_particles;
+    if (r->bytes > sv->budget) return fail(PASTA_ERR_NOMEM, "request larger than the HBM budget");
+    r->t_submit = std::chrono::steady_clock::now();
+    *id = r->id;
+    sv->waiting.push_back(r.get());
+    sv->all[r->id] = std::move(r);
+    return PASTA_OK;
+}
+
+// Download the PSO state of every running, started star from the current batch.
+static int server_save_state(pasta_server* sv) {
+    if (!sv->pb) return PASTA_OK;
+    Batch& B = sv->pb->B;
+    std::vector<DevPso> ps(B.n_stars);
+    HIPCHK(hipMemcpy(ps.data(), B.d_pso.p, sizeof(DevPso) * B.n_stars, hipMemcpyDeviceToHost));
+    std::vector<double> pos((size_t)B.n_evals * 5), vel(pos.size()), bpos(pos.size()), bcost(B.n_evals);
+    HIPCHK(hipMemcpy(pos.data(), B.d_labels.p, sizeof(double) * pos.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vel.data(), B.d_vel.p, sizeof(double) * vel.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bpos.data(), B.d_bpos.p, sizeof(double) * bpos.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bcost.data(), B.d_bcost.p, sizeof(double) * bcost.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < B.n_stars; s++) {
+        Request* r = sv->running[s];
+        int f0 = B.first_eval[s], P = B.count[s];
+        r->pso = ps[s];
+        r->started = true;
+        r->pos.assign(pos.begin() + 5 * f0, pos.begin() + 5 * (f0 + P));
+        r->vel.assign(vel.begin() + 5 * f0, vel.begin() + 5 * (f0 + P));
+        r->bpos.assign(bpos.begin() + 5 * f0, bpos.begin() + 5 * (f0 + P));
+        r->bcost.assign(bcost.begin() + f0, bcost.begin() + f0 + P);
+    }
+    return PASTA_OK;
+}
+
+// Build a batch for the running set; started stars resume from their saved state,
+// new stars get their initial population evaluated and sorted.
+static int server_rebuild(pasta_server* sv) {
+    sv->pb.reset();
+    if (sv->running.empty()) return PASTA_OK;
+    auto pb = std::make_unique<PsoBatch>();
+    std::vector<pasta_star*> stars;
+    std::vector<const pasta_fit_settings*> sts;
+    std::vector<uint64_t> seeds;
+    for (Request* r : sv->running) {
+        stars.push_back(r->star);
+        sts.push_back(&r->st);
+        seeds.push_back(r->seed);
+    }
+    int max_it = 0;
+    CHK(pso_batch_setup(*pb, sv->g, stars, sts, seeds, false, &max_it));
+    Batch& B = pb->B;
+    bool any_new = false;
+    for (int s = 0; s < B.n_stars; s++) {
+        Request* r = sv->running[s];
+        int f0 = B.first_eval[s], P = B.count[s];
+        if (r->started) {
+            DevPso p = r->pso;
+            p.first_eval = f0;
+            pb->h_pso[s] = p;
+            HIPCHK(hipMemcpy(B.d_labels.p + 5 * f0, r->pos.data(), 40 * P, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(B.d_vel.p + 5 * f0, r->vel.data(), 40 * P, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(B.d_bpos.p + 5 * f0, r->bpos.data(), 40 * P, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(B.d_bcost.p + f0, r->bcost.data(), 8 * P, hipMemcpyHostToDevice));
+            for (int q = 0; q < P; q++) B.h_evals[f0 + q].active = 0;  // not part of the init pass
+        } else {
+            any_new = true;
+        }
+    }
+    CHK(B.d_pso.upload(pb->h_pso));
+    CHK(B.d_evals.upload(B.h_evals));
+    if (any_new) CHK(pso_first_eval(*pb));
+    CHK(pso_capture(*pb));
+    sv->pb = std::move(pb);
+    return PASTA_OK;
+}
+
+static int server_retire(pasta_server* sv) {
+    Batch& B = sv->pb->B;
+    std::vector<DevPso> ps(B.n_stars);
+    HIPCHK(hipMemcpy(ps.data(), B.d_pso.p, sizeof(DevPso) * B.n_stars, hipMemcpyDeviceToHost));
+    std::vector<int> done_idx;
+    for (int s = 0; s < B.n_stars; s++)
+        if (ps[s].done || ps[s].status) done_idx.push_back(s);
+    if (done_idx.empty()) return PASTA_OK;
+    CHK(server_save_state(sv));
+    // continuum parameters at the best labels of the finished stars
+    std::vector<pasta_star*> fs;
+    std::vector<double> lab;
+    for (int s : done_idx) {
+        fs.push_back(sv->running[s]->star);
+        for (int d = 0; d < 5; d++) lab.push_back(ps[s].gbest[d]);
+    }
+    Batch F;
+    CHK(batch_build(F, sv->g, fs, std::vector<int>(fs.size(), 1), 0));
+    std::vector<double> c(fs.size());
+    std::vector<int> st(fs.size());
+    CHK(batch_eval(F, lab.data(), c.data(), st.data()));
+    std::vector<float> pf((size_t)std::max<int64_t>(F.pf_len, 1));
+    HIPCHK(hipMemcpy(pf.data(), F.d_pf.p, sizeof(float) * pf.size(), hipMemcpyDeviceToHost));
+    auto now = std::chrono::steady_clock::now();
+    std::vector<Request*> keep;
+    for (int s = 0, k = 0; s < B.n_stars; s++) {
+        Request* r = sv->running[s];
+        if (k < (int)done_idx.size() && done_idx[k] == s) {
+            pasta_fit_result& res = r->res;
+            std::memset(&res, 0, sizeof(res));
+            res.status = ps[s].status;
+            res.iterations = r->st.max_iters;
+            res.chi2 = ps[s].gcost;
+            std::memcpy(res.label, ps[s].gbest, 40);
+            res.time_s = std::chrono::duration<double>(now - r->t_submit).count();
+            int np = std::min(512, std::max(1, r->star->n_params));
+            res.n_params = np;
+            for (int q = 0; q < np; q++)
+                res.continuum_params[q] =
+                    r->star->cont_kind == CONT_FIXED ? 0.0f : pf[F.h_evals[k].pf_off + q];
+            r->state = 2;
+            sv->in_use -= r->bytes;
+            sv->finished++;
+            k++;
+        } else {
+            keep.push_back(r);
+        }
+    }
+    sv->running.swap(keep);
+    sv->dirty = true;
+    return PASTA_OK;
+}
+
+int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
+    if (!sv) return fail(PASTA_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    std::lock_guard<std::mutex> lg(sv->g->mu);
+    // admission: free slots and HBM budget
+    while (!sv->waiting.empty() && (int)sv->running.size() < sv->cfg.max_stars) {
+        Request* r = sv->waiting.front();
+        if (r->state == 3) { sv->waiting.pop_front(); continue; }
+        if (sv->in_use + r->bytes > sv->budget && !sv->running.empty()) break;
+        if (!sv->dirty) CHK(server_save_state(sv));
+        sv->waiting.pop_front();
+        r->state = 1;
+        sv->in_use += r->bytes;
+        sv->running.push_back(r);
+        sv->dirty = true;
+    }
+    if (sv->dirty) {
+        CHK(server_rebuild(sv));
+        sv->dirty = false;
+    }
+    if (sv->pb) {
+        Batch& B = sv->pb->B;
+        for (int it = 0; it < sv->cfg.iters_per_step; it++)
+            HIPCHK(hipGraphLaunch(B.graph, B.stream));
+        HIPCHK(hipStreamSynchronize(B.stream));
+        sv->iters_run += sv->cfg.iters_per_step;
+        sv->evals_done += (int64_t)B.n_evals * sv->cfg.iters_per_step;
+        CHK(server_retire(sv));
+        if (sv->dirty && sv->running.empty()) { sv->pb.reset(); sv->dirty = false; }
+    }
+    if (active) *active = (int)sv->running.size();
+    if (waiting) *waiting = (int)sv->waiting.size();
+    return PASTA_OK;
+}
+
+int pasta_server_poll(pasta_server* sv, uint64_t id, int* done, pasta_fit_result* out) {
+    if (!sv || !done) return fail(PASTA_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) return fail(PASTA_ERR_STATE, "unknown request");
+    Request* r = it->second.get();
+    if (r->state == 3) return fail(PASTA_ERR_STATE, "request was cancelled");
+    *done = r->state == 2;
+    if (*done && out) *out = r->res;
+    return PASTA_OK;
+}
+
+int pasta_server_cancel(pasta_server* sv, uint64_t id) {
+    if (!sv) return fail(PASTA_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) return fail(PASTA_ERR_STATE, "unknown request");
+    Request* r = it->second.get();
+    if (r->state == 0) { r->state = 3; return PASTA_OK; }
+    if (r->state == 1) {
+        CHK(server_save_state(sv));
+        sv->running.erase(std::find(sv->running.begin(), sv->running.end(), r));
+        sv->in_use -= r->bytes;
+        r->state = 3;
+        sv->dirty = true;
+        return PASTA_OK;
+    }
+    return fail(PASTA_ERR_STATE, "request already finished");
+}
+
+int pasta_server_stats(const pasta_server* sv, int64_t out[4]) {
+    if (!sv) return fail(PASTA_ERR_ARG, "null server");
+    out[0] = sv->evals_done;
+    out[1] = sv->finished;
+    out[2] = sv->iters_run;
+    out[3] = sv->in_use;
+    return PASTA_OK;
+}

@@ -1,0 +1,771 @@
+// kernels.hip -- gfx950 kernels of the model-evaluation path.
+//
+// One chi2 evaluation (reference: CostFunction::cost, src/fitting.rs:313-326) is
+//   k_prep     neighbour indices + 64 Lagrange weights, rotation taps, RV shift
+//              (src/cubic.rs:232-243, src/interpolate.rs:424-533,
+//               src/convolve_rv.rs:438-475,532-567)
+//   k_model    per (evaluation, 1792-pixel synthetic tile), all in LDS:
+//              interpolate (src/interpolate.rs:930-983, incl. the n%128 tail
+//              quirk) -> rotation convolution (zero-padded "same",
+//              src/convolve_rv.rs:484-529) -> instrumental convolution
+//              (src/convolve_rv.rs:160-211 / 405-431) -> RV shift + linear
+//              resample with the f32-quantised weight (src/convolve_rv.rs:569-595)
+//   k_moments  per-chunk Aᵀ(f/m) in f64 (src/continuum_fitting.rs:216-226)
+//   k_solve    per (evaluation, chunk) f64 Cholesky solve of the normal equations
+//   k_chi2     blended-polynomial chi2 terms, f64 accumulation
+//              (src/continuum_fitting.rs:280-360, 69-95, 442-500)
+//   k_final    fixed-order sum of the chi2 partials
+// All reductions use fixed trees, so results are bit-reproducible run to run
+// and independent of which other evaluations share the launch.
+//
+// Built with -ffp-contract=off: only the explicit fmaf() calls fuse.
+#include "engine.h"
+
+namespace pasta {
+
+constexpr int ILDS = TILE + 1 + 2 * HD_MAX + (KROT_MAX - 1) + 19;  // 4112
+constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 16;                   // 2064
+constexpr double C_KMS = 299792.0;   // src/convolve_rv.rs:440,547
+constexpr double LN10 = 2.302585092994045684017991454684364208;
+
+__device__ __forceinline__ long long jfloor(double x) {
+    double f = floor(x);
+    return f > 0.0 ? (long long)f : 0;  // `as usize` saturates at 0
+}
+
+// Workgroup i is dispatched to XCD i % 8: give each XCD a contiguous range of
+// logical blocks so neighbouring tiles (shared halos, shared neighbour rows)
+// meet in the same L2.
+__device__ __forceinline__ int xcd_remap(int i, int n) {
+    int q = n >> 3, r = n & 7;
+    int x = i & 7, k = i >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ int rotation_len(double vsini, double dvelo) {
+    const double eps = 0.6;
+    double vrot = vsini / C_KMS;
+    double a = 4.0 * (eps - 1.0) / (M_PI * eps);
+    double v = ceil(2.0 * vrot / dvelo * sqrt(1.0 + a * a));
+    long long nme = (v >= 0.0) ? (long long)v + 2 : 2;
+    if (v > 1e9) return 1 << 30;
+    if (nme < 1) nme = 1;
+    return (int)(nme + 1 - nme % 2);
+}
+
+// ---------------------------------------------------------------------------
+// k_prep: one wavefront per evaluation
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restrict__ stars,
+                                             const double* __restrict__ labels,
+                                             DevEval* __restrict__ evals, float* __restrict__ rot,
+                                             int mode) {
+    const int e = blockIdx.x;
+    const int lane = threadIdx.x;
+    DevEval& ev = evals[e];
+    __shared__ int s_K;
+    __shared__ float s_sum;
+    const double* lab = labels + 5 * (size_t)e;
+    const double vsini = lab[3];
+    const double dvelo = LN10 * G.sg_step;
+    if (lane == 0) {
+        int st = ST_OK, krot = 0;
+        if (!ev.active) {
+            st = -1;
+        } else {
+            const DevStar& S = stars[ev.star];
+            st = local_weights(*G.geom, lab[0], lab[1], lab[2], ev.idx, ev.w);
+            for (int j = 0; j < 64; j++)
+                ev.f[j] = (ev.idx[j] >= 0 && G.factors) ? G.factors[ev.idx[j]] : 1.0f / 65535.0f;
+            if (!st && vsini < 0.0) st = ST_VSINI;
+            if (!st) {
+                int K = rotation_len(vsini, dvelo);
+                if (vsini == 0.0 || K <= 2) krot = 0;
+                else if (K > KROT_MAX) st = ST_KERNEL;
+                else krot = K;
+            }
+            if (!st && mode == 1) {  // synthetic-grid output: no RV shift / resample
+                ev.shift_log = 0.0;
+                ev.cmax = G.W;
+                ev.tile_base = 0;
+            } else if (!st) {
+                double shift = 1.0 - lab[4] / C_KMS;
+                if (S.wl0 * shift < G.synth_first_wl) st = ST_RANGE;
+                else if (S.wlN * shift > G.synth_last_wl) st = ST_RANGE;
+                double sl = log10(shift) / G.sg_step;
+                ev.shift_log = sl;
+                int cmax = G.W;
+                if (S.disp_kind == DISP_NONE && krot > 0) {
+                    long long step = KROT_MAX - (krot - 1);
+                    long long nx = ((G.W + step - 1) / step) * step;
+                    cmax = (int)(nx + krot - 1 - krot / 2);
+                }
+                ev.cmax = cmax;
+                long long jl = jfloor(S.fidx[S.N - 1] + sl);
+                if (!st && jl + 1 >= cmax) st = ST_INDEX;
+                ev.tile_base = (int)(jfloor(S.fidx[0] + sl) / TILE);
+            }
+        }
+        ev.status = st;
+        ev.krot = krot;
+        s_K = (st == ST_OK) ? krot : 0;
+    }
+    __syncthreads();
+    const int K = s_K;
+    if (K > 0) {
+        float* h = rot + (size_t)e * KROT_MAX;
+        const double eps = 0.6;
+        const double vrot = vsini / C_KMS;
+        const double mid = ((double)(K - 1) * dvelo) / 2.0;
+        for (int i = lane; i < K; i += 64) {
+            double vk = (double)i * dvelo - mid;
+            double q = vk / vrot;
+            double y = 1.0 - q * q;
+            float k = 0.0f;
+            if (y > 0.0)
+                k = (float)((2.0 * (1.0 - eps) * sqrt(y) + M_PI * eps / 2.0 * y) /
+                            (M_PI * vrot * (1.0 - eps / 3.0)));
+            h[i] = k;
+        }
+        __syncthreads();
+        if (lane == 0) {  // f32 sequential sum, as kernel.iter().sum() (convolve_rv.rs:470)
+            float s = 0.0f;
+            for (int i = 0; i < K; i++) s = s + h[i];
+            s_sum = s;
+        }
+        __syncthreads();
+        const float s = s_sum;
+        for (int i = lane; i < K; i += 64) h[i] = h[i] / s;
+    }
+}
+
+// first observed pixel whose resample index floor(fidx + shift) >= t
+__device__ __forceinline__ int obs_lower_bound(const double* __restrict__ fidx, int N, double sl,
+                                               long long t) {
+    if (t <= 0) return 0;
+    int lo = 0, hi = N;
+    const double tt = (double)t;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (fidx[mid] + sl >= tt) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void unpack8(const uint4 v, float u[8]) {
+    u[0] = (float)(v.x & 0xffffu); u[1] = (float)(v.x >> 16);
+    u[2] = (float)(v.y & 0xffffu); u[3] = (float)(v.y >> 16);
+    u[4] = (float)(v.z & 0xffffu); u[5] = (float)(v.z >> 16);
+    u[6] = (float)(v.w & 0xffffu); u[7] = (float)(v.w >> 16);
+}
+
+// Interpolated synthetic flux on [base, base+cnt) into dst (zero outside [0, W)).
+// Groups of 8 pixels aligned to 8 in absolute pixel coordinates, 16-byte loads,
+// two groups per thread in flight.
+__device__ __forceinline__ void interp_tile(const DevGrid& G, const int* sidx, const float* sw,
+                                            const float* sf, long long base, int cnt, float* dst) {
+    const long long g0 = (base >= 0) ? (base & ~7LL) : -(((-base) + 7) & ~7LL);
+    const int ngroups = (int)((base + cnt - g0 + 7) >> 3);
+    for (int gg = threadIdx.x; gg < ngroups; gg += 2 * NT) {
+        const long long xa = g0 + 8LL * gg;
+        const long long xb = xa + 8LL * NT;
+        const bool ha = xa >= 0 && xa < G.W;
+        const bool hb = (gg + NT) < ngroups && xb >= 0 && xb < G.W;
+        const bool fa = ha && xa + 7 < G.Wq;  // all 8 pixels before the tail
+        const bool fb = hb && xb + 7 < G.Wq;
+        float aa[8], ab[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) { aa[t] = 0.0f; ab[t] = 0.0f; }
+        for (int j = 0; j < 64; j++) {
+            const int id = sidx[j];
+            if (id < 0) continue;
+            const float w = sw[j], f = sf[j];
+            const uint16_t* row = G.spectra + (long long)id * G.pitch;
+            if (ha) {
+                float u[8];
+                unpack8(*(const uint4*)(row + xa), u);
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    float M = (fa || xa + t < G.Wq) ? u[t] * f : u[t] / 65535.0f * f;
+                    aa[t] = fmaf(M, w, aa[t]);
+                }
+            }
+            if (hb) {
+                float u[8];
+                unpack8(*(const uint4*)(row + xb), u);
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    float M = (fb || xb + t < G.Wq) ? u[t] * f : u[t] / 65535.0f * f;
+                    ab[t] = fmaf(M, w, ab[t]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            long long x = xa + t;
+            if (x >= base && x < base + cnt) dst[x - base] = (x >= 0 && x < G.W) ? aa[t] : 0.0f;
+            x = xb + t;
+            if ((gg + NT) < ngroups && x >= base && x < base + cnt)
+                dst[x - base] = (x >= 0 && x < G.W) ? ab[t] : 0.0f;
+        }
+    }
+}
+
+// out[q] = sum_u tr[u] * in[q + u] for q in [0, n); 8 outputs per thread with a
+// sliding register window.  tr is zero-padded to a multiple of 8 and `in` has
+// 16 finite values beyond the last one read for a valid output.
+__device__ __forceinline__ void conv_tile(const float* in, const float* tr, int K, int n,
+                                          float* out, bool mask, long long x0, int W) {
+    const int q0 = threadIdx.x * 8;
+    if (q0 >= n) return;
+    float acc[8], win[16];
+#pragma unroll
+    for (int t = 0; t < 8; t++) { acc[t] = 0.0f; win[t] = in[q0 + t]; }
+    for (int u0 = 0; u0 < K; u0 += 8) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) win[8 + t] = in[q0 + u0 + 8 + t];
+#pragma unroll
+        for (int uu = 0; uu < 8; uu++) {
+            const float h = tr[u0 + uu];
+#pragma unroll
+            for (int t = 0; t < 8; t++) acc[t] = fmaf(h, win[t + uu], acc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++) win[t] = win[8 + t];
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        int q = q0 + t;
+        if (q < n) {
+            long long x = x0 + q;
+            out[q] = (mask && (x < 0 || x >= W)) ? 0.0f : acc[t];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_model: one workgroup per (evaluation, synthetic tile)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restrict__ stars,
+                                              const DevEval* __restrict__ evals,
+                                              const float* __restrict__ rot,
+                                              const int2* __restrict__ blocks,
+                                              float* __restrict__ model, int mode) {
+    __shared__ float Ib[ILDS];
+    __shared__ float Rb[RLDS];
+    __shared__ float hrev[KROT_MAX + 8];
+    __shared__ float grev[2 * HD_MAX + 1 + 8];
+    __shared__ int s_idx[64];
+    __shared__ float s_w[64], s_f[64];
+    __shared__ int s_rng[2];
+
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int2 bt = blocks[b];
+    const DevEval& ev = evals[bt.x];
+    if (ev.status != ST_OK) return;
+    const DevStar& S = stars[ev.star];
+    const int tid = threadIdx.x;
+    const double sl = ev.shift_log;
+    const long long t0 = (long long)(ev.tile_base + bt.y) * TILE;
+    if (tid == 0 && mode == 0) {
+        s_rng[0] = obs_lower_bound(S.fidx, S.N, sl, t0);
+        s_rng[1] = obs_lower_bound(S.fidx, S.N, sl, t0 + TILE);
+    }
+    if (tid < 64) { s_idx[tid] = ev.idx[tid]; s_w[tid] = ev.w[tid]; s_f[tid] = ev.f[tid]; }
+    __syncthreads();
+    int ilo = 0, ihi = 0;
+    long long clo, chi;
+    if (mode == 0) {
+        ilo = s_rng[0];
+        ihi = s_rng[1];
+        if (ilo >= ihi) return;
+        clo = jfloor(S.fidx[ilo] + sl);
+        chi = jfloor(S.fidx[ihi - 1] + sl) + 1;
+    } else {
+        if (t0 >= G.W) return;
+        clo = t0;
+        chi = (t0 + TILE < G.W ? t0 + TILE : G.W) - 1;
+    }
+    const int nC = (int)(chi - clo + 1);
+    const int hd = (S.disp_kind == DISP_NONE) ? 0 : (S.Kd - 1) / 2;
+    const int K = ev.krot;
+    const int hr = K > 0 ? K / 2 : 0;
+    const long long rlo = clo - hd;
+    const int nR = nC + 2 * hd;
+    const int nI = nR + 2 * hr;
+    const bool mask = S.disp_kind != DISP_NONE;
+
+    if (K > 0) {
+        const float* h = rot + (size_t)bt.x * KROT_MAX;
+        const int Kp = (K + 7) & ~7;
+        for (int i = tid; i < Kp; i += NT) hrev[i] = i < K ? h[K - 1 - i] : 0.0f;
+    }
+    if (S.disp_kind == DISP_FIXED) {
+        const int Kp = (S.Kd + 7) & ~7;
+        for (int i = tid; i < Kp; i += NT) grev[i] = i < S.Kd ? S.dkern[S.Kd - 1 - i] : 0.0f;
+    }
+
+    // stage 1: interpolation (pass-through rotation writes straight into R)
+    if (K > 0) {
+        interp_tile(G, s_idx, s_w, s_f, rlo - hr, nI, Ib);
+        if (tid < 16) Ib[nI + tid] = 0.0f;
+    } else {
+        interp_tile(G, s_idx, s_w, s_f, rlo, nR, Rb);
+    }
+    if (tid < 16) Rb[nR + tid] = 0.0f;
+    __syncthreads();
+
+    // stage 2: rotational broadening
+    if (K > 0) {
+        conv_tile(Ib, hrev, K, nR, Rb, mask, rlo, G.W);
+        __syncthreads();
+    }
+
+    // stage 3: instrumental broadening -> C (reuses Ib)
+    const float* Cb = Rb;
+    if (S.disp_kind == DISP_FIXED) {
+        conv_tile(Rb, grev, S.Kd, nC, Ib, false, 0, 0);
+        Cb = Ib;
+        __syncthreads();
+    } else if (S.disp_kind == DISP_VARIABLE) {
+        const int Kd = S.Kd;
+        for (int q = tid; q < nC; q += NT) {
+            const long long x = clo + q;
+            float c;
+            if (x < hd || x >= G.W - hd) {
+                c = Rb[q + hd];
+            } else {
+                const float* kr = S.dkern + x * Kd;
+                float a = 0.0f;
+                for (int u = 0; u < Kd; u++) a = fmaf(kr[u], Rb[q + u], a);
+                c = a;
+            }
+            Ib[q] = c;
+        }
+        Cb = Ib;
+        __syncthreads();
+    }
+
+    // stage 4: RV shift + resample (f32-quantised weight, convolve_rv.rs:569-573)
+    float* out = model + ev.m_off;
+    if (mode == 1) {
+        for (int q = tid; q < nC; q += NT) out[clo + q] = Cb[q];
+        return;
+    }
+    for (int i = ilo + tid; i < ihi; i += NT) {
+        const double idx = S.fidx[i] + sl;
+        const long long j = jfloor(idx);
+        const float w = (float)idx - (float)j;
+        const float a = Cb[j - clo], c = Cb[j + 1 - clo];
+        out[i] = (1.0f - w) * a + w * c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sums acc[0..n) over the workgroup; result valid in thread 0's out[].
+template <int PM>
+__device__ __forceinline__ void block_sum(double (&acc)[PM], int n, double* red /* 4*PM */) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < PM; k++) {
+        if (k < n) {
+            double v = wave_sum(acc[k]);
+            if (lane == 0) red[wv * PM + k] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < PM; k++)
+            if (k < n) acc[k] = ((red[k] + red[PM + k]) + red[2 * PM + k]) + red[3 * PM + k];
+    }
+}
+
+// f32 design-matrix row of a chunk: A[i,k] = f32(powi(xhat, k)), with the
+// squares shared between k exactly as __powidf2 forms them.
+template <int PM>
+__device__ __forceinline__ void design_row(double xh, int p1, float (&a)[PM]) {
+    double sq[5];
+    sq[0] = xh;
+#pragma unroll
+    for (int t = 1; t < 5; t++) sq[t] = sq[t - 1] * sq[t - 1];
+#pragma unroll
+    for (int k = 0; k < PM; k++) {
+        if (k < p1) {
+            double r = 1.0;
+#pragma unroll
+            for (int t = 0; t < 5; t++)
+                if (k & (1 << t)) r *= sq[t];
+            a[k] = (float)r;
+        }
+    }
+}
+
+// nalgebra 0.32 dot order: 8 partial accumulators, pairwise merge, scalar tail.
+template <int PM>
+__device__ __forceinline__ float dot_na(const float (&a)[PM], const float* b, int n) {
+    float c[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) c[t] = 0.0f;
+    const int nb = (n >> 3) << 3;
+#pragma unroll
+    for (int k = 0; k < PM; k++)
+        if (k < nb) c[k & 7] = c[k & 7] + a[k] * b[k];
+    float r = 0.0f;
+    r = r + (c[0] + c[4]);
+    r = r + (c[1] + c[5]);
+    r = r + (c[2] + c[6]);
+    r = r + (c[3] + c[7]);
+#pragma unroll
+    for (int k = 0; k < PM; k++)
+        if (k >= nb && k < n) r = r + a[k] * b[k];
+    return r;
+}
+
+template <int PM>
+__global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ stars,
+                                                const DevEval* __restrict__ evals,
+                                                const int2* __restrict__ blocks,
+                                                const float* __restrict__ model,
+                                                double* __restrict__ mpart) {
+    __shared__ double red[4 * PM];
+    const int2 bt = blocks[blockIdx.x];
+    const DevEval& ev = evals[bt.x];
+    if (ev.status != ST_OK) return;
+    const DevStar& S = stars[ev.star];
+    const int4 mb = S.mblocks[bt.y];
+    const float* m = model + ev.m_off;
+    double acc[PM];
+#pragma unroll
+    for (int k = 0; k < PM; k++) acc[k] = 0.0;
+    int np = 1;
+    if (S.cont_kind == CONT_CHUNK) {
+        np = S.p1;
+        const double lo = S.chunk_lohi[2 * mb.x], hi = S.chunk_lohi[2 * mb.x + 1];
+        for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
+            const double y = (double)(S.flux[i] / m[i]);
+            float a[PM];
+            design_row<PM>(chunk_xhat(S.wl[i], lo, hi), np, a);
+#pragma unroll
+            for (int k = 0; k < PM; k++)
+                if (k < np) acc[k] = fma((double)a[k], y, acc[k]);
+        }
+    } else if (S.cont_kind == CONT_LINEAR) {
+        np = S.p1;
+        for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
+            const double y = (double)(S.flux[i] / m[i]);
+#pragma unroll
+            for (int k = 0; k < PM; k++)
+                if (k < np) acc[k] = fma((double)S.design[(size_t)k * S.N + i], y, acc[k]);
+        }
+    } else {  // constant: sum of f/m
+        for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) acc[0] += (double)(S.flux[i] / m[i]);
+    }
+    block_sum<PM>(acc, np, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < PM; k++)
+            if (k < np) mpart[ev.mp_off + (size_t)bt.y * np + k] = acc[k];
+    }
+}
+
+// one thread per (evaluation, chunk)
+__global__ void k_solve(const DevStar* __restrict__ stars, const DevEval* __restrict__ evals,
+                        const int2* __restrict__ items, int n_items,
+                        const double* __restrict__ mpart, float* __restrict__ pf) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_items) return;
+    const int2 it = items[t];
+    DevEval& ev = const_cast<DevEval&>(evals[it.x]);
+    if (ev.status != ST_OK) return;
+    const DevStar& S = stars[ev.star];
+    if (S.cont_kind == CONT_CONSTANT) {
+        double s = 0.0;
+        for (int mb = 0; mb < S.n_mblocks; mb++) s += mpart[ev.mp_off + mb];
+        pf[ev.pf_off] = (float)s / (float)S.N;  // sum / n as FluxFloat (continuum_fitting.rs:492)
+        return;
+    }
+    const int c = it.y, p1 = S.p1;
+    int mb0, mb1;
+    if (S.cont_kind == CONT_CHUNK) { mb0 = S.mb_off[c]; mb1 = S.mb_off[c + 1]; }
+    else { mb0 = 0; mb1 = S.n_mblocks; }
+    double bvec[P1_MAX], z[P1_MAX];
+    for (int k = 0; k < p1; k++) {
+        double s = 0.0;
+        for (int mb = mb0; mb < mb1; mb++) s += mpart[ev.mp_off + (size_t)mb * p1 + k];
+        bvec[k] = s;
+    }
+    const double* L = S.chol + (size_t)c * p1 * p1;
+    for (int i = 0; i < p1; i++) {  // L z = b
+        double s = bvec[i];
+        for (int j = 0; j < i; j++) s -= L[i * p1 + j] * z[j];
+        z[i] = s / L[i * p1 + i];
+    }
+    for (int i = p1 - 1; i >= 0; i--) {  // Lᵀ x = z
+        double s = z[i];
+        for (int j = i + 1; j < p1; j++) s -= L[j * p1 + i] * bvec[j];
+        bvec[i] = s / L[i * p1 + i];
+    }
+    for (int k = 0; k < p1; k++) pf[ev.pf_off + (size_t)c * p1 + k] = (float)bvec[k];
+}
+
+template <int PM>
+__global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
+                                             const DevEval* __restrict__ evals,
+                                             const int2* __restrict__ blocks,
+                                             const float* __restrict__ model,
+                                             const float* __restrict__ pf,
+                                             double* __restrict__ cpart) {
+    __shared__ double red[4];
+    __shared__ float spf[2 * PM];
+    const int2 bt = blocks[blockIdx.x];
+    const DevEval& ev = evals[bt.x];
+    if (ev.status != ST_OK) return;
+    const DevStar& S = stars[ev.star];
+    const int4 cb = S.cblocks[bt.y];  // (kind, chunk, i0, i1)
+    const float* m = model + ev.m_off;
+    const float* P = pf + ev.pf_off;
+    double acc[1] = {0.0};
+    if (S.cont_kind == CONT_CHUNK) {
+        const int p1 = S.p1, c = cb.y;
+        if (threadIdx.x < p1) {
+            spf[threadIdx.x] = P[c * p1 + threadIdx.x];
+            spf[PM + threadIdx.x] = c > 0 ? P[(c - 1) * p1 + threadIdx.x] : 0.0f;
+        }
+        __syncthreads();
+        const double lo = S.chunk_lohi[2 * c], hi = S.chunk_lohi[2 * c + 1];
+        const double plo = c > 0 ? S.chunk_lohi[2 * c - 2] : 0.0;
+        const double phi = c > 0 ? S.chunk_lohi[2 * c - 1] : 1.0;
+        for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
+            float a[PM];
+            design_row<PM>(chunk_xhat(S.wl[i], lo, hi), p1, a);
+            float p = dot_na<PM>(a, spf, p1);
+            if (cb.x == 1) {
+                float ap[PM];
+                design_row<PM>(chunk_xhat(S.wl[i], plo, phi), p1, ap);
+                const float prev = dot_na<PM>(ap, spf + PM, p1);
+                const float fac = S.fac[i];
+                p = prev * fac + p * (1.0f - fac);
+            }
+            const float r = S.flux[i] - p * m[i];
+            acc[0] += (double)((r * r) / S.var[i]);
+        }
+    } else if (S.cont_kind == CONT_LINEAR) {
+        const int p1 = S.p1;
+        if (threadIdx.x < p1) spf[threadIdx.x] = P[threadIdx.x];
+        __syncthreads();
+        for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
+            const float bq = S.flux[i] / m[i];
+            float y = 0.0f;
+#pragma unroll
+            for (int k = 0; k < PM; k++) {
+                if (k < p1) {
+                    const float t = S.design[(size_t)k * S.N + i] * spf[k];
+                    y = (k == 0) ? t : t + y;
+                }
+            }
+            const float r = y - bq;
+            acc[0] += (double)((r / S.var[i]) * r);
+        }
+    } else if (S.cont_kind == CONT_FIXED) {
+        for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
+            const float r = S.flux[i] - m[i] * S.fixedc[i];
+            acc[0] += (double)((r / S.var[i]) * r);
+        }
+    } else {
+        const float mean = P[0];
+        for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
+            const float r = S.flux[i] - m[i] * mean;
+            acc[0] += (double)((r / S.var[i]) * r);
+        }
+    }
+    block_sum<1>(acc, 1, red);
+    if (threadIdx.x == 0) cpart[ev.cp_off + bt.y] = acc[0];
+}
+
+__global__ void k_final(const DevStar* __restrict__ stars, const DevEval* __restrict__ evals,
+                        int n, const double* __restrict__ cpart, double* __restrict__ chi2,
+                        int* __restrict__ status) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const DevEval& ev = evals[e];
+    if (status) status[e] = ev.status;
+    if (ev.status != ST_OK) {
+        if (ev.status > 0) chi2[e] = __longlong_as_double(0x7ff8000000000000LL);
+        return;
+    }
+    const DevStar& S = stars[ev.star];
+    double s = 0.0;
+    for (int q = 0; q < S.n_cblocks; q++) s += cpart[ev.cp_off + q];
+    chi2[e] = s / S.chi2_div;
+}
+
+// ---------------------------------------------------------------------------
+// device-side PSO (batched path)
+// ---------------------------------------------------------------------------
+// thread per particle: velocity/position update (particleswarm.rs:234-276)
+__global__ void k_pso_update(const GridGeom* __restrict__ geom, DevPso* __restrict__ pso,
+                             DevEval* __restrict__ evals, int n, double* __restrict__ pos,
+                             double* __restrict__ vel, const double* __restrict__ bpos) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    DevEval& ev = evals[e];
+    DevPso& ps = pso[ev.star];
+    if (ps.done || ps.status) { ev.active = 0; return; }
+    ev.active = 1;
+    const int q = e - ps.first_eval;
+    double gb[5];
+    for (int d = 0; d < 5; d++) gb[d] = ps.gbest[d];
+    int rc = pso_update_particle(*geom, ps.b, ps.s, ps.key_iter, ps.iter, ps.P, q, pos + 5 * (size_t)e,
+                                 vel + 5 * (size_t)e, bpos + 5 * (size_t)e, gb);
+    if (rc) atomicCAS(&ps.status, 0, rc);
+}
+
+// thread per star: personal / global best update (particleswarm.rs:283-298)
+__global__ void k_pso_best(DevPso* __restrict__ pso, int n_stars, const DevEval* __restrict__ evals,
+                           const double* __restrict__ pos, double* __restrict__ bpos,
+                           double* __restrict__ bcost, const double* __restrict__ cost,
+                           double* __restrict__ trace) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_stars) return;
+    DevPso& ps = pso[s];
+    if (ps.done || ps.status) return;
+    for (int q = 0; q < ps.P; q++) {
+        const int e = ps.first_eval + q;
+        if (evals[e].status != ST_OK) { ps.status = evals[e].status; return; }
+    }
+    for (int q = 0; q < ps.P; q++) {
+        const int e = ps.first_eval + q;
+        const double c = cost[e];
+        if (c < bcost[e]) {
+            for (int d = 0; d < 5; d++) bpos[5 * e + d] = pos[5 * e + d];
+            bcost[e] = c;
+            if (c < ps.gcost) {
+                for (int d = 0; d < 5; d++) ps.gbest[d] = pos[5 * e + d];
+                ps.gcost = c;
+            }
+        }
+    }
+    if (trace) {
+        double* tr = trace + ((size_t)s * (ps.max_iters + 1) + ps.iter) * 6;
+        tr[0] = ps.gcost;
+        for (int d = 0; d < 5; d++) tr[1 + d] = ps.gbest[d];
+    }
+    ps.iter += 1;
+    if (ps.iter > ps.max_iters) ps.done = 1;
+}
+
+// thread per star after the first evaluation: stable sort by cost, bests
+// (particleswarm.rs:117-131, 209-215)
+__global__ void k_pso_init(DevPso* __restrict__ pso, int n_stars, const DevEval* __restrict__ evals,
+                           double* __restrict__ pos, double* __restrict__ vel,
+                           double* __restrict__ bpos, double* __restrict__ bcost,
+                           double* __restrict__ cost, double* __restrict__ trace) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_stars) return;
+    DevPso& ps = pso[s];
+    if (ps.done || ps.status || ps.iter != 0) return;  // only stars not started yet
+    const int f0 = ps.first_eval, P = ps.P;
+    for (int q = 0; q < P; q++)
+        if (evals[f0 + q].status != ST_OK) { ps.status = evals[f0 + q].status; return; }
+    for (int i = 1; i < P; i++) {  // stable insertion sort
+        double c = cost[f0 + i], p[5], v[5];
+        for (int d = 0; d < 5; d++) { p[d] = pos[5 * (f0 + i) + d]; v[d] = vel[5 * (f0 + i) + d]; }
+        int j = i - 1;
+        while (j >= 0 && cost[f0 + j] > c) {
+            cost[f0 + j + 1] = cost[f0 + j];
+            for (int d = 0; d < 5; d++) {
+                pos[5 * (f0 + j + 1) + d] = pos[5 * (f0 + j) + d];
+                vel[5 * (f0 + j + 1) + d] = vel[5 * (f0 + j) + d];
+            }
+            j--;
+        }
+        cost[f0 + j + 1] = c;
+        for (int d = 0; d < 5; d++) { pos[5 * (f0 + j + 1) + d] = p[d]; vel[5 * (f0 + j + 1) + d] = v[d]; }
+    }
+    for (int q = 0; q < P; q++) {
+        for (int d = 0; d < 5; d++) bpos[5 * (f0 + q) + d] = pos[5 * (f0 + q) + d];
+        bcost[f0 + q] = cost[f0 + q];
+    }
+    ps.gcost = cost[f0];
+    for (int d = 0; d < 5; d++) ps.gbest[d] = pos[5 * f0 + d];
+    if (trace) {
+        double* tr = trace + (size_t)s * (ps.max_iters + 1) * 6;
+        tr[0] = ps.gcost;
+        for (int d = 0; d < 5; d++) tr[1 + d] = ps.gbest[d];
+    }
+    ps.iter = 1;
+    if (ps.max_iters < 1) ps.done = 1;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (called from the driver in api.hip)
+// ---------------------------------------------------------------------------
+hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
+                       const double* labels, float* rot, const int2* mblk_model, int n_model,
+                       const int2* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
+                       const int2* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
+                       double* cpart, double* chi2, int* status, int pmax, int mode,
+                       hipStream_t st) {
+    if (n_evals <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64), 0, st, G, stars, labels, evals, rot, mode);
+    if (n_model > 0)
+        hipLaunchKernelGGL(k_model, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot,
+                           mblk_model, model, mode);
+    if (n_mom > 0) {
+        if (pmax <= 12)
+            hipLaunchKernelGGL(k_moments<12>, dim3(n_mom), dim3(NT), 0, st, stars, evals,
+                               mblk_mom, model, mpart);
+        else
+            hipLaunchKernelGGL(k_moments<P1_MAX>, dim3(n_mom), dim3(NT), 0, st, stars, evals,
+                               mblk_mom, model, mpart);
+    }
+    if (n_solve > 0)
+        hipLaunchKernelGGL(k_solve, dim3((n_solve + 127) / 128), dim3(128), 0, st, stars, evals,
+                           solve_items, n_solve, mpart, pf);
+    if (n_chi2 > 0) {
+        if (pmax <= 12)
+            hipLaunchKernelGGL(k_chi2<12>, dim3(n_chi2), dim3(NT), 0, st, stars, evals, mblk_chi2,
+                               model, pf, cpart);
+        else
+            hipLaunchKernelGGL(k_chi2<P1_MAX>, dim3(n_chi2), dim3(NT), 0, st, stars, evals,
+                               mblk_chi2, model, pf, cpart);
+    }
+    hipLaunchKernelGGL(k_final, dim3((n_evals + 127) / 128), dim3(128), 0, st, stars, evals,
+                       n_evals, cpart, chi2, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, int n, double* pos,
+                             double* vel, const double* bpos, hipStream_t st) {
+    hipLaunchKernelGGL(k_pso_update, dim3((n + 127) / 128), dim3(128), 0, st, geom, pso, evals, n,
+                       pos, vel, bpos);
+    return hipGetLastError();
+}
+
+hipError_t launch_pso_best(DevPso* pso, int n_stars, const DevEval* evals, const double* pos,
+                           double* bpos, double* bcost, const double* cost, double* trace,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_pso_best, dim3((n_stars + 63) / 64), dim3(64), 0, st, pso, n_stars, evals,
+                       pos, bpos, bcost, cost, trace);
+    return hipGetLastError();
+}
+
+hipError_t launch_pso_init(DevPso* pso, int n_stars, const DevEval* evals, double* pos, double* vel,
+                           double* bpos, double* bcost, double* cost, double* trace,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_pso_init, dim3((n_stars + 63) / 64), dim3(64), 0, st, pso, n_stars, evals,
+                       pos, vel, bpos, bcost, cost, trace);
+    return hipGetLastError();
+}
+
+}  // namespace pasta

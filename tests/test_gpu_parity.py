@@ -1,0 +1,153 @@
+"""GPU parity: the HIP path against the C oracle on identical inputs (small config).
+
+Tolerance: model spectra and chi2 within 1e-5 relative in FP32 (the bar the
+north star sets).  The oracle restates the reference's f32 arithmetic; the HIP
+path differs in summation order (direct convolution vs the reference's FFT,
+fused multiply-adds, f64 reductions), which the tolerance absorbs.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-5
+
+
+def _labels_inside(spec, n, seed, vsini=(0.0, 300.0), rv=(-150.0, 150.0)):
+    from pasta_amd import synth
+    return synth.random_labels(spec, n, seed, vsini=vsini, rv=rv)
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)))
+
+
+def test_interpolate_matches_oracle(small, gpu_small):
+    pasta, wlg, interp = gpu_small
+    rng = np.random.default_rng(11)
+    spec = small.spec
+    labs = [
+        (5000.0, -1.0, 3.5), (7000.0, 0.5, 5.0), (6000.0, 0.0, 4.0),  # nodes and corners
+        (5010.0, -0.99, 3.51), (6990.0, 0.49, 4.99),                    # quadratic edges
+        (5250.0, -0.75, 4.5), (6875.3, 0.31, 3.77),
+    ]
+    labs += [tuple(x) for x in np.stack([rng.uniform(5000, 7000, 40), rng.uniform(-1, 0.5, 40),
+                                         rng.uniform(3.5, 5.0, 40)], 1)]
+    print(f"interpolate: {len(labs)} labels, W={spec.n_pix}", flush=True)
+    out, st = interp.interpolate_bulk(np.array(labs))
+    assert not st.any()
+    for k, lab in enumerate(labs):
+        ref = small.og.interpolate(*lab)
+        assert _rel(out[k], ref) < RTOL, lab
+        # tail quirk: the last W % 128 pixels carry the extra 1/65535
+        tail = spec.n_pix - (spec.n_pix // 128) * 128
+        assert np.all(out[k][-tail:] < 1e-3)
+
+
+def test_interpolate_out_of_grid_errors(gpu_small):
+    pasta, wlg, interp = gpu_small
+    out, st = interp.interpolate_bulk(np.array([[4999.0, 0.0, 4.0], [6000.0, 0.6, 4.0],
+                                                [6000.0, 0.0, 5.01]]))
+    assert (st == 1).all()
+    with pytest.raises(ValueError):
+        interp.interpolate(4000.0, 0.0, 4.0)
+
+
+@pytest.mark.parametrize("disp_kind", ["fixed", "none", "variable"])
+def test_produce_model_matches_oracle(small, gpu_small, disp_kind):
+    pasta, wlg, interp = gpu_small
+    wl = small.wl
+    if disp_kind == "fixed":
+        d = pasta.FixedResolutionDispersion(wl, 85_000.0, wlg)
+        od = small.odisp
+    elif disp_kind == "none":
+        d = pasta.NoConvolutionDispersion(wl, wlg)
+        od = O.Dispersion(small.og, O.DISP_NONE, wl)
+    else:
+        disp = (wl / np.interp(wl, [wl[0], wl[-1]], [60_000.0, 90_000.0])).astype(np.float32)
+        d = pasta.VariableResolutionDispersion(wl, disp, wlg)
+        od = O.Dispersion(small.og, O.DISP_VARIABLE, wl, disp=disp)
+    labs = np.array([[6123.0, -0.3, 4.21, 10.0, 20.0], [6123.0, -0.3, 4.21, 0.0, 0.0],
+                     [5300.0, 0.2, 3.9, 300.0, -150.0], [6800.0, -0.9, 4.9, 55.5, 149.0],
+                     [5000.0, -1.0, 3.5, 1.0, -3.0]])
+    labs = np.concatenate([labs, _labels_inside(small.spec, 12, 5)])
+    print(f"produce_model[{disp_kind}]: {len(labs)} labels", flush=True)
+    out = interp.produce_model_bulk(d, labs)
+    for k, lab in enumerate(labs):
+        ref = O.produce_model(small.og, od, lab)
+        assert _rel(out[k], ref) < RTOL, (disp_kind, lab.tolist(), _rel(out[k], ref))
+
+
+def test_interpolate_and_convolve_matches_oracle(small, gpu_small):
+    pasta, wlg, interp = gpu_small
+    d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
+    for lab in ([6123.0, -0.3, 4.21, 10.0], [5500.0, 0.1, 4.6, 250.0], [6500.0, -0.5, 3.6, 0.0]):
+        out = interp.interpolate_and_convolve(d, *lab)
+        ref = O.interpolate_and_convolve(small.og, small.odisp, lab)
+        # the segment's first/last pixels see only half a kernel: compare relative to max
+        assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) < RTOL, lab
+
+
+def _cont_pair(small, pasta, kind):
+    n = len(small.wl)
+    if kind == "chunk":
+        return (pasta.ChunkContinuumFitter(small.wl, 5, 4, 0.2), small.ocont)
+    if kind == "chunk10x8":
+        return (pasta.ChunkContinuumFitter(small.wl, 10, 8, 0.2),
+                O.Continuum("chunk", n, wl=small.wl, n_chunks=10, p_order=8, overlap=0.2))
+    if kind == "linear":
+        from pasta_amd import synth
+        dm = synth.smooth_design(n, 12, 7)
+        return pasta.LinearModelContinuumFitter(dm), O.Continuum("linear", n, design=dm)
+    if kind == "fixed":
+        x = np.linspace(-1, 1, n)
+        c = (1e4 * (1 + 0.05 * x)).astype(np.float32)
+        return pasta.FixedContinuum(c, 30), O.Continuum("fixed", n, fixed=c, ignore=30)
+    return pasta.ConstantContinuum(), O.Continuum("constant", n)
+
+
+@pytest.mark.parametrize("kind", ["chunk", "chunk10x8", "linear", "fixed", "constant"])
+def test_chi2_matches_oracle(small, gpu_small, kind):
+    pasta, wlg, interp = gpu_small
+    cont, ocont = _cont_pair(small, pasta, kind)
+    d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
+    labs = np.concatenate([[small.true, [6000.0, 0.0, 4.0, 30.0, -10.0]],
+                           _labels_inside(small.spec, 30, 9)])
+    print(f"chi2[{kind}]: {len(labs)} labels", flush=True)
+    star = pasta.Star(interp, d, cont, small.flux, small.var)
+    chi2, st, params = star.chi2(labs)
+    assert not st.any()
+    for k, lab in enumerate(labs):
+        ref, rp = O.cost(small.og, small.odisp, ocont, small.flux, small.var, lab)
+        assert abs(chi2[k] - ref) / ref < RTOL, (kind, lab.tolist(), chi2[k], ref)
+        if kind in ("chunk", "chunk10x8", "linear", "constant"):
+            np.testing.assert_allclose(params[k][: len(rp)], rp, rtol=2e-4, atol=1e-3 * np.abs(rp).max())
+
+
+def test_chi2_errors_are_nan(small, gpu_small):
+    pasta, wlg, interp = gpu_small
+    d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
+    fitter = interp.get_fitter(d, pasta.ChunkContinuumFitter(small.wl, 5, 4, 0.2),
+                               pasta.PSOSettings(4, 1), (0, 300), (-150, 150))
+    bad = [[4000.0, 0.0, 4.0, 10.0, 0.0], [6000.0, 0.0, 4.0, -1.0, 0.0],
+           [6000.0, 0.0, 4.0, 10.0, 5000.0], small.true.tolist()]
+    out = fitter.chi2(interp, small.flux, small.var, bad)
+    assert np.isnan(out[:3]).all() and np.isfinite(out[3])
+    with pytest.raises(ValueError):
+        fitter.chi2(interp, small.flux, small.var, bad, allow_nan=False)
+
+
+def test_results_are_deterministic(small, gpu_small):
+    pasta, wlg, interp = gpu_small
+    d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
+    star = pasta.Star(interp, d, pasta.ChunkContinuumFitter(small.wl, 5, 4, 0.2), small.flux,
+                      small.var)
+    labs = _labels_inside(small.spec, 24, 3)
+    a, _, _ = star.chi2(labs)
+    b, _, _ = star.chi2(labs)
+    c, _, _ = star.chi2(labs[::-1].copy())
+    assert np.array_equal(a, b)
+    assert np.array_equal(a, c[::-1])
