@@ -559,17 +559,24 @@ class InMemSingleFitter:
         out._trace = trace
         return out
 
-    def fit_bulk(self, interpolator, observed_fluxes, observed_vars, constraints=None, seeds=None):
+    def fit_bulk(self, interpolator, observed_fluxes, observed_vars, constraints=None, seeds=None,
+                 trace=False):
         """Many stars in one device-driven batched PSO (src/lib.rs:1511-1540)."""
         stars = [self._star(interpolator, f, v) for f, v in zip(observed_fluxes, observed_vars)]
         n = len(stars)
         s = _settings_c(self.settings, self.vsini_range, self.rv_range, constraints)
-        seeds = np.arange(n, dtype=np.uint64) if seeds is None else np.asarray(seeds, np.uint64)
+        seeds = np.arange(n, dtype=np.uint64) if seeds is None else np.ascontiguousarray(seeds, np.uint64)
         arr = (C.c_void_p * n)(*[st._h for st in stars])
         res = (_lib.FitResult * n)()
-        rc = lib.pasta_fit_bulk(arr, n, C.byref(s), ptr(seeds, C.c_uint64), res, None)
+        tr = np.zeros((n, self.settings.max_iters + 1, 6)) if trace else None
+        rc = lib.pasta_fit_bulk(arr, n, C.byref(s), ptr(seeds, C.c_uint64), res,
+                                None if tr is None else ptr(tr, C.c_double))
         _raise(rc, "fit_bulk")
-        return [OptimizationResult._from_c(r) for r in res]
+        out = [OptimizationResult._from_c(r) for r in res]
+        if trace:
+            for k, o in enumerate(out):
+                o._trace = tr[k]
+        return out
 
     def chi2(self, interpolator, observed_flux, observed_var, labels, allow_nan=None):
         """chi2 at each label vector (src/lib.rs:1603-1627): NaN on failure unless allow_nan=False."""

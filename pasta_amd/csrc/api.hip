@@ -469,8 +469,13 @@ static int cholesky(std::vector<double>& A, int n) {  // in place, lower
     return 0;
 }
 
+// chi2 blocks: (kind, chunk, i0, i1)
 static void push_blocks(std::vector<int4>& out, int kind, int chunk, int i0, int i1) {
     for (int s = i0; s < i1; s += BLK) out.push_back(make_int4(kind, chunk, s, std::min(s + BLK, i1)));
+}
+// moment blocks: (chunk, i0, i1, 0)
+static void push_mblocks(std::vector<int4>& out, int chunk, int i0, int i1) {
+    for (int s = i0; s < i1; s += BLK) out.push_back(make_int4(chunk, s, std::min(s + BLK, i1), 0));
 }
 
 static void host_design_row(double xh, int p1, float* a) {
@@ -540,10 +545,9 @@ int pasta_cont_create_chunk(const double* wl, int n, int n_chunks, int p_order, 
     // moment blocks per chunk
     c->mb_off.push_back(0);
     for (int ch = 0; ch < n_chunks; ch++) {
-        push_blocks(c->mblocks, ch, 0, c->startstop[2 * ch], c->startstop[2 * ch + 1]);
+        push_mblocks(c->mblocks, ch, c->startstop[2 * ch], c->startstop[2 * ch + 1]);
         c->mb_off.push_back((int)c->mblocks.size());
     }
-    for (auto& b : c->mblocks) { b.x = b.y; b.y = b.z; b.z = b.w; b.w = 0; }  // (chunk, i0, i1, 0)
     // chi2 segments (src/continuum_fitting.rs:300-358)
     const int* ss = c->startstop.data();
     if (ss[2] < 50) return fail(PASTA_ERR_ARG, "first chunk shorter than the 50-pixel trim");
@@ -582,8 +586,7 @@ int pasta_cont_create_linear(const float* design, int n, int ncols, pasta_cont**
         for (int l = k + 1; l < ncols; l++) G[k * ncols + l] = G[l * ncols + k];
     if (cholesky(G, ncols)) return fail(PASTA_ERR_ARG, "design matrix is rank deficient");
     c->chol = G;
-    push_blocks(c->mblocks, 0, 0, 0, n);
-    for (auto& b : c->mblocks) { b.x = 0; b.y = b.z; b.z = b.w; b.w = 0; }
+    push_mblocks(c->mblocks, 0, 0, n);
     c->mb_off = {0, (int)c->mblocks.size()};
     push_blocks(c->cblocks, 0, 0, 50, n - 50);
     c->chi2_div = (double)n;
@@ -610,8 +613,7 @@ int pasta_cont_create_constant(int n, pasta_cont** out) {
     auto c = std::make_unique<pasta_cont>();
     c->kind = CONT_CONSTANT;
     c->n = n;
-    push_blocks(c->mblocks, 0, 0, 0, n);
-    for (auto& b : c->mblocks) { b.x = 0; b.y = b.z; b.z = b.w; b.w = 0; }
+    push_mblocks(c->mblocks, 0, 0, n);
     c->mb_off = {0, (int)c->mblocks.size()};
     push_blocks(c->cblocks, 0, 0, 0, n);
     c->chi2_div = (double)n;
