@@ -1,0 +1,275 @@
+#!/usr/bin/env python3
+"""bench.py -- chi2 evaluations per second of the batched, served PSO path on one MI355X.
+
+Workload (synthetic data, seeded): the full-size grid (Teff 4000-10000/250 x [M/H]
+-1..0.5/0.1 x logg 3-5/0.1 = 8,400 models of 200,000 uint16 pixels over 3800-9000 A,
+5,000 lines, 3.36 GB resident in HBM) and 512 stars, each with its own log-uniform
+sampling of 40,000-200,000 pixels, labels uniform in the grid, vsini U(0,300),
+RV U(-150,150), SNR U(30,300), R = 85,000, a 10-chunk degree-8 continuum; 64 PSO
+particles per star.  One step = one PSO iteration of every star (32,768 chi2
+evaluations) through the server's hipGraph-captured iteration.
+
+Also measured in the same run: the plain one-fit-at-a-time GPU path (one star,
+150,000 pixels, 100 particles, 200 iterations, host PSO) and the CPU baseline:
+the C oracle (a restatement of the reference algorithm) on the host cores, over a
+bounded sample of evaluations of that star.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+R_INSTR = 85_000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_stars(pasta, synth, interp, wlg, spec, n_stars, seed=4000):
+    rng = np.random.default_rng(seed)
+    stars, meta = [], []
+    for s in range(n_stars):
+        n = int(rng.integers(40_000, 200_001))
+        wl = synth.observed_wavelengths(spec, n, 600)
+        lab = [rng.uniform(4000, 10000), rng.uniform(-1.0, 0.5), rng.uniform(3.0, 5.0),
+               rng.uniform(0, 300), rng.uniform(-150, 150)]
+        snr = rng.uniform(30, 300)
+        disp = pasta.FixedResolutionDispersion(wl, R_INSTR, wlg)
+        model = interp.produce_model(disp, *lab)
+        flux, var = synth.observe(model, wl, snr, seed + 1 + s)
+        cont = pasta.ChunkContinuumFitter(wl, 10, 8, 0.2)
+        stars.append(pasta.Star(interp, disp, cont, flux, var))
+        fidx = (np.log10(wl) - spec.log10_first) / spec.log10_step
+        span = int(np.floor(fidx[-1]) - np.floor(fidx[0]) + 2)
+        meta.append(dict(n=n, span=span, wl=wl, flux=flux, var=var, label=lab))
+    return stars, meta
+
+
+def settings(pasta, P, iters):
+    st = pasta.api._settings_c(pasta.PSOSettings(P, iters), (0.0, 300.0), (-150.0, 150.0), None)
+    return st
+
+
+def run_batched(pasta, _lib, interp, stars, meta, P, warmup, steps):
+    cfg = _lib.ServerConfig(len(stars), P, 1, 0, 0)
+    sv = C.c_void_p()
+    _lib.check(_lib.lib.pasta_server_create(interp._h, C.byref(cfg), C.byref(sv)), "server")
+    n_timed_kernel = 3
+    st = settings(pasta, P, warmup + steps + n_timed_kernel + 3)
+    try:
+        for k, s in enumerate(stars):
+            rid = C.c_uint64()
+            _lib.check(_lib.lib.pasta_server_submit(sv, s.release(), C.byref(st), 5000 + k,
+                                                    C.byref(rid)), "submit")
+        a, w = C.c_int(), C.c_int()
+        t0 = time.perf_counter()
+        _lib.check(_lib.lib.pasta_server_step(sv, C.byref(a), C.byref(w)), "step")
+        log(f"[bench] admitted {a.value} stars ({w.value} waiting), first step "
+            f"{time.perf_counter() - t0:.2f} s")
+        for _ in range(warmup):
+            _lib.check(_lib.lib.pasta_server_step(sv, C.byref(a), C.byref(w)), "step")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _lib.check(_lib.lib.pasta_server_step(sv, C.byref(a), C.byref(w)), "step")
+        elapsed = time.perf_counter() - t0
+        assert a.value == len(stars), "a star left the batch during the timed steps"
+        # k_model's launch duration: the same iterations as eager launches on the
+        # server's stream, bracketed by HIP events
+        tm = (C.c_double * 5)()
+        _lib.check(_lib.lib.pasta_server_set_timing(sv, 1), "timing")
+        _lib.lib.pasta_server_timing(sv, 1, tm)
+        for _ in range(n_timed_kernel):
+            _lib.check(_lib.lib.pasta_server_step(sv, C.byref(a), C.byref(w)), "step")
+        _lib.lib.pasta_server_timing(sv, 0, tm)
+        return elapsed, tm[1] / tm[4], tm[3] / tm[4]
+    finally:
+        _lib.lib.pasta_server_destroy(sv)
+
+
+def run_plain(pasta, synth, interp, wlg, spec, iters=200, P=100, seed=2000):
+    """one star, 150,000 pixels, 10 chunks of degree 8, 100 particles."""
+    rng = np.random.default_rng(seed)
+    wl = synth.observed_wavelengths(spec, 150_000, 600)
+    lab = [rng.uniform(4500, 9500), rng.uniform(-0.9, 0.4), rng.uniform(3.2, 4.8),
+           rng.uniform(0, 300), rng.uniform(-150, 150)]
+    disp = pasta.FixedResolutionDispersion(wl, R_INSTR, wlg)
+    model = interp.produce_model(disp, *lab)
+    flux, var = synth.observe(model, wl, rng.uniform(50, 200), seed + 1)
+    cont = pasta.ChunkContinuumFitter(wl, 10, 8, 0.2)
+    warm = interp.get_fitter(disp, cont, pasta.PSOSettings(P, 2), (0.0, 300.0), (-150.0, 150.0))
+    warm.fit(interp, flux, var, seed=1)  # warm-up (code objects, allocations)
+    fitter = interp.get_fitter(disp, cont, pasta.PSOSettings(P, iters), (0.0, 300.0), (-150.0, 150.0))
+    t0 = time.perf_counter()
+    res = fitter.fit(interp, flux, var, seed=3000)
+    dt = time.perf_counter() - t0
+    n_evals = P * (iters + 1) + 1
+    return dict(evals_per_s=n_evals / dt, fit_latency_s=dt, label=res.label.as_list(), true=lab,
+                chi2=res.chi2, wl=wl, flux=flux, var=var)
+
+
+def cpu_baseline(interp, spec, plain, budget_s):
+    """The oracle on the host cores over a bounded sample of the plain star's evaluations."""
+    from oracle import oracle as O
+    from pasta_amd import synth
+    import pasta_amd as pasta
+
+    t0 = time.perf_counter()
+    spectra = synth.read_grid(interp)
+    og = O.Grid(spec.labels(), spectra, None, spec.log10_first, spec.log10_step)
+    od = O.Dispersion(og, O.DISP_FIXED, plain["wl"], resolution=R_INSTR)
+    oc = O.Continuum("chunk", len(plain["wl"]), wl=plain["wl"], n_chunks=10, p_order=8, overlap=0.2)
+    log(f"[bench] oracle set-up {time.perf_counter() - t0:.1f} s")
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    rng = np.random.default_rng(77)
+    labels = np.stack([rng.uniform(4200, 9800, 4096), rng.uniform(-0.95, 0.45, 4096),
+                       rng.uniform(3.1, 4.9, 4096), rng.uniform(0, 300, 4096),
+                       rng.uniform(-150, 150, 4096)], axis=1)
+    t0 = time.perf_counter()
+    first = O.cost_batch(og, od, oc, plain["flux"], plain["var"], labels[:threads], threads)
+    t_first = time.perf_counter() - t0
+    n = int(max(threads, min(4096 - threads, threads * round(budget_s / max(t_first, 1e-3)))))
+    t0 = time.perf_counter()
+    rest = O.cost_batch(og, od, oc, plain["flux"], plain["var"], labels[threads:threads + n], threads)
+    dt = time.perf_counter() - t0
+    # full-size parity on the same sample: GPU chi2 vs the oracle's
+    disp = pasta.FixedResolutionDispersion(plain["wl"], R_INSTR, pasta.WlGrid(
+        spec.log10_first, spec.log10_step, spec.n_pix, log=True))
+    star = pasta.Star(interp, disp, pasta.ChunkContinuumFitter(plain["wl"], 10, 8, 0.2),
+                      plain["flux"], plain["var"])
+    lab_all = labels[: threads + n]
+    gpu, st, _ = star.chi2(lab_all)
+    ref = np.concatenate([first, rest])
+    ok = st == 0
+    rel = float(np.max(np.abs(gpu[ok] - ref[ok]) / np.abs(ref[ok])))
+    return dict(value=n / dt, cores=threads, n=n, seconds=dt, parity_max_rel=rel, n_parity=int(ok.sum()))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--stars", type=int, default=512)
+    ap.add_argument("--particles", type=int, default=64)
+    ap.add_argument("--plain-iters", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="also run one plain copy of this many GiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(stars would shard across GPUs with no exchange); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import pasta_amd as pasta
+    from pasta_amd import _lib, synth
+
+    t_all = time.perf_counter()
+    spec = synth.full_grid_spec()
+    wlg = pasta.WlGrid(spec.log10_first, spec.log10_step, spec.n_pix, log=True)
+    t0 = time.perf_counter()
+    interp = pasta.InMemInterpolator.empty(spec.labels(), wlg)
+    synth.fill_grid_gpu(interp, spec)
+    log(f"[bench] grid {interp.n_models} x {interp.n_pix} filled in HBM in {time.perf_counter() - t0:.1f} s")
+    if args.copy_probe_gib > 0:
+        synth.copy_probe(int(args.copy_probe_gib * (1 << 30)))
+
+    t0 = time.perf_counter()
+    stars, meta = make_stars(pasta, synth, interp, wlg, spec, args.stars)
+    log(f"[bench] {len(stars)} stars set up in {time.perf_counter() - t0:.1f} s "
+        f"(N {min(m['n'] for m in meta)}..{max(m['n'] for m in meta)})")
+
+    elapsed, model_ms, iter_ms = run_batched(pasta, _lib, interp, stars, meta, args.particles,
+                                             args.warmup, args.steps)
+    n_evals_step = args.particles * len(stars)
+    value = n_evals_step * args.steps / elapsed
+    bytes_launch = float(sum(args.particles * (64 * 2 * m["span"] + 8 * m["n"]) for m in meta))
+    achieved = bytes_launch / (model_ms * 1e-3) / 1e9
+    log(f"[bench] batched: {value:,.0f} evals/s, {elapsed / args.steps * 1e3:.1f} ms/step, "
+        f"k_model {model_ms:.2f} ms ({achieved:,.0f} GB/s algorithmic), iteration {iter_ms:.2f} ms")
+
+    plain = None
+    if not args.no_plain:
+        plain = run_plain(pasta, synth, interp, wlg, spec, iters=args.plain_iters)
+        log(f"[bench] plain: {plain['evals_per_s']:,.0f} evals/s, fit {plain['fit_latency_s']:.2f} s, "
+            f"label {np.round(plain['label'], 3).tolist()} true {np.round(plain['true'], 3).tolist()}")
+
+    cpu = None
+    if not args.no_cpu_baseline and plain is not None:
+        cpu = cpu_baseline(interp, spec, plain, args.cpu_seconds)
+        log(f"[bench] cpu oracle: {cpu['value']:.1f} evals/s on {cpu['cores']} threads "
+            f"({cpu['n']} evals, {cpu['seconds']:.1f} s); GPU-vs-oracle chi2 max rel "
+            f"{cpu['parity_max_rel']:.2e} over {cpu['n_parity']} full-size evaluations")
+
+    out = {
+        "metric": "chi2 evals/s (batched PSO)",
+        "value": round(value, 1),
+        "unit": "chi2 evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded grid, spectra and noise)",
+        "config": {
+            "workload": "batched PSO: 512 stars x 64 particles, full grid 8400 x 200000 u16, "
+                        "N U(40k,200k), R=85000, 10 chunks deg 8",
+            "stars": len(stars),
+            "particles": args.particles,
+            "evals_per_step": n_evals_step,
+            "grid": [25, 16, 21, 200_000],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_model",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_launch,
+            "kernel_ms": round(model_ms, 3),
+            "iteration_ms": round(iter_ms, 3),
+        },
+    }
+    if plain is not None:
+        out["plain_gpu"] = {
+            "workload": "one star, N=150000, 100 particles x 200 iterations, host PSO",
+            "evals_per_s": round(plain["evals_per_s"], 1),
+            "fit_latency_s": round(plain["fit_latency_s"], 3),
+            "batched_over_plain": round(value / plain["evals_per_s"], 2),
+        }
+        out["batched_fit_latency_s_est"] = round(201 * elapsed / args.steps, 3)
+    if cpu is not None:
+        out["cpu_baseline"] = {
+            "value": round(cpu["value"], 2),
+            "unit": "chi2 evals/s",
+            "cores": cpu["cores"],
+            "kind": "port",
+            "sample": f"{cpu['n']} chi2 evaluations of the N=150000 star on the full grid "
+                      f"(C oracle, {cpu['cores']} threads)",
+        }
+        out["parity_full_size"] = {"max_rel_chi2": cpu["parity_max_rel"], "n": cpu["n_parity"]}
+    log(f"[bench] total {time.perf_counter() - t_all:.1f} s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

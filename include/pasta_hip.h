@@ -68,6 +68,8 @@ int pasta_grid_axes(const pasta_grid *g, double *teff, double *m, double *logg, 
 /* Device rows in storage order (list_gridpoints): pointer to row 0 and the row
  * pitch in elements, so a device-side producer can fill the grid in place. */
 int pasta_grid_device_rows(const pasta_grid *g, uint16_t **rows, int64_t *pitch);
+/* Copy rows [row0, row0+nrows) (storage order, n_pix each) back to the host. */
+int pasta_grid_read_rows(const pasta_grid *g, int row0, int nrows, uint16_t *out);
 /* Upload per-row factors after creation (storage order), or NULL to clear. */
 int pasta_grid_set_factors(pasta_grid *g, const float *factors_storage_order);
 /* get_local_grid + calculate_interpolation_coefficients (src/interpolate.rs:424-533,
@@ -200,6 +202,7 @@ typedef struct pasta_server_config {
     int max_particles;        /* particles per star (upper bound) */
     int iters_per_step;       /* PSO iterations per pasta_server_step */
     int64_t hbm_budget_bytes; /* 0 = use free memory minus 4 GiB */
+    int timing;               /* 1 = run iterations as eager launches timed with HIP events */
 } pasta_server_config;
 int pasta_server_create(pasta_grid *g, const pasta_server_config *cfg, pasta_server **out);
 int pasta_server_destroy(pasta_server *sv);
@@ -215,6 +218,13 @@ int pasta_server_cancel(pasta_server *sv, uint64_t request_id);
 /* counters: [0] evals done, [1] requests finished, [2] iterations run,
  * [3] HBM bytes in use by in-flight stars */
 int pasta_server_stats(const pasta_server *sv, int64_t out[4]);
+/* With timing on: [0] last iteration's k_model ms, [1] sum of k_model ms, [2] last
+ * iteration ms (PSO update through best update), [3] sum of iteration ms,
+ * [4] iterations summed.  reset != 0 clears the sums after reading. */
+int pasta_server_timing(pasta_server *sv, int reset, double out[5]);
+/* Switch between graph replays (0, the default) and eager launches timed with
+ * HIP events (1); both run the same kernels on the server's stream. */
+int pasta_server_set_timing(pasta_server *sv, int on);
 
 #ifdef __cplusplus
 }

@@ -97,6 +97,8 @@ _sig = {
     "or_produce_model_f64": (C.c_int, [C.POINTER(_Model), C.c_void_p, _dp, _dp]),
     "or_interpolate_and_convolve": (C.c_int, [C.POINTER(_Model), C.c_void_p, _dp, _fp]),
     "or_cost": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, _dp, _fp]),
+    "or_cost_batch": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, C.c_int,
+                                C.c_int, _dp]),
     "or_cost_f64": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, _dp]),
     "or_rng_u64": (C.c_uint64, [C.c_uint64, C.c_uint64]),
     "or_rng_uniform": (C.c_double, [C.c_uint64, C.c_uint64]),
@@ -425,3 +427,14 @@ def rng_uniform(key, ctr):
 
 def rng_u64(key, ctr):
     return _L.or_rng_u64(key, ctr)
+
+
+def cost_batch(grid, disp, cont, flux, var, labels, n_threads=1):
+    """chi2 of many label vectors on n_threads threads (errors raise)."""
+    lab = _d(labels).reshape(-1, 5)
+    flux, var = _f(flux), _f(var)
+    M = grid.model()
+    out = np.zeros(len(lab))
+    _chk(_L.or_cost_batch(C.byref(M), disp._h, cont._h, _p(flux, C.c_float), _p(var, C.c_float),
+                          _p(lab, C.c_double), len(lab), n_threads, _p(out, C.c_double)), "cost_batch")
+    return out

@@ -1619,3 +1619,11 @@ int or_fit(const or_model *M, const or_disp *d, const or_cont *c, const float *f
     if (n_evals) *n_evals += 1;
     return rc;
 }
+
+/* CostFunction::cost over many label vectors on n_threads threads (the
+ * reference's rayon par_iter over particles, src/fitting.rs:328-330). */
+int or_cost_batch(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                  const float *var, const double *labels, int n, int n_threads, double *chi2) {
+    fit_ctx fc = {M, d, c, flux, var, n_threads};
+    return model_cost_fn(&fc, n, labels, chi2);
+}

@@ -153,6 +153,8 @@ int or_interpolate_and_convolve(const or_model *M, const or_disp *d, const doubl
                                 float *out /* W */);
 int or_cost(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
             const float *var, const double lab[5], double *chi2, float *params);
+int or_cost_batch(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                  const float *var, const double *labels, int n, int n_threads, double *chi2);
 int or_cost_f64(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
                 const float *var, const double lab[5], double *chi2);
 

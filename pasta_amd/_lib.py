@@ -1,7 +1,7 @@
 """ctypes binding of libpasta_hip.so (include/pasta_hip.h).
 
 The library is the only compute path: if it is missing, importing fails loudly
-(build it with `python -m pasta_amd.build`).
+(build it with `python pasta_amd/build.py`).
 """
 import ctypes as C
 import os
@@ -63,6 +63,7 @@ class ServerConfig(C.Structure):
         ("max_particles", C.c_int),
         ("iters_per_step", C.c_int),
         ("hbm_budget_bytes", C.c_int64),
+        ("timing", C.c_int),
     ]
 
 
@@ -118,13 +119,16 @@ SIGNATURES = {
     "pasta_server_poll": [_P, C.c_uint64, _ip, C.POINTER(FitResult)],
     "pasta_server_cancel": [_P, C.c_uint64],
     "pasta_server_stats": [_P, _i64p],
+    "pasta_server_timing": [_P, C.c_int, _dp],
+    "pasta_server_set_timing": [_P, C.c_int],
+    "pasta_grid_read_rows": [_P, C.c_int, C.c_int, _u16p],
 }
 
 
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -m pasta_amd.build` "
+            f"{LIB_PATH} is missing: build it with `python pasta_amd/build.py` "
             "(there is no CPU fallback)"
         )
     lib = C.CDLL(LIB_PATH)

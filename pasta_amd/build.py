@@ -1,7 +1,7 @@
 """Builds libpasta_hip.so (gfx950) in-tree with hipcc.
 
-    python -m pasta_amd.build            # library only
-    python -m pasta_amd.build --all      # library + CPU oracle (test infrastructure)
+    python pasta_amd/build.py            # libraries
+    python pasta_amd/build.py --all      # + the CPU oracle (test infrastructure)
 
 Flags: -ffp-contract=off keeps every f32/f64 expression rounding as written
 (only explicit fmaf/fma fuse); packed FP32 instructions are disabled for the
@@ -41,6 +41,19 @@ def build_lib(verbose=True):
     return LIB
 
 
+SYNTH_LIB = os.path.join(HERE, "libpasta_synth.so")
+
+
+def build_synth(verbose=True):
+    tmp = SYNTH_LIB + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-o", tmp, os.path.join(CSRC, "synth.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, SYNTH_LIB)
+    return SYNTH_LIB
+
+
 def build_oracle(verbose=True):
     cmd = ["make", "-s", "-C", os.path.join(ROOT, "oracle")]
     if verbose:
@@ -50,6 +63,7 @@ def build_oracle(verbose=True):
 
 def main(argv):
     build_lib()
+    build_synth()
     if "--all" in argv:
         build_oracle()
 

@@ -31,7 +31,7 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
                        const int2* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
                        const int2* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int pmax, int mode,
-                       hipStream_t st);
+                       hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st);
 hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, int n, double* pos,
                              double* vel, const double* bpos, hipStream_t st);
 hipError_t launch_pso_best(DevPso* pso, int n_stars, const DevEval* evals, const double* pos,
@@ -289,6 +289,15 @@ int pasta_grid_device_rows(const pasta_grid* g, uint16_t** rows, int64_t* pitch)
     if (!g) return fail(PASTA_ERR_ARG, "null grid");
     *rows = g->d_spectra.p;
     *pitch = g->pitch;
+    return PASTA_OK;
+}
+
+int pasta_grid_read_rows(const pasta_grid* g, int row0, int nrows, uint16_t* out) {
+    if (!g || !out || row0 < 0 || nrows < 0 || row0 + nrows > g->geom.n_models)
+        return fail(PASTA_ERR_ARG, "bad row range");
+    HIPCHK(hipMemcpy2D(out, (size_t)g->W * sizeof(uint16_t),
+                       g->d_spectra.p + (size_t)row0 * g->pitch, (size_t)g->pitch * sizeof(uint16_t),
+                       (size_t)g->W * sizeof(uint16_t), (size_t)nrows, hipMemcpyDeviceToHost));
     return PASTA_OK;
 }
 
@@ -817,8 +826,12 @@ struct Batch {
     DBuf<double> d_vel, d_bpos, d_bcost, d_trace;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // model begin/end, iteration begin/end
+    bool eager_events = false;  // record ev[] around eager (non-graph) launches
     ~Batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -901,7 +914,8 @@ static int batch_launch(Batch& B) {
     HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p, B.d_bm.p,
                        B.n_bm, B.d_bmom.p, B.n_bmom, B.d_solve.p, B.n_solve, B.d_bchi2.p,
                        B.n_bchi2, B.d_model.p, B.d_mpart.p, B.d_pf.p, B.d_cpart.p, B.d_chi2.p,
-                       B.d_status.p, B.pmax, B.mode, B.stream));
+                       B.d_status.p, B.pmax, B.mode, B.eager_events ? B.ev[0] : nullptr,
+                       B.eager_events ? B.ev[1] : nullptr, B.stream));
     return PASTA_OK;
 }
 
@@ -1256,11 +1270,13 @@ static int pso_first_eval(PsoBatch& PB) {
 
 static int pso_iteration_launches(PsoBatch& PB) {
     Batch& B = PB.B;
+    if (B.eager_events) HIPCHK(hipEventRecord(B.ev[2], B.stream));
     HIPCHK(launch_pso_update(B.grid->d_geom.p, B.d_pso.p, B.d_evals.p, B.n_evals, B.d_labels.p,
                              B.d_vel.p, B.d_bpos.p, B.stream));
     CHK(batch_launch(B));
     HIPCHK(launch_pso_best(B.d_pso.p, B.n_stars, B.d_evals.p, B.d_labels.p, B.d_bpos.p,
                            B.d_bcost.p, B.d_chi2.p, B.d_trace.p, B.stream));
+    if (B.eager_events) HIPCHK(hipEventRecord(B.ev[3], B.stream));
     return PASTA_OK;
 }
 
@@ -1377,6 +1393,7 @@ struct pasta_server {
     std::unique_ptr<PsoBatch> pb;
     bool dirty = false;
     int64_t evals_done = 0, finished = 0, iters_run = 0;
+    double t_model_last = 0, t_model_sum = 0, t_iter_last = 0, t_iter_sum = 0, t_n = 0;
     std::mutex mu;
     ~pasta_server() {
         pb.reset();
@@ -1490,6 +1507,7 @@ static int server_rebuild(pasta_server* sv) {
     CHK(B.d_pso.upload(pb->h_pso));
     CHK(B.d_evals.upload(B.h_evals));
     if (any_new) CHK(pso_first_eval(*pb));
+    for (auto& e : B.ev) HIPCHK(hipEventCreate(&e));
     CHK(pso_capture(*pb));
     sv->pb = std::move(pb);
     return PASTA_OK;
@@ -1570,8 +1588,25 @@ int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
     }
     if (sv->pb) {
         Batch& B = sv->pb->B;
-        for (int it = 0; it < sv->cfg.iters_per_step; it++)
-            HIPCHK(hipGraphLaunch(B.graph, B.stream));
+        for (int it = 0; it < sv->cfg.iters_per_step; it++) {
+            if (!sv->cfg.timing) {
+                HIPCHK(hipGraphLaunch(B.graph, B.stream));
+            } else {  // the same launches without the graph, bracketed by HIP events
+                B.eager_events = true;
+                int rc = pso_iteration_launches(*sv->pb);
+                B.eager_events = false;
+                CHK(rc);
+                HIPCHK(hipStreamSynchronize(B.stream));
+                float m = 0.0f, t = 0.0f;
+                HIPCHK(hipEventElapsedTime(&m, B.ev[0], B.ev[1]));
+                HIPCHK(hipEventElapsedTime(&t, B.ev[2], B.ev[3]));
+                sv->t_model_last = m;
+                sv->t_model_sum += m;
+                sv->t_iter_last = t;
+                sv->t_iter_sum += t;
+                sv->t_n += 1;
+            }
+        }
         HIPCHK(hipStreamSynchronize(B.stream));
         sv->iters_run += sv->cfg.iters_per_step;
         sv->evals_done += (int64_t)B.n_evals * sv->cfg.iters_per_step;
@@ -1619,5 +1654,24 @@ int pasta_server_stats(const pasta_server* sv, int64_t out[4]) {
     out[1] = sv->finished;
     out[2] = sv->iters_run;
     out[3] = sv->in_use;
+    return PASTA_OK;
+}
+
+int pasta_server_timing(pasta_server* sv, int reset, double out[5]) {
+    if (!sv || !out) return fail(PASTA_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    out[0] = sv->t_model_last;
+    out[1] = sv->t_model_sum;
+    out[2] = sv->t_iter_last;
+    out[3] = sv->t_iter_sum;
+    out[4] = sv->t_n;
+    if (reset) sv->t_model_sum = sv->t_iter_sum = sv->t_n = 0;
+    return PASTA_OK;
+}
+
+int pasta_server_set_timing(pasta_server* sv, int on) {
+    if (!sv) return fail(PASTA_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->cfg.timing = on ? 1 : 0;
     return PASTA_OK;
 }

@@ -161,44 +161,55 @@ __device__ __forceinline__ void unpack8(const uint4 v, float u[8]) {
 }
 
 // Interpolated synthetic flux on [base, base+cnt) into dst (zero outside [0, W)).
-// Groups of 8 pixels aligned to 8 in absolute pixel coordinates, 16-byte loads,
-// two groups per thread in flight.
-__device__ __forceinline__ void interp_tile(const DevGrid& G, const int* sidx, const float* sw,
+// Groups of 8 pixels aligned to 8 in absolute pixel coordinates, 16-byte loads.
+// Each thread owns two groups; the 64 neighbour rows are walked 8 at a time with
+// all 16 loads issued before any is consumed.  Absent neighbours read row 0 with
+// weight 0 (their weight is 0 in the reference too), so the loop has no branch.
+__device__ __forceinline__ void acc8(float (&acc)[8], const uint4 v, float f, float w, long long x0,
+                                     bool fast, int Wq) {
+    float u[8];
+    unpack8(v, u);
+    if (fast) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) acc[t] = fmaf(u[t] * f, w, acc[t]);  // interpolate.rs:962
+    } else {
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const float q = (x0 + t < Wq) ? u[t] : u[t] / 65535.0f;     // tail quirk :978
+            acc[t] = fmaf(q * f, w, acc[t]);
+        }
+    }
+}
+
+__device__ __forceinline__ void interp_tile(const DevGrid& G, const long long* srow, const float* sw,
                                             const float* sf, long long base, int cnt, float* dst) {
     const long long g0 = (base >= 0) ? (base & ~7LL) : -(((-base) + 7) & ~7LL);
     const int ngroups = (int)((base + cnt - g0 + 7) >> 3);
+    const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
     for (int gg = threadIdx.x; gg < ngroups; gg += 2 * NT) {
         const long long xa = g0 + 8LL * gg;
         const long long xb = xa + 8LL * NT;
         const bool ha = xa >= 0 && xa < G.W;
         const bool hb = (gg + NT) < ngroups && xb >= 0 && xb < G.W;
-        const bool fa = ha && xa + 7 < G.Wq;  // all 8 pixels before the tail
-        const bool fb = hb && xb + 7 < G.Wq;
+        const bool fa = xa + 7 < G.Wq;  // all 8 pixels before the n % 128 tail
+        const bool fb = xb + 7 < G.Wq;
+        const long long oa = ha ? xa : 0, ob = hb ? xb : 0;
         float aa[8], ab[8];
 #pragma unroll
         for (int t = 0; t < 8; t++) { aa[t] = 0.0f; ab[t] = 0.0f; }
-        for (int j = 0; j < 64; j++) {
-            const int id = sidx[j];
-            if (id < 0) continue;
-            const float w = sw[j], f = sf[j];
-            const uint16_t* row = G.spectra + (long long)id * G.pitch;
-            if (ha) {
-                float u[8];
-                unpack8(*(const uint4*)(row + xa), u);
+        for (int j0 = 0; j0 < 64; j0 += 8) {
+            uint4 va[8], vb[8];
 #pragma unroll
-                for (int t = 0; t < 8; t++) {
-                    float M = (fa || xa + t < G.Wq) ? u[t] * f : u[t] / 65535.0f * f;
-                    aa[t] = fmaf(M, w, aa[t]);
-                }
+            for (int jj = 0; jj < 8; jj++) {
+                const uint16_t* row = G.spectra + srow[j0 + jj];
+                va[jj] = ha ? *(const uint4*)(row + oa) : zero4;
+                vb[jj] = hb ? *(const uint4*)(row + ob) : zero4;
             }
-            if (hb) {
-                float u[8];
-                unpack8(*(const uint4*)(row + xb), u);
 #pragma unroll
-                for (int t = 0; t < 8; t++) {
-                    float M = (fb || xb + t < G.Wq) ? u[t] * f : u[t] / 65535.0f * f;
-                    ab[t] = fmaf(M, w, ab[t]);
-                }
+            for (int jj = 0; jj < 8; jj++) {
+                const float w = sw[j0 + jj], f = sf[j0 + jj];
+                acc8(aa, va[jj], f, w, xa, fa, G.Wq);
+                acc8(ab, vb[jj], f, w, xb, fb, G.Wq);
             }
         }
 #pragma unroll
@@ -256,7 +267,7 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     __shared__ float Rb[RLDS];
     __shared__ float hrev[KROT_MAX + 8];
     __shared__ float grev[2 * HD_MAX + 1 + 8];
-    __shared__ int s_idx[64];
+    __shared__ long long s_row[64];
     __shared__ float s_w[64], s_f[64];
     __shared__ int s_rng[2];
 
@@ -272,7 +283,12 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
         s_rng[0] = obs_lower_bound(S.fidx, S.N, sl, t0);
         s_rng[1] = obs_lower_bound(S.fidx, S.N, sl, t0 + TILE);
     }
-    if (tid < 64) { s_idx[tid] = ev.idx[tid]; s_w[tid] = ev.w[tid]; s_f[tid] = ev.f[tid]; }
+    if (tid < 64) {
+        const int id = ev.idx[tid];
+        s_row[tid] = id >= 0 ? (long long)id * G.pitch : 0;
+        s_w[tid] = id >= 0 ? ev.w[tid] : 0.0f;
+        s_f[tid] = ev.f[tid];
+    }
     __syncthreads();
     int ilo = 0, ihi = 0;
     long long clo, chi;
@@ -308,10 +324,10 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
 
     // stage 1: interpolation (pass-through rotation writes straight into R)
     if (K > 0) {
-        interp_tile(G, s_idx, s_w, s_f, rlo - hr, nI, Ib);
+        interp_tile(G, s_row, s_w, s_f, rlo - hr, nI, Ib);
         if (tid < 16) Ib[nI + tid] = 0.0f;
     } else {
-        interp_tile(G, s_idx, s_w, s_f, rlo, nR, Rb);
+        interp_tile(G, s_row, s_w, s_f, rlo, nR, Rb);
     }
     if (tid < 16) Rb[nR + tid] = 0.0f;
     __syncthreads();
@@ -715,12 +731,14 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
                        const int2* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
                        const int2* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int pmax, int mode,
-                       hipStream_t st) {
+                       hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st) {
     if (n_evals <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64), 0, st, G, stars, labels, evals, rot, mode);
+    if (ev_model0) (void)hipEventRecord(ev_model0, st);
     if (n_model > 0)
         hipLaunchKernelGGL(k_model, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot,
                            mblk_model, model, mode);
+    if (ev_model1) (void)hipEventRecord(ev_model1, st);
     if (n_mom > 0) {
         if (pmax <= 12)
             hipLaunchKernelGGL(k_moments<12>, dim3(n_mom), dim3(NT), 0, st, stars, evals,

@@ -138,3 +138,52 @@ def smooth_design(n, ncols, seed):
         ph = rng.uniform(0, np.pi)
         cols.append(np.cos(np.pi * k * x / 2.0 + ph))
     return np.stack(cols, axis=1).astype(np.float32)
+
+
+def fill_grid_gpu(interp, spec):
+    """Fill a resident grid (InMemInterpolator.empty(spec.labels(), ...)) in place on the
+    GPU with the same recipe as make_grid (libpasta_synth.so)."""
+    import ctypes as C
+    import os
+
+    from . import _lib
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpasta_synth.so")
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python pasta_amd/build.py`")
+    sl = C.CDLL(path)
+    rows = C.POINTER(C.c_uint16)()
+    pitch = C.c_int64()
+    _lib.check(_lib.lib.pasta_grid_device_rows(interp._h, C.byref(rows), C.byref(pitch)))
+    centre, width, a, b = spec.lines()
+    ul = np.ascontiguousarray(spec.unit_labels(spec.labels()), dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    sl.pasta_synth_fill_grid.argtypes = [C.POINTER(C.c_uint16), C.c_int64, C.c_int, C.c_int,
+                                         C.c_double, C.c_double, dp, C.c_int, dp, dp, dp, dp]
+    rc = sl.pasta_synth_fill_grid(rows, pitch.value, len(ul), spec.n_pix, spec.log10_first,
+                                  spec.log10_step, ul.ctypes.data_as(dp), spec.n_lines,
+                                  centre.ctypes.data_as(dp), width.ctypes.data_as(dp),
+                                  a.ctypes.data_as(dp), b.ctypes.data_as(dp))
+    if rc != 0:
+        raise RuntimeError(f"pasta_synth_fill_grid failed ({rc})")
+
+
+def copy_probe(nbytes):
+    """One plain device copy of nbytes (for scaling FETCH_SIZE / WRITE_SIZE)."""
+    import ctypes as C
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpasta_synth.so")
+    sl = C.CDLL(path)
+    sl.pasta_synth_copy_probe.argtypes = [C.c_int64]
+    return sl.pasta_synth_copy_probe(int(nbytes))
+
+
+def read_grid(interp):
+    """Resident grid rows back to the host (storage order)."""
+    import ctypes as C
+    from . import _lib
+    out = np.zeros((interp.n_models, interp.n_pix), np.uint16)
+    _lib.check(_lib.lib.pasta_grid_read_rows(interp._h, 0, interp.n_models,
+                                             out.ctypes.data_as(C.POINTER(C.c_uint16))))
+    return out
