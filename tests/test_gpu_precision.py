@@ -1,0 +1,90 @@
+"""Precision configuration: a non-normalised grid (includes_factor: per-model f32 max
+factor spanning 1e3-1e7, flux/max stored as uint16), wavelength-dependent
+resolution R = 60,000 -> 90,000 (per-pixel Gaussian kernels), 16 chunks of degree
+10.  The GPU (FP32 with f64 reductions and FP64 solves) is held to 1e-5 of the
+f32-faithful oracle, and its error against the FP64 oracle is measured and
+bounded; both figures are printed."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def prec():
+    import pasta_amd as pasta
+    from pasta_amd import synth
+    spec = synth.small_grid_spec()
+    labels = spec.labels()
+    spectra, factors = synth.make_grid(spec, includes_factor_seed=77)
+    og = O.Grid(labels, spectra, factors, spec.log10_first, spec.log10_step)
+    wl = synth.observed_wavelengths(spec, 15_000, 450)
+    R = np.interp(wl, [wl[0], wl[-1]], [60_000.0, 90_000.0])
+    disp = (wl / R).astype(np.float32)   # per-pixel width in Angstrom (convolve_rv.rs:360-364)
+    od = O.Dispersion(og, O.DISP_VARIABLE, wl, disp=disp)
+    true = np.array([6123.0, -0.3, 4.21, 10.0, 20.0])
+    flux, var = synth.observe(O.produce_model(og, od, true), wl, 150.0, 2100)
+    oc = O.Continuum("chunk", len(wl), wl=wl, n_chunks=16, p_order=10, overlap=0.2)
+    wlg = pasta.WlGrid(spec.log10_first, spec.log10_step, spec.n_pix, log=True)
+    interp = pasta.InMemInterpolator.from_arrays(spectra, labels, wlg, factors=factors)
+    d = pasta.VariableResolutionDispersion(wl, disp, wlg)
+    c = pasta.ChunkContinuumFitter(wl, 16, 10, 0.2)
+    star = pasta.Star(interp, d, c, flux, var)
+    labs = np.concatenate([[true], synth.random_labels(spec, 24, 31)])
+    return dict(og=og, od=od, oc=oc, interp=interp, d=d, star=star, labs=labs, flux=flux, var=var,
+                factors=factors)
+
+
+def test_factor_range(prec):
+    f = prec["factors"] * 65535.0
+    assert f.min() < 1e4 and f.max() > 1e6
+
+
+def test_models_vs_f32_and_fp64_oracles(prec):
+    out = prec["interp"].produce_model_bulk(prec["d"], prec["labs"])
+    worst32 = worst64 = 0.0
+    for k, lab in enumerate(prec["labs"]):
+        m32 = O.produce_model(prec["og"], prec["od"], lab)
+        m64 = O.produce_model(prec["og"], prec["od"], lab, f64=True)
+        scale = np.abs(m64).max()
+        worst32 = max(worst32, float(np.max(np.abs(out[k] - m32) / np.abs(m32))))
+        worst64 = max(worst64, float(np.max(np.abs(out[k] - m64)) / scale))
+    print(f"model: GPU vs f32 oracle max rel {worst32:.2e}; GPU vs FP64 oracle max err / max "
+          f"{worst64:.2e}", flush=True)
+    assert worst32 < 1e-5
+    # FP32 arithmetic plus the reference's f32-quantised resample weight (<= 1.5e-3 of a
+    # pixel-to-pixel flux step at these indices)
+    assert worst64 < 2e-3
+
+
+def test_chi2_vs_f32_and_fp64_oracles(prec):
+    chi, st, _ = prec["star"].chi2(prec["labs"])
+    assert not st.any()
+    worst32 = worst64 = 0.0
+    for k, lab in enumerate(prec["labs"]):
+        c32, _ = O.cost(prec["og"], prec["od"], prec["oc"], prec["flux"], prec["var"], lab)
+        c64, _ = O.cost(prec["og"], prec["od"], prec["oc"], prec["flux"], prec["var"], lab, f64=True)
+        worst32 = max(worst32, abs(chi[k] - c32) / c32)
+        worst64 = max(worst64, abs(chi[k] - c64) / c64)
+    print(f"chi2: GPU vs f32 oracle max rel {worst32:.2e}; GPU vs FP64 oracle max rel "
+          f"{worst64:.2e}", flush=True)
+    assert worst32 < 1e-5
+    assert worst64 < 5e-2
+
+
+def test_fit_best_labels_vs_fp64(prec):
+    import pasta_amd as pasta
+    fitter = prec["interp"].get_fitter(prec["d"], pasta.ChunkContinuumFitter(
+        prec["od"].wl, 16, 10, 0.2), pasta.PSOSettings(40, 60), (0.0, 300.0), (-150.0, 150.0))
+    res = fitter.fit(prec["interp"], prec["flux"], prec["var"], seed=9)
+    best = np.array(res.label.as_list())
+    c64_best, _ = O.cost(prec["og"], prec["od"], prec["oc"], prec["flux"], prec["var"], best, f64=True)
+    c64_true, _ = O.cost(prec["og"], prec["od"], prec["oc"], prec["flux"], prec["var"],
+                         prec["labs"][0], f64=True)
+    print(f"fit: {best.tolist()} chi2 {res.chi2:.6f}; FP64 chi2 at best {c64_best:.6f}, at truth "
+          f"{c64_true:.6f}", flush=True)
+    # the FP64 oracle agrees that the GPU's best is at least as good as the truth
+    assert c64_best <= c64_true * (1 + 1e-3)
+    assert np.all(np.abs(best - prec["labs"][0]) <= np.array([80.0, 0.08, 0.12, 2.5, 0.6]))
