@@ -27,7 +27,8 @@
 
 namespace pasta {
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
-                       const double* labels, float* rot, const int2* mblk_model, int n_model,
+                       const double* labels, float* rot, int* tbound, int tb_stride,
+                       int* sort_hist, int* perm, int n_model,
                        const int2* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
                        const int2* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int pmax, int mode,
@@ -122,6 +123,8 @@ struct pasta_grid {
         g.sg_step = step;
         g.synth_first_wl = first_wl;
         g.synth_last_wl = last_wl;
+        g.n_models_bins = geom.n_models + 1;
+        g.pad0 = 0;
         return g;
     }
 };
@@ -818,8 +821,9 @@ struct Batch {
     DBuf<DevEval> d_evals;
     DBuf<double> d_labels, d_mpart, d_cpart, d_chi2;
     DBuf<float> d_rot, d_model, d_pf;
-    DBuf<int> d_status;
-    DBuf<int2> d_bm, d_bmom, d_solve, d_bchi2;
+    DBuf<int> d_status, d_hist, d_perm, d_tbound;
+    int tb_stride = 1;
+    DBuf<int2> d_bmom, d_solve, d_bchi2;
     int n_bm = 0, n_bmom = 0, n_solve = 0, n_bchi2 = 0;
     // batched PSO state
     DBuf<DevPso> d_pso;
@@ -854,7 +858,7 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
     B.n_evals = ne;
     B.h_evals.assign(ne, DevEval{});
     int64_t mo = 0, mp = 0, cp = 0, pf = 0;
-    std::vector<int2> bm, bmom, solve, bchi2;
+    std::vector<int2> bmom, solve, bchi2;
     int max_tiles = 0;
     for (int s = 0; s < B.n_stars; s++) max_tiles = std::max(max_tiles, stars[s]->dev.n_tiles);
     for (int s = 0; s < B.n_stars; s++) {
@@ -881,13 +885,10 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
                 solve.push_back(make_int2(e, 0));
         }
     }
-    // model blocks tile-major: all evaluations working on one synthetic tile run together
-    for (int t = 0; t < max_tiles; t++)
-        for (int s = 0; s < B.n_stars; s++)
-            if (t < stars[s]->dev.n_tiles)
-                for (int q = 0; q < counts[s]; q++) bm.push_back(make_int2(B.first_eval[s] + q, t));
+    // model blocks: tile-major over the evaluations (sorted on the device each launch)
+    if ((int64_t)max_tiles * ne > 0x7fffffff) return fail(PASTA_ERR_ARG, "batch too large");
     B.model_len = mo; B.mpart_len = mp; B.cpart_len = cp; B.pf_len = pf;
-    B.n_bm = (int)bm.size(); B.n_bmom = (int)bmom.size();
+    B.n_bm = max_tiles * ne; B.n_bmom = (int)bmom.size();
     B.n_solve = (int)solve.size(); B.n_bchi2 = (int)bchi2.size();
     std::vector<DevStar> ds(B.n_stars);
     for (int s = 0; s < B.n_stars; s++) ds[s] = stars[s]->dev;
@@ -901,7 +902,10 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
     CHK(B.d_pf.alloc((size_t)std::max<int64_t>(pf, 1)));
     CHK(B.d_chi2.alloc(ne));
     CHK(B.d_status.alloc(ne));
-    CHK(B.d_bm.upload(bm));
+    B.tb_stride = max_tiles + 1;
+    CHK(B.d_tbound.alloc((size_t)ne * B.tb_stride));
+    CHK(B.d_hist.alloc((size_t)g->geom.n_models + 1));
+    CHK(B.d_perm.alloc(ne));
     CHK(B.d_bmom.upload(bmom));
     CHK(B.d_solve.upload(solve));
     CHK(B.d_bchi2.upload(bchi2));
@@ -911,8 +915,8 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
 
 static int batch_launch(Batch& B) {
     DevGrid G = B.grid->dev();
-    HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p, B.d_bm.p,
-                       B.n_bm, B.d_bmom.p, B.n_bmom, B.d_solve.p, B.n_solve, B.d_bchi2.p,
+    HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p,
+                       B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.n_bm, B.d_bmom.p, B.n_bmom, B.d_solve.p, B.n_solve, B.d_bchi2.p,
                        B.n_bchi2, B.d_model.p, B.d_mpart.p, B.d_pf.p, B.d_cpart.p, B.d_chi2.p,
                        B.d_status.p, B.pmax, B.mode, B.eager_events ? B.ev[0] : nullptr,
                        B.eager_events ? B.ev[1] : nullptr, B.stream));

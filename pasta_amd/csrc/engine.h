@@ -25,6 +25,8 @@ struct DevGrid {
     int W, Wq;                // pixels; first pixel of the n % 128 tail
     double sg_first, sg_step; // log10 of the first wavelength, log10 step
     double synth_first_wl, synth_last_wl;  // 10^first, 10^(first + step*W)
+    int n_models_bins;        // n_models + 1 (sort bins; the last holds failed evaluations)
+    int pad0;
 };
 
 struct DevStar {
@@ -56,6 +58,9 @@ struct DevEval {
     int tile_base;   // first synthetic tile touched
     int cmax;        // exclusive upper bound of readable model-segment indices
     int active;      // 0 = skip (finished / failed star in the batched path)
+    int key;         // storage row of the first present neighbour: groups evaluations
+                     // that gather the same rows (n_models for failed ones)
+    int pad1;
     double shift_log;
     long long m_off;  // offset of this evaluation's resampled model in the model buffer
     long long mp_off; // offset of its continuum-moment partials (doubles)

@@ -53,13 +53,27 @@ __device__ __forceinline__ int rotation_len(double vsini, double dvelo) {
     return (int)(nme + 1 - nme % 2);
 }
 
+// first observed pixel whose resample index floor(fidx + shift) >= t
+__device__ __forceinline__ int obs_lower_bound(const double* __restrict__ fidx, int N, double sl,
+                                               long long t) {
+    if (t <= 0) return 0;
+    int lo = 0, hi = N;
+    const double tt = (double)t;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (fidx[mid] + sl >= tt) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------
 // k_prep: one wavefront per evaluation
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restrict__ stars,
                                              const double* __restrict__ labels,
                                              DevEval* __restrict__ evals, float* __restrict__ rot,
-                                             int mode) {
+                                             int* __restrict__ tbound, int tb_stride, int mode) {
     const int e = blockIdx.x;
     const int lane = threadIdx.x;
     DevEval& ev = evals[e];
@@ -108,9 +122,23 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
         }
         ev.status = st;
         ev.krot = krot;
+        int key = G.geom->n_models;
+        if (st == ST_OK)
+            for (int j = 0; j < 64; j++)
+                if (ev.idx[j] >= 0) { key = ev.idx[j]; break; }
+        ev.key = key;
         s_K = (st == ST_OK) ? krot : 0;
     }
     __syncthreads();
+    // observed-pixel range of every synthetic tile of this evaluation, 64 searches at a time
+    if (mode == 0 && ev.status == ST_OK) {
+        const DevStar& S = stars[ev.star];
+        const double sl = ev.shift_log;
+        const long long tb0 = (long long)ev.tile_base * TILE;
+        int* out = tbound + (size_t)e * tb_stride;
+        for (int k = lane; k <= S.n_tiles; k += 64)
+            out[k] = obs_lower_bound(S.fidx, S.N, sl, tb0 + (long long)k * TILE);
+    }
     const int K = s_K;
     if (K > 0) {
         float* h = rot + (size_t)e * KROT_MAX;
@@ -139,21 +167,10 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     }
 }
 
-// first observed pixel whose resample index floor(fidx + shift) >= t
-__device__ __forceinline__ int obs_lower_bound(const double* __restrict__ fidx, int N, double sl,
-                                               long long t) {
-    if (t <= 0) return 0;
-    int lo = 0, hi = N;
-    const double tt = (double)t;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (fidx[mid] + sl >= tt) hi = mid;
-        else lo = mid + 1;
-    }
-    return lo;
-}
+// native 16-byte vector (HIP's uint4 struct is copied member-wise into 4 dword loads)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void unpack8(const uint4 v, float u[8]) {
+__device__ __forceinline__ void unpack8(const u32x4 v, float u[8]) {
     u[0] = (float)(v.x & 0xffffu); u[1] = (float)(v.x >> 16);
     u[2] = (float)(v.y & 0xffffu); u[3] = (float)(v.y >> 16);
     u[4] = (float)(v.z & 0xffffu); u[5] = (float)(v.z >> 16);
@@ -165,27 +182,30 @@ __device__ __forceinline__ void unpack8(const uint4 v, float u[8]) {
 // Each thread owns two groups; the 64 neighbour rows are walked 8 at a time with
 // all 16 loads issued before any is consumed.  Absent neighbours read row 0 with
 // weight 0 (their weight is 0 in the reference too), so the loop has no branch.
-__device__ __forceinline__ void acc8(float (&acc)[8], const uint4 v, float f, float w, long long x0,
+// fw = factor * weight of the neighbour: the reference forms f32(u) * factor, then the
+// gemv multiplies by the weight (interpolate.rs:962,966); one product fw per neighbour
+// differs from that by an f32 rounding per term.
+__device__ __forceinline__ void acc8(float (&acc)[8], const u32x4 v, float fw, long long x0,
                                      bool fast, int Wq) {
     float u[8];
     unpack8(v, u);
     if (fast) {
 #pragma unroll
-        for (int t = 0; t < 8; t++) acc[t] = fmaf(u[t] * f, w, acc[t]);  // interpolate.rs:962
+        for (int t = 0; t < 8; t++) acc[t] = fmaf(u[t], fw, acc[t]);
     } else {
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            const float q = (x0 + t < Wq) ? u[t] : u[t] / 65535.0f;     // tail quirk :978
-            acc[t] = fmaf(q * f, w, acc[t]);
+            const float q = (x0 + t < Wq) ? u[t] : u[t] / 65535.0f;  // tail quirk :978
+            acc[t] = fmaf(q, fw, acc[t]);
         }
     }
 }
 
-__device__ __forceinline__ void interp_tile(const DevGrid& G, const long long* srow, const float* sw,
-                                            const float* sf, long long base, int cnt, float* dst) {
+__device__ __forceinline__ void interp_tile(const DevGrid& G, const long long* srow,
+                                            const float* sfw, long long base, int cnt, float* dst) {
     const long long g0 = (base >= 0) ? (base & ~7LL) : -(((-base) + 7) & ~7LL);
     const int ngroups = (int)((base + cnt - g0 + 7) >> 3);
-    const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
     for (int gg = threadIdx.x; gg < ngroups; gg += 2 * NT) {
         const long long xa = g0 + 8LL * gg;
         const long long xb = xa + 8LL * NT;
@@ -198,18 +218,18 @@ __device__ __forceinline__ void interp_tile(const DevGrid& G, const long long* s
 #pragma unroll
         for (int t = 0; t < 8; t++) { aa[t] = 0.0f; ab[t] = 0.0f; }
         for (int j0 = 0; j0 < 64; j0 += 8) {
-            uint4 va[8], vb[8];
+            u32x4 va[8], vb[8];
 #pragma unroll
             for (int jj = 0; jj < 8; jj++) {
                 const uint16_t* row = G.spectra + srow[j0 + jj];
-                va[jj] = ha ? *(const uint4*)(row + oa) : zero4;
-                vb[jj] = hb ? *(const uint4*)(row + ob) : zero4;
+                va[jj] = ha ? *(const u32x4*)(row + oa) : zero4;
+                vb[jj] = hb ? *(const u32x4*)(row + ob) : zero4;
             }
 #pragma unroll
             for (int jj = 0; jj < 8; jj++) {
-                const float w = sw[j0 + jj], f = sf[j0 + jj];
-                acc8(aa, va[jj], f, w, xa, fa, G.Wq);
-                acc8(ab, vb[jj], f, w, xb, fb, G.Wq);
+                const float fw = sfw[j0 + jj];
+                acc8(aa, va[jj], fw, xa, fa, G.Wq);
+                acc8(ab, vb[jj], fw, xb, fb, G.Wq);
             }
         }
 #pragma unroll
@@ -261,33 +281,38 @@ __device__ __forceinline__ void conv_tile(const float* in, const float* tr, int 
 __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restrict__ stars,
                                               const DevEval* __restrict__ evals,
                                               const float* __restrict__ rot,
-                                              const int2* __restrict__ blocks,
+                                              const int* __restrict__ perm, int n_evals,
+                                              const int* __restrict__ tbound, int tb_stride,
                                               float* __restrict__ model, int mode) {
     __shared__ float Ib[ILDS];
     __shared__ float Rb[RLDS];
     __shared__ float hrev[KROT_MAX + 8];
     __shared__ float grev[2 * HD_MAX + 1 + 8];
     __shared__ long long s_row[64];
-    __shared__ float s_w[64], s_f[64];
+    __shared__ float s_fw[64];
     __shared__ int s_rng[2];
 
+    // tile-major over evaluations sorted by neighbour rows: the workgroups resident on
+    // one XCD work on one tile of evaluations that gather overlapping rows
     const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int2 bt = blocks[b];
+    const int tile = b / n_evals;
+    const int2 bt = make_int2(perm[b - tile * n_evals], tile);
     const DevEval& ev = evals[bt.x];
     if (ev.status != ST_OK) return;
     const DevStar& S = stars[ev.star];
+    if (tile >= S.n_tiles) return;
     const int tid = threadIdx.x;
     const double sl = ev.shift_log;
     const long long t0 = (long long)(ev.tile_base + bt.y) * TILE;
     if (tid == 0 && mode == 0) {
-        s_rng[0] = obs_lower_bound(S.fidx, S.N, sl, t0);
-        s_rng[1] = obs_lower_bound(S.fidx, S.N, sl, t0 + TILE);
+        const int* tb = tbound + (size_t)bt.x * tb_stride + bt.y;
+        s_rng[0] = tb[0];
+        s_rng[1] = tb[1];
     }
     if (tid < 64) {
         const int id = ev.idx[tid];
         s_row[tid] = id >= 0 ? (long long)id * G.pitch : 0;
-        s_w[tid] = id >= 0 ? ev.w[tid] : 0.0f;
-        s_f[tid] = ev.f[tid];
+        s_fw[tid] = id >= 0 ? ev.f[tid] * ev.w[tid] : 0.0f;
     }
     __syncthreads();
     int ilo = 0, ihi = 0;
@@ -324,10 +349,10 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
 
     // stage 1: interpolation (pass-through rotation writes straight into R)
     if (K > 0) {
-        interp_tile(G, s_row, s_w, s_f, rlo - hr, nI, Ib);
+        interp_tile(G, s_row, s_fw, rlo - hr, nI, Ib);
         if (tid < 16) Ib[nI + tid] = 0.0f;
     } else {
-        interp_tile(G, s_row, s_w, s_f, rlo, nR, Rb);
+        interp_tile(G, s_row, s_fw, rlo, nR, Rb);
     }
     if (tid < 16) Rb[nR + tid] = 0.0f;
     __syncthreads();
@@ -724,20 +749,71 @@ __global__ void k_pso_init(DevPso* __restrict__ pso, int n_stars, const DevEval*
 }
 
 // ---------------------------------------------------------------------------
+// counting sort of evaluations by key (order only: results do not depend on it)
+// ---------------------------------------------------------------------------
+__global__ void k_sort_zero(int* __restrict__ hist, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) hist[i] = 0;
+}
+
+__global__ void k_sort_hist(const DevEval* __restrict__ evals, int n, int* __restrict__ hist) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) atomicAdd(&hist[evals[e].key], 1);
+}
+
+// exclusive scan of hist[0..n) in place, one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_sort_scan(int* __restrict__ hist, int n) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int lo = t * per, hi = min(n, lo + per);
+    int s = 0;
+    for (int i = lo; i < hi; i++) s += hist[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? part[t - 1] : 0;
+    for (int i = lo; i < hi; i++) {
+        int c = hist[i];
+        hist[i] = run;
+        run += c;
+    }
+}
+
+__global__ void k_sort_scatter(const DevEval* __restrict__ evals, int n, int* __restrict__ offs,
+                               int* __restrict__ perm) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) perm[atomicAdd(&offs[evals[e].key], 1)] = e;
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers (called from the driver in api.hip)
 // ---------------------------------------------------------------------------
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
-                       const double* labels, float* rot, const int2* mblk_model, int n_model,
+                       const double* labels, float* rot, int* tbound, int tb_stride,
+                       int* sort_hist, int* perm, int n_model,
                        const int2* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
                        const int2* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int pmax, int mode,
                        hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st) {
     if (n_evals <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64), 0, st, G, stars, labels, evals, rot, mode);
+    hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64), 0, st, G, stars, labels, evals, rot, tbound,
+                       tb_stride, mode);
+    const int nb = G.n_models_bins;
+    hipLaunchKernelGGL(k_sort_zero, dim3((nb + 255) / 256), dim3(256), 0, st, sort_hist, nb);
+    hipLaunchKernelGGL(k_sort_hist, dim3((n_evals + 255) / 256), dim3(256), 0, st, evals, n_evals,
+                       sort_hist);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, sort_hist, nb);
+    hipLaunchKernelGGL(k_sort_scatter, dim3((n_evals + 255) / 256), dim3(256), 0, st, evals,
+                       n_evals, sort_hist, perm);
     if (ev_model0) (void)hipEventRecord(ev_model0, st);
     if (n_model > 0)
-        hipLaunchKernelGGL(k_model, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot,
-                           mblk_model, model, mode);
+        hipLaunchKernelGGL(k_model, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot, perm,
+                           n_evals, tbound, tb_stride, model, mode);
     if (ev_model1) (void)hipEventRecord(ev_model1, st);
     if (n_mom > 0) {
         if (pmax <= 12)

@@ -72,8 +72,9 @@ def full_grid_spec():
 
 def make_grid(spec, labels=None, includes_factor_seed=None):
     """uint16 spectra (n_models x n_pix) for `labels` (default: every grid point, storage
-    order).  With includes_factor_seed, each model is multiplied by a continuum-like
-    max factor ~ logU(1e3, 1e7) and the factors (max/65535, f32) are returned too."""
+    order).  With includes_factor_seed, each model is multiplied by a continuum whose
+    level rises smoothly with Teff from 1e3 to 1e7, stored as flux/max, and the factors
+    (max/65535, f32) are returned too."""
     if labels is None:
         labels = spec.labels()
     wl = spec.synth_wl()
@@ -92,7 +93,8 @@ def make_grid(spec, labels=None, includes_factor_seed=None):
     if includes_factor_seed is None:
         return (flux * 65535.0).astype(np.uint16), None
     rng = np.random.default_rng(includes_factor_seed)
-    mx = np.exp(rng.uniform(np.log(1e3), np.log(1e7), len(labels)))
+    # continuum level rising smoothly with Teff over 1e3..1e7 (0.02 dex jitter)
+    mx = 10.0 ** (3.0 + 4.0 * u[:, 0] + rng.normal(0.0, 0.02, len(labels)))
     x = np.linspace(-1, 1, spec.n_pix)
     shape = 1.0 + 0.3 * x[None, :] * rng.uniform(-1, 1, (len(labels), 1))
     cont = flux * shape

@@ -49,9 +49,11 @@ def test_models_vs_f32_and_fp64_oracles(prec):
         m32 = O.produce_model(prec["og"], prec["od"], lab)
         m64 = O.produce_model(prec["og"], prec["od"], lab, f64=True)
         scale = np.abs(m64).max()
-        worst32 = max(worst32, float(np.max(np.abs(out[k] - m32) / np.abs(m32))))
+        # errors relative to the spectrum's scale (max |flux|): with per-model factors the
+        # interpolation cancels large terms, so pointwise relative error is not meaningful
+        worst32 = max(worst32, float(np.max(np.abs(out[k] - m32)) / np.abs(m32).max()))
         worst64 = max(worst64, float(np.max(np.abs(out[k] - m64)) / scale))
-    print(f"model: GPU vs f32 oracle max rel {worst32:.2e}; GPU vs FP64 oracle max err / max "
+    print(f"model: GPU vs f32 oracle max err / max {worst32:.2e}; GPU vs FP64 oracle max err / max "
           f"{worst64:.2e}", flush=True)
     assert worst32 < 1e-5
     # FP32 arithmetic plus the reference's f32-quantised resample weight (<= 1.5e-3 of a
