@@ -23,7 +23,7 @@
 
 namespace pasta {
 
-constexpr int ILDS = TILE + 1 + 2 * HD_MAX + (KROT_MAX - 1) + 19;  // 4112
+constexpr int ILDS = 4160;  // >= TILE + 1 + 2*HD_MAX + KROT_MAX - 1 + 48 and every MFMA read
 constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 16;                   // 2064
 constexpr double C_KMS = 299792.0;   // src/convolve_rv.rs:440,547
 constexpr double LN10 = 2.302585092994045684017991454684364208;
@@ -275,6 +275,57 @@ __device__ __forceinline__ void conv_tile(const float* in, const float* tr, int 
     }
 }
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Rotational broadening on the matrix cores: out[q] = sum_u tr[u] * in[q + u], q < n,
+// as the Toeplitz product R(blocks x 16) = X(blocks x J) * T(J x 16) with
+// X[a][j] = in[16a + j] and T[j][b] = tr[j - b], on v_mfma_f32_16x16x4f32 (fp32
+// products, fp32 accumulation).  Operand layout of the instruction (checked on
+// gfx950): A lane l = A[l%16][l/16], B lane l = B[l/16][l%16], D lane l register i =
+// D[4(l/16)+i][l%16].  hpad[16 + u] = tr[u] for u < K, zero around it.  Each wave owns
+// the 16-block tiles w and w+4; `in` holds finite values up to index n + K + 34.
+__device__ __forceinline__ void conv_rot_mfma(const float* in, const float* hpad, int K, int n,
+                                              float* out, bool mask, long long x0, int W) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nblk = (n + 15) >> 4;
+    const int tiles = (nblk + 15) >> 4;
+    const int Jp = (K + 15 + 3) & ~3;
+    const int m = lane & 15, kk = lane >> 4;
+    const float* hb = hpad + 16 + kk - m;
+    for (int t0 = wv; t0 < tiles; t0 += 8) {
+        const int t1 = t0 + 4;
+        const bool has1 = t1 < tiles;
+        const float* ia0 = in + 16 * (16 * t0 + m) + kk;
+        const float* ia1 = in + 16 * (16 * t1 + m) + kk;
+        f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (has1) {
+#pragma unroll 4
+            for (int j0 = 0; j0 < Jp; j0 += 4) {
+                const float b = hb[j0];
+                c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ia0[j0], b, c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ia1[j0], b, c1, 0, 0, 0);
+            }
+        } else {
+#pragma unroll 4
+            for (int j0 = 0; j0 < Jp; j0 += 4)
+                c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ia0[j0], hb[j0], c0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int q = 16 * (16 * t0 + 4 * kk + i) + m;
+            if (q < n) {
+                long long x = x0 + q;
+                out[q] = (mask && (x < 0 || x >= W)) ? 0.0f : c0[i];
+            }
+            q = 16 * (16 * t1 + 4 * kk + i) + m;
+            if (has1 && q < n) {
+                long long x = x0 + q;
+                out[q] = (mask && (x < 0 || x >= W)) ? 0.0f : c1[i];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_model: one workgroup per (evaluation, synthetic tile)
 // ---------------------------------------------------------------------------
@@ -286,7 +337,7 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
                                               float* __restrict__ model, int mode) {
     __shared__ float Ib[ILDS];
     __shared__ float Rb[RLDS];
-    __shared__ float hrev[KROT_MAX + 8];
+    __shared__ float hrev[KROT_MAX + 48];  // zero, then reversed taps from index 16, zeros after
     __shared__ float grev[2 * HD_MAX + 1 + 8];
     __shared__ long long s_row[64];
     __shared__ float s_fw[64];
@@ -339,8 +390,11 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
 
     if (K > 0) {
         const float* h = rot + (size_t)bt.x * KROT_MAX;
-        const int Kp = (K + 7) & ~7;
-        for (int i = tid; i < Kp; i += NT) hrev[i] = i < K ? h[K - 1 - i] : 0.0f;
+        const int Kp = ((K + 15 + 3) & ~3) + 20;
+        for (int i = tid; i < Kp; i += NT) {
+            const int u = i - 16;
+            hrev[i] = (u >= 0 && u < K) ? h[K - 1 - u] : 0.0f;
+        }
     }
     if (S.disp_kind == DISP_FIXED) {
         const int Kp = (S.Kd + 7) & ~7;
@@ -350,7 +404,7 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     // stage 1: interpolation (pass-through rotation writes straight into R)
     if (K > 0) {
         interp_tile(G, s_row, s_fw, rlo - hr, nI, Ib);
-        if (tid < 16) Ib[nI + tid] = 0.0f;
+        if (tid < 48) Ib[nI + tid] = 0.0f;  // the MFMA steps read up to nI + 34
     } else {
         interp_tile(G, s_row, s_fw, rlo, nR, Rb);
     }
@@ -359,7 +413,7 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
 
     // stage 2: rotational broadening
     if (K > 0) {
-        conv_tile(Ib, hrev, K, nR, Rb, mask, rlo, G.W);
+        conv_rot_mfma(Ib, hrev, K, nR, Rb, mask, rlo, G.W);
         __syncthreads();
     }
 
