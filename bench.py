@@ -30,6 +30,19 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 R_INSTR = 85_000.0
+# HBM bytes per batched k_model launch (512 stars x 64 particles), measured with
+# rocprofv3 FETCH_SIZE / WRITE_SIZE and scaled by a plain copy kernel; written there
+# as "traffic_bytes_per_launch: <bytes>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_model_batched_512x64_hbm_traffic.md")
+
+
+def measured_traffic(stars, particles):
+    if (stars, particles) != (512, 64) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def log(*a):
@@ -243,7 +256,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(len(stars), args.particles),
             "algorithmic_bytes_per_launch": bytes_launch,
             "kernel_ms": round(model_ms, 3),
             "iteration_ms": round(iter_ms, 3),
