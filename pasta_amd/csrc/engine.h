@@ -68,7 +68,8 @@ struct DevEval {
     long long pf_off; // offset of its continuum parameters (floats)
     int idx[64];
     float w[64];
-    float f[64];
+    float fw[64];     // factor * weight of each neighbour, 0 where the neighbour is absent
+    const uint16_t* rowp[64];  // its grid row (row 0 where absent)
 };
 
 // Batched PSO state per star (device).

@@ -23,8 +23,12 @@
 
 namespace pasta {
 
-constexpr int ILDS = 4160;  // >= TILE + 1 + 2*HD_MAX + KROT_MAX - 1 + 48 and every MFMA read
-constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 16;                   // 2064
+// Interpolated tile for the rotation stage: logical index k lives in slot k + 4*(k/16)
+// (4 pad floats after every 16), so that the 16-float-strided MFMA operand rows fall
+// in distinct LDS banks.  Logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KROT_MAX + 55.
+constexpr int ILOG = 4160;
+constexpr int ILDS = ILOG + ILOG / 4;                               // 5200 slots
+constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 8 + 16 + 8;           // 2080
 constexpr double C_KMS = 299792.0;   // src/convolve_rv.rs:440,547
 constexpr double LN10 = 2.302585092994045684017991454684364208;
 
@@ -89,8 +93,14 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
         } else {
             const DevStar& S = stars[ev.star];
             st = local_weights(*G.geom, lab[0], lab[1], lab[2], ev.idx, ev.w);
-            for (int j = 0; j < 64; j++)
-                ev.f[j] = (ev.idx[j] >= 0 && G.factors) ? G.factors[ev.idx[j]] : 1.0f / 65535.0f;
+            // fw = factor * weight: the reference forms f32(u) * factor, then the gemv
+            // multiplies by the weight (interpolate.rs:962,966); one product per neighbour
+            // differs from that by an f32 rounding per term
+            for (int j = 0; j < 64; j++) {
+                const float f = (ev.idx[j] >= 0 && G.factors) ? G.factors[ev.idx[j]] : 1.0f / 65535.0f;
+                ev.fw[j] = ev.idx[j] >= 0 ? f * ev.w[j] : 0.0f;
+                ev.rowp[j] = G.spectra + (ev.idx[j] >= 0 ? (long long)ev.idx[j] * G.pitch : 0LL);
+            }
             if (!st && vsini < 0.0) st = ST_VSINI;
             if (!st) {
                 int K = rotation_len(vsini, dvelo);
@@ -169,6 +179,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
 
 // native 16-byte vector (HIP's uint4 struct is copied member-wise into 4 dword loads)
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void unpack8(const u32x4 v, float u[8]) {
     u[0] = (float)(v.x & 0xffffu); u[1] = (float)(v.x >> 16);
@@ -177,14 +188,17 @@ __device__ __forceinline__ void unpack8(const u32x4 v, float u[8]) {
     u[6] = (float)(v.w & 0xffffu); u[7] = (float)(v.w >> 16);
 }
 
-// Interpolated synthetic flux on [base, base+cnt) into dst (zero outside [0, W)).
-// Groups of 8 pixels aligned to 8 in absolute pixel coordinates, 16-byte loads.
-// Each thread owns two groups; the 64 neighbour rows are walked 8 at a time with
-// all 16 loads issued before any is consumed.  Absent neighbours read row 0 with
-// weight 0 (their weight is 0 in the reference too), so the loop has no branch.
-// fw = factor * weight of the neighbour: the reference forms f32(u) * factor, then the
-// gemv multiplies by the weight (interpolate.rs:962,966); one product fw per neighbour
-// differs from that by an f32 rounding per term.
+// Interpolated synthetic flux of 8-pixel groups aligned to 8 in absolute pixel
+// coordinates: group gg covers pixels [g0 + 8gg, g0 + 8gg + 8) (g0 a multiple of 8)
+// and lands at logical index 8gg of dst, in the padded slot layout when PAD.  Pixels
+// outside [0, W) are zero.  The 64 neighbour rows are addressed through one buffer
+// descriptor per row built from scalar registers (row index and factor*weight are
+// uniform over the workgroup), so a 16-byte gather costs no vector address
+// arithmetic; a group outside the row is given an offset past the descriptor's
+// extent and reads zeros.  16 rows are in flight per thread at a time.  Absent
+// neighbours read row 0 with weight 0 (their weight is 0 in the reference too).
+constexpr int OOB_OFFSET = 0x7ffffff0;
+
 __device__ __forceinline__ void acc8(float (&acc)[8], const u32x4 v, float fw, long long x0,
                                      bool fast, int Wq) {
     float u[8];
@@ -201,45 +215,44 @@ __device__ __forceinline__ void acc8(float (&acc)[8], const u32x4 v, float fw, l
     }
 }
 
-__device__ __forceinline__ void interp_tile(const DevGrid& G, const long long* srow,
-                                            const float* sfw, long long base, int cnt, float* dst) {
-    const long long g0 = (base >= 0) ? (base & ~7LL) : -(((-base) + 7) & ~7LL);
-    const int ngroups = (int)((base + cnt - g0 + 7) >> 3);
-    const u32x4 zero4 = {0u, 0u, 0u, 0u};
-    for (int gg = threadIdx.x; gg < ngroups; gg += 2 * NT) {
-        const long long xa = g0 + 8LL * gg;
-        const long long xb = xa + 8LL * NT;
-        const bool ha = xa >= 0 && xa < G.W;
-        const bool hb = (gg + NT) < ngroups && xb >= 0 && xb < G.W;
-        const bool fa = xa + 7 < G.Wq;  // all 8 pixels before the n % 128 tail
-        const bool fb = xb + 7 < G.Wq;
-        const long long oa = ha ? xa : 0, ob = hb ? xb : 0;
-        float aa[8], ab[8];
+__device__ __forceinline__ int islot(int k) { return k + ((k >> 4) << 2); }
+
+template <bool PAD>
+__device__ __forceinline__ void interp_groups(const DevGrid& G, const DevEval& ev, long long g0,
+                                              int ngroups, float* dst) {
+    const int nrec = (int)(G.pitch * 2);
+    for (int gg = threadIdx.x; gg < ngroups; gg += NT) {
+        const long long x0 = g0 + 8LL * gg;
+        const bool inside = x0 >= 0 && x0 < G.W;
+        const int voff = inside ? (int)(2 * x0) : OOB_OFFSET;
+        const bool fast = x0 + 7 < G.Wq;  // all 8 pixels before the n % 128 tail
+        float acc[8];
 #pragma unroll
-        for (int t = 0; t < 8; t++) { aa[t] = 0.0f; ab[t] = 0.0f; }
-        for (int j0 = 0; j0 < 64; j0 += 8) {
-            u32x4 va[8], vb[8];
+        for (int t = 0; t < 8; t++) acc[t] = 0.0f;
+#pragma unroll 1
+        for (int j0 = 0; j0 < 64; j0 += 16) {
+            u32x4 v[16];
 #pragma unroll
-            for (int jj = 0; jj < 8; jj++) {
-                const uint16_t* row = G.spectra + srow[j0 + jj];
-                va[jj] = ha ? *(const u32x4*)(row + oa) : zero4;
-                vb[jj] = hb ? *(const u32x4*)(row + ob) : zero4;
+            for (int jj = 0; jj < 16; jj++) {
+                const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                    (void*)ev.rowp[j0 + jj], 0, nrec, 0x00020000);
+                v[jj] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
             }
 #pragma unroll
-            for (int jj = 0; jj < 8; jj++) {
-                const float fw = sfw[j0 + jj];
-                acc8(aa, va[jj], fw, xa, fa, G.Wq);
-                acc8(ab, vb[jj], fw, xb, fb, G.Wq);
+            for (int jj = 0; jj < 16; jj++) {
+                const float fw = __builtin_bit_cast(
+                    float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, ev.fw[j0 + jj])));
+                acc8(acc, v[jj], fw, x0, fast, G.Wq);
             }
         }
+        if (x0 + 8 > G.W) {
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            long long x = xa + t;
-            if (x >= base && x < base + cnt) dst[x - base] = (x >= 0 && x < G.W) ? aa[t] : 0.0f;
-            x = xb + t;
-            if ((gg + NT) < ngroups && x >= base && x < base + cnt)
-                dst[x - base] = (x >= 0 && x < G.W) ? ab[t] : 0.0f;
+            for (int t = 0; t < 8; t++)
+                if (x0 + t >= G.W) acc[t] = 0.0f;
         }
+        float* d = dst + (PAD ? islot(8 * gg) : 8 * gg);
+        *(f4*)d = f4{acc[0], acc[1], acc[2], acc[3]};
+        *(f4*)(d + 4) = f4{acc[4], acc[5], acc[6], acc[7]};
     }
 }
 
@@ -275,52 +288,66 @@ __device__ __forceinline__ void conv_tile(const float* in, const float* tr, int 
     }
 }
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-// Rotational broadening on the matrix cores: out[q] = sum_u tr[u] * in[q + u], q < n,
-// as the Toeplitz product R(blocks x 16) = X(blocks x J) * T(J x 16) with
-// X[a][j] = in[16a + j] and T[j][b] = tr[j - b], on v_mfma_f32_16x16x4f32 (fp32
-// products, fp32 accumulation).  Operand layout of the instruction (checked on
-// gfx950): A lane l = A[l%16][l/16], B lane l = B[l/16][l%16], D lane l register i =
-// D[4(l/16)+i][l%16].  hpad[16 + u] = tr[u] for u < K, zero around it.  Each wave owns
-// the 16-block tiles w and w+4; `in` holds finite values up to index n + K + 34.
-__device__ __forceinline__ void conv_rot_mfma(const float* in, const float* hpad, int K, int n,
-                                              float* out, bool mask, long long x0, int W) {
+// Rotational broadening on the matrix cores: R[p] = sum_u tr[u] * I[p + u] for
+// p < np, as the Toeplitz product D(blocks x 16) = A(blocks x J) * T(J x 16) with
+// A[a][j] = I[16a + j], T[j][n] = tr[j - n] and R[16a + n] = D[a][n], on
+// v_mfma_f32_16x16x4f32 (fp32 products, fp32 accumulation).  Operand layout of the
+// instruction (checked on gfx950): A lane l = A[l%16][l/16], B lane l = B[l/16][l%16],
+// D lane l register i = D[4(l/16)+i][l%16].  The contraction index is dealt out so
+// that lane (m, kk) of step r in group c takes j = 16c + 4kk + r: its four A operands
+// of a group are I[16(a+c) + 4kk .. +3], one 16-byte LDS read in the padded layout
+// (slot 20(a+c) + 4kk), conflict-free across the 16 rows.  hpad[16 + u] = tr[u] for
+// u < K and zero elsewhere in [0, 16C + 16).  Each wave owns 16-block tiles w and w+4;
+// rows past the last block read the last block (their results are not stored).
+// Outputs are stored to out[p - off] for off <= p < np, zero outside [0, W) when mask.
+__device__ __forceinline__ void conv_rot_mfma(const float* Ipad, const float* hpad, int K, int np,
+                                              int off, float* out, bool mask, long long x0, int W) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int nblk = (n + 15) >> 4;
+    const int nblk = (np + 15) >> 4;
     const int tiles = (nblk + 15) >> 4;
-    const int Jp = (K + 15 + 3) & ~3;
+    const int C = (K + 15 + 15) >> 4;
     const int m = lane & 15, kk = lane >> 4;
-    const float* hb = hpad + 16 + kk - m;
+    const float* hb = hpad + 16 + 4 * kk - m;
     for (int t0 = wv; t0 < tiles; t0 += 8) {
         const int t1 = t0 + 4;
         const bool has1 = t1 < tiles;
-        const float* ia0 = in + 16 * (16 * t0 + m) + kk;
-        const float* ia1 = in + 16 * (16 * t1 + m) + kk;
+        const int a0 = min(16 * t0 + m, nblk - 1), a1 = min(16 * t1 + m, nblk - 1);
+        const f4* ia0 = (const f4*)(Ipad + 20 * a0 + 4 * kk);
+        const f4* ia1 = (const f4*)(Ipad + 20 * a1 + 4 * kk);
         f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
         if (has1) {
-#pragma unroll 4
-            for (int j0 = 0; j0 < Jp; j0 += 4) {
-                const float b = hb[j0];
-                c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ia0[j0], b, c0, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ia1[j0], b, c1, 0, 0, 0);
+#pragma unroll 2
+            for (int c = 0; c < C; c++) {
+                const f4 x0v = ia0[5 * c], x1v = ia1[5 * c];
+                const float* h = hb + 16 * c;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float bv = h[r];
+                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0v[r], bv, c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1v[r], bv, c1, 0, 0, 0);
+                }
             }
         } else {
-#pragma unroll 4
-            for (int j0 = 0; j0 < Jp; j0 += 4)
-                c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ia0[j0], hb[j0], c0, 0, 0, 0);
+#pragma unroll 2
+            for (int c = 0; c < C; c++) {
+                const f4 x0v = ia0[5 * c];
+                const float* h = hb + 16 * c;
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0v[r], h[r], c0, 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            int q = 16 * (16 * t0 + 4 * kk + i) + m;
-            if (q < n) {
-                long long x = x0 + q;
-                out[q] = (mask && (x < 0 || x >= W)) ? 0.0f : c0[i];
+            int p = 16 * (16 * t0 + 4 * kk + i) + m;
+            if (p >= off && p < np) {
+                const long long x = x0 + (p - off);
+                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c0[i];
             }
-            q = 16 * (16 * t1 + 4 * kk + i) + m;
-            if (has1 && q < n) {
-                long long x = x0 + q;
-                out[q] = (mask && (x < 0 || x >= W)) ? 0.0f : c1[i];
+            p = 16 * (16 * t1 + 4 * kk + i) + m;
+            if (has1 && p >= off && p < np) {
+                const long long x = x0 + (p - off);
+                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c1[i];
             }
         }
     }
@@ -339,9 +366,6 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     __shared__ float Rb[RLDS];
     __shared__ float hrev[KROT_MAX + 48];  // zero, then reversed taps from index 16, zeros after
     __shared__ float grev[2 * HD_MAX + 1 + 8];
-    __shared__ long long s_row[64];
-    __shared__ float s_fw[64];
-    __shared__ int s_rng[2];
 
     // tile-major over evaluations sorted by neighbour rows: the workgroups resident on
     // one XCD work on one tile of evaluations that gather overlapping rows
@@ -355,22 +379,12 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     const int tid = threadIdx.x;
     const double sl = ev.shift_log;
     const long long t0 = (long long)(ev.tile_base + bt.y) * TILE;
-    if (tid == 0 && mode == 0) {
-        const int* tb = tbound + (size_t)bt.x * tb_stride + bt.y;
-        s_rng[0] = tb[0];
-        s_rng[1] = tb[1];
-    }
-    if (tid < 64) {
-        const int id = ev.idx[tid];
-        s_row[tid] = id >= 0 ? (long long)id * G.pitch : 0;
-        s_fw[tid] = id >= 0 ? ev.f[tid] * ev.w[tid] : 0.0f;
-    }
-    __syncthreads();
     int ilo = 0, ihi = 0;
     long long clo, chi;
     if (mode == 0) {
-        ilo = s_rng[0];
-        ihi = s_rng[1];
+        const int* tb = tbound + (size_t)bt.x * tb_stride + bt.y;
+        ilo = __builtin_amdgcn_readfirstlane(tb[0]);
+        ihi = __builtin_amdgcn_readfirstlane(tb[1]);
         if (ilo >= ihi) return;
         clo = jfloor(S.fidx[ilo] + sl);
         chi = jfloor(S.fidx[ihi - 1] + sl) + 1;
@@ -390,7 +404,7 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
 
     if (K > 0) {
         const float* h = rot + (size_t)bt.x * KROT_MAX;
-        const int Kp = ((K + 15 + 3) & ~3) + 20;
+        const int Kp = 16 * ((K + 30) >> 4) + 16;  // every index the MFMA steps read
         for (int i = tid; i < Kp; i += NT) {
             const int u = i - 16;
             hrev[i] = (u >= 0 && u < K) ? h[K - 1 - u] : 0.0f;
@@ -401,26 +415,33 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
         for (int i = tid; i < Kp; i += NT) grev[i] = i < S.Kd ? S.dkern[S.Kd - 1 - i] : 0.0f;
     }
 
-    // stage 1: interpolation (pass-through rotation writes straight into R)
+    // stage 1: interpolation.  Groups start at the multiple of 8 at or below the first
+    // pixel; `off` is the first pixel's logical index.  With rotation the tile goes to
+    // Ib (padded slots), else straight into Rb, read from R = Rb + off.
+    float* R = Rb;
     if (K > 0) {
-        interp_tile(G, s_row, s_fw, rlo - hr, nI, Ib);
-        if (tid < 48) Ib[nI + tid] = 0.0f;  // the MFMA steps read up to nI + 34
+        const long long g0 = (rlo - hr) & ~7LL;
+        const int off = (int)(rlo - hr - g0);
+        const int ng = (off + nI + 7) >> 3;
+        interp_groups<true>(G, ev, g0, ng, Ib);
+        if (tid < 48) Ib[islot(8 * ng + tid)] = 0.0f;  // the MFMA steps read past the tile
+        __syncthreads();
+        // stage 2: rotational broadening, outputs p = off + q for R index q
+        conv_rot_mfma(Ib, hrev, K, off + nR, off, Rb, mask, rlo, G.W);
     } else {
-        interp_tile(G, s_row, s_fw, rlo, nR, Rb);
+        const long long g0 = rlo & ~7LL;
+        const int off = (int)(rlo - g0);
+        const int ng = (off + nR + 7) >> 3;
+        interp_groups<false>(G, ev, g0, ng, Rb);
+        R = Rb + off;
     }
-    if (tid < 16) Rb[nR + tid] = 0.0f;
+    if (tid < 16) R[nR + tid] = 0.0f;
     __syncthreads();
 
-    // stage 2: rotational broadening
-    if (K > 0) {
-        conv_rot_mfma(Ib, hrev, K, nR, Rb, mask, rlo, G.W);
-        __syncthreads();
-    }
-
     // stage 3: instrumental broadening -> C (reuses Ib)
-    const float* Cb = Rb;
+    const float* Cb = R;
     if (S.disp_kind == DISP_FIXED) {
-        conv_tile(Rb, grev, S.Kd, nC, Ib, false, 0, 0);
+        conv_tile(R, grev, S.Kd, nC, Ib, false, 0, 0);
         Cb = Ib;
         __syncthreads();
     } else if (S.disp_kind == DISP_VARIABLE) {
@@ -429,11 +450,11 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
             const long long x = clo + q;
             float c;
             if (x < hd || x >= G.W - hd) {
-                c = Rb[q + hd];
+                c = R[q + hd];
             } else {
                 const float* kr = S.dkern + x * Kd;
                 float a = 0.0f;
-                for (int u = 0; u < Kd; u++) a = fmaf(kr[u], Rb[q + u], a);
+                for (int u = 0; u < Kd; u++) a = fmaf(kr[u], R[q + u], a);
                 c = a;
             }
             Ib[q] = c;
