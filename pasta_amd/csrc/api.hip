@@ -29,8 +29,8 @@ namespace pasta {
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
                        int* sort_hist, int* perm, int n_model,
-                       const int2* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
-                       const int2* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
+                       const int4* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
+                       const int4* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int pmax, int mode,
                        hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st);
 hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, int n, double* pos,
@@ -823,7 +823,8 @@ struct Batch {
     DBuf<float> d_rot, d_model, d_pf;
     DBuf<int> d_status, d_hist, d_perm, d_tbound;
     int tb_stride = 1;
-    DBuf<int2> d_bmom, d_solve, d_bchi2;
+    DBuf<int4> d_bmom, d_bchi2;
+    DBuf<int2> d_solve;
     int n_bm = 0, n_bmom = 0, n_solve = 0, n_bchi2 = 0;
     // batched PSO state
     DBuf<DevPso> d_pso;
@@ -858,7 +859,8 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
     B.n_evals = ne;
     B.h_evals.assign(ne, DevEval{});
     int64_t mo = 0, mp = 0, cp = 0, pf = 0;
-    std::vector<int2> bmom, solve, bchi2;
+    std::vector<int4> bmom, bchi2;
+    std::vector<int2> solve;
     int max_tiles = 0;
     for (int s = 0; s < B.n_stars; s++) max_tiles = std::max(max_tiles, stars[s]->dev.n_tiles);
     for (int s = 0; s < B.n_stars; s++) {
@@ -877,13 +879,23 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
             cp += D.n_cblocks;
             ev.pf_off = pf;
             pf += std::max(1, D.n_params);
-            for (int b = 0; b < D.n_mblocks; b++) bmom.push_back(make_int2(e, b));
-            for (int b = 0; b < D.n_cblocks; b++) bchi2.push_back(make_int2(e, b));
             if (D.cont_kind == CONT_CHUNK)
                 for (int c = 0; c < D.n_chunks; c++) solve.push_back(make_int2(e, c));
             else if (D.cont_kind == CONT_LINEAR || D.cont_kind == CONT_CONSTANT)
                 solve.push_back(make_int2(e, 0));
         }
+    }
+    // continuum blocks: groups of evaluations of one star share each block of pixels
+    const int gm = mom_group(B.pmax), gc = chi2_group(B.pmax);
+    for (int s = 0; s < B.n_stars; s++) {
+        const DevStar& D = stars[s]->dev;
+        const int e0 = B.first_eval[s], e1 = e0 + counts[s];
+        for (int e = e0; e < e1; e += gm)
+            for (int b = 0; b < D.n_mblocks; b++)
+                bmom.push_back(make_int4(e, std::min(gm, e1 - e), b, 0));
+        for (int e = e0; e < e1; e += gc)
+            for (int b = 0; b < D.n_cblocks; b++)
+                bchi2.push_back(make_int4(e, std::min(gc, e1 - e), b, 0));
     }
     // model blocks: tile-major over the evaluations (sorted on the device each launch)
     if ((int64_t)max_tiles * ne > 0x7fffffff) return fail(PASTA_ERR_ARG, "batch too large");

@@ -12,7 +12,12 @@ constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
 constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)
 constexpr int CH_MAX = 32;        // max chunks
 constexpr int P1_MAX = 32;        // max continuum parameters per chunk / linear columns
-constexpr int BLK = 2048;         // observed pixels per reduction block
+constexpr int BLK = 4096;         // observed pixels per reduction block
+
+// Evaluations of one star that one continuum-moment / chi2 workgroup handles together
+// (they share the star's design rows), by the largest parameter count in the batch.
+constexpr int mom_group(int pmax) { return pmax <= 10 ? 4 : (pmax <= 16 ? 2 : 1); }
+constexpr int chi2_group(int pmax) { return pmax <= 16 ? 8 : 4; }
 
 enum DispKind : int { DISP_NONE = 0, DISP_FIXED = 1, DISP_VARIABLE = 2 };
 enum ContKind : int { CONT_CHUNK = 0, CONT_LINEAR = 1, CONT_FIXED = 2, CONT_CONSTANT = 3 };

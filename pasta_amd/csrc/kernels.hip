@@ -481,29 +481,57 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
+// Sums v[0..N) over the 64 lanes of a wave by recursive halving: at each step a lane
+// keeps one half of its values and adds its partner's copy of that half, so N values
+// cost about N exchanges instead of 6N.  Afterwards v[0] of lane l holds the wave sum
+// of value halve_index<N>(l) (an index >= N marks a lane that holds only padding).
+template <int N, int O>
+struct Halve {
+    __device__ static __forceinline__ void run(double* v, int lane) {
+        constexpr int H = (N + 1) / 2;
+        const bool up = (lane & O) != 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+        for (int i = 0; i < H; i++) {
+            const double hi = (H + i < N) ? v[H + i] : 0.0;
+            const double keep = up ? hi : v[i];
+            const double give = up ? v[i] : hi;
+            v[i] = keep + __shfl_xor(give, O, 64);
+        }
+        Halve<H, O / 2>::run(v, lane);
+    }
+};
+template <int N>
+struct Halve<N, 0> {
+    __device__ static __forceinline__ void run(double*, int) {}
+};
+
+template <int N>
+__device__ __forceinline__ int halve_index(int lane) {
+    int idx = 0, n = N, real = N;  // real: how many of the lane's n values are not padding
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int h = (n + 1) / 2;
+        if (lane & o) {
+            idx += h;
+            real = real > h ? real - h : 0;
+        } else {
+            real = real < h ? real : h;
+        }
+        n = h;
+    }
+    return real > 0 ? idx : N;
 }
 
-// Sums acc[0..n) over the workgroup; result valid in thread 0's out[].
-template <int PM>
-__device__ __forceinline__ void block_sum(double (&acc)[PM], int n, double* red /* 4*PM */) {
+// Workgroup sums of v[0..N) (N <= 64): the result for value t is returned to thread t < N.
+template <int N>
+__device__ __forceinline__ double group_sums(double (&v)[N], double* red /* 4*N */) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < PM; k++) {
-        if (k < n) {
-            double v = wave_sum(acc[k]);
-            if (lane == 0) red[wv * PM + k] = v;
-        }
-    }
+    Halve<N, 32>::run(v, lane);
+    const int idx = halve_index<N>(lane);
+    if (idx < N) red[wv * N + idx] = v[0];
     __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < PM; k++)
-            if (k < n) acc[k] = ((red[k] + red[PM + k]) + red[2 * PM + k]) + red[3 * PM + k];
-    }
+    const int t = threadIdx.x;
+    return t < N ? ((red[t] + red[N + t]) + red[2 * N + t]) + red[3 * N + t] : 0.0;
 }
 
 // f32 design-matrix row of a chunk: A[i,k] = f32(powi(xhat, k)), with the
@@ -547,50 +575,101 @@ __device__ __forceinline__ float dot_na(const float (&a)[PM], const float* b, in
     return r;
 }
 
-template <int PM>
+// Continuum moments of E evaluations of one star over one block of observed pixels:
+// block descriptor (first evaluation, count, moment block, 0).  The design row of a
+// pixel is formed once and used for every evaluation of the group.
+template <int PM, int E>
 __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ stars,
                                                 const DevEval* __restrict__ evals,
-                                                const int2* __restrict__ blocks,
+                                                const int4* __restrict__ blocks,
                                                 const float* __restrict__ model,
                                                 double* __restrict__ mpart) {
-    __shared__ double red[4 * PM];
-    const int2 bt = blocks[blockIdx.x];
-    const DevEval& ev = evals[bt.x];
-    if (ev.status != ST_OK) return;
-    const DevStar& S = stars[ev.star];
-    const int4 mb = S.mblocks[bt.y];
-    const float* m = model + ev.m_off;
-    double acc[PM];
+    __shared__ double red[4 * E * PM];
+    const int4 bd = blocks[blockIdx.x];
+    const DevStar& S = stars[evals[bd.x].star];
+    const int4 mb = S.mblocks[bd.z];
+    const float* m[E];
+    bool ok[E];
+    bool any = false;
 #pragma unroll
-    for (int k = 0; k < PM; k++) acc[k] = 0.0;
+    for (int e = 0; e < E; e++) {
+        ok[e] = e < bd.y && evals[bd.x + e].status == ST_OK;
+        m[e] = model + (ok[e] ? evals[bd.x + e].m_off : 0);
+        any |= ok[e];
+    }
+    if (!any) return;
+    double acc[E * PM];
+#pragma unroll
+    for (int k = 0; k < E * PM; k++) acc[k] = 0.0;
     int np = 1;
     if (S.cont_kind == CONT_CHUNK) {
         np = S.p1;
         const double lo = S.chunk_lohi[2 * mb.x], hi = S.chunk_lohi[2 * mb.x + 1];
-        for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
-            const double y = (double)(S.flux[i] / m[i]);
-            float a[PM];
-            design_row<PM>(chunk_xhat(S.wl[i], lo, hi), np, a);
+        // U pixels per thread and round: every load of a round is issued before the
+        // first is used
+        constexpr int U = 4;
+        for (int i0 = mb.y + threadIdx.x; i0 < mb.z; i0 += U * NT) {
+            double wl[U];
+            float fl[U], mv[U][E];
 #pragma unroll
-            for (int k = 0; k < PM; k++)
-                if (k < np) acc[k] = fma((double)a[k], y, acc[k]);
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * NT;
+                const bool in = i < mb.z;
+                wl[u] = in ? S.wl[i] : lo;
+                fl[u] = in ? S.flux[i] : 0.0f;
+#pragma unroll
+                for (int e = 0; e < E; e++) mv[u][e] = (in && ok[e]) ? m[e][i] : 1.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (i0 + u * NT >= mb.z) break;
+                float a[PM];
+                design_row<PM>(chunk_xhat(wl[u], lo, hi), np, a);
+                double ad[PM];
+#pragma unroll
+                for (int k = 0; k < PM; k++) ad[k] = (double)a[k];
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    if (!ok[e]) continue;
+                    const double y = (double)(fl[u] / mv[u][e]);
+#pragma unroll
+                    for (int k = 0; k < PM; k++)
+                        if (k < np) acc[e * PM + k] = fma(ad[k], y, acc[e * PM + k]);
+                }
+            }
         }
     } else if (S.cont_kind == CONT_LINEAR) {
         np = S.p1;
         for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
-            const double y = (double)(S.flux[i] / m[i]);
+            double ad[PM];
 #pragma unroll
-            for (int k = 0; k < PM; k++)
-                if (k < np) acc[k] = fma((double)S.design[(size_t)k * S.N + i], y, acc[k]);
+            for (int k = 0; k < PM; k++) ad[k] = k < np ? (double)S.design[(size_t)k * S.N + i] : 0.0;
+            const float f = S.flux[i];
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                if (!ok[e]) continue;
+                const double y = (double)(f / m[e][i]);
+#pragma unroll
+                for (int k = 0; k < PM; k++)
+                    if (k < np) acc[e * PM + k] = fma(ad[k], y, acc[e * PM + k]);
+            }
         }
     } else {  // constant: sum of f/m
-        for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) acc[0] += (double)(S.flux[i] / m[i]);
-    }
-    block_sum<PM>(acc, np, red);
-    if (threadIdx.x == 0) {
+        for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
+            const float f = S.flux[i];
 #pragma unroll
-        for (int k = 0; k < PM; k++)
-            if (k < np) mpart[ev.mp_off + (size_t)bt.y * np + k] = acc[k];
+            for (int e = 0; e < E; e++)
+                if (ok[e]) acc[e * PM] += (double)(f / m[e][i]);
+        }
+    }
+    const double tot = group_sums<E * PM>(acc, red);
+    const int t = threadIdx.x;
+    if (t < E * PM) {
+        const int e = t / PM, k = t - e * PM;
+        if (e < bd.y && k < np) {
+            const DevEval& ev = evals[bd.x + e];
+            if (ev.status == ST_OK) mpart[ev.mp_off + (size_t)bd.z * np + k] = tot;
+        }
     }
 }
 
@@ -634,78 +713,145 @@ __global__ void k_solve(const DevStar* __restrict__ stars, const DevEval* __rest
     for (int k = 0; k < p1; k++) pf[ev.pf_off + (size_t)c * p1 + k] = (float)bvec[k];
 }
 
-template <int PM>
+// chi2 terms of E evaluations of one star over one block of observed pixels: block
+// descriptor (first evaluation, count, chi2 block, 0); the star's blocks are (kind,
+// chunk, i0, i1) with kind 1 where two chunk polynomials blend.  Design rows are formed
+// once per pixel; each evaluation's continuum parameters are read from LDS.
+template <int PM, int E>
 __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
                                              const DevEval* __restrict__ evals,
-                                             const int2* __restrict__ blocks,
+                                             const int4* __restrict__ blocks,
                                              const float* __restrict__ model,
                                              const float* __restrict__ pf,
                                              double* __restrict__ cpart) {
-    __shared__ double red[4];
-    __shared__ float spf[2 * PM];
-    const int2 bt = blocks[blockIdx.x];
-    const DevEval& ev = evals[bt.x];
-    if (ev.status != ST_OK) return;
-    const DevStar& S = stars[ev.star];
-    const int4 cb = S.cblocks[bt.y];  // (kind, chunk, i0, i1)
-    const float* m = model + ev.m_off;
-    const float* P = pf + ev.pf_off;
-    double acc[1] = {0.0};
-    if (S.cont_kind == CONT_CHUNK) {
-        const int p1 = S.p1, c = cb.y;
-        if (threadIdx.x < p1) {
-            spf[threadIdx.x] = P[c * p1 + threadIdx.x];
-            spf[PM + threadIdx.x] = c > 0 ? P[(c - 1) * p1 + threadIdx.x] : 0.0f;
+    __shared__ double red[4 * E];
+    __shared__ float spf[E][2 * PM];
+    const int4 bd = blocks[blockIdx.x];
+    const DevStar& S = stars[evals[bd.x].star];
+    const int4 cb = S.cblocks[bd.z];  // (kind, chunk, i0, i1)
+    const float* m[E];
+    bool ok[E];
+    bool any = false;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        ok[e] = e < bd.y && evals[bd.x + e].status == ST_OK;
+        m[e] = model + (ok[e] ? evals[bd.x + e].m_off : 0);
+        any |= ok[e];
+    }
+    if (!any) return;
+    double acc[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) acc[e] = 0.0;
+    const int p1 = S.p1;
+    if (S.cont_kind == CONT_CHUNK || S.cont_kind == CONT_LINEAR || S.cont_kind == CONT_CONSTANT) {
+        const int c = S.cont_kind == CONT_CHUNK ? cb.y : 0;
+        const int np = S.cont_kind == CONT_CONSTANT ? 1 : p1;
+        for (int t = threadIdx.x; t < E * PM; t += NT) {
+            const int e = t / PM, k = t - e * PM;
+            float v = 0.0f, vp = 0.0f;
+            if (e < bd.y && k < np) {
+                const float* P = pf + evals[bd.x + e].pf_off;
+                v = P[c * np + k];
+                vp = c > 0 ? P[(c - 1) * np + k] : 0.0f;
+            }
+            spf[e][k] = v;
+            spf[e][PM + k] = vp;
         }
         __syncthreads();
+    }
+    if (S.cont_kind == CONT_CHUNK) {
+        const int c = cb.y;
         const double lo = S.chunk_lohi[2 * c], hi = S.chunk_lohi[2 * c + 1];
         const double plo = c > 0 ? S.chunk_lohi[2 * c - 2] : 0.0;
         const double phi = c > 0 ? S.chunk_lohi[2 * c - 1] : 1.0;
-        for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
-            float a[PM];
-            design_row<PM>(chunk_xhat(S.wl[i], lo, hi), p1, a);
-            float p = dot_na<PM>(a, spf, p1);
-            if (cb.x == 1) {
-                float ap[PM];
-                design_row<PM>(chunk_xhat(S.wl[i], plo, phi), p1, ap);
-                const float prev = dot_na<PM>(ap, spf + PM, p1);
-                const float fac = S.fac[i];
-                p = prev * fac + p * (1.0f - fac);
-            }
-            const float r = S.flux[i] - p * m[i];
-            acc[0] += (double)((r * r) / S.var[i]);
-        }
-    } else if (S.cont_kind == CONT_LINEAR) {
-        const int p1 = S.p1;
-        if (threadIdx.x < p1) spf[threadIdx.x] = P[threadIdx.x];
-        __syncthreads();
-        for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
-            const float bq = S.flux[i] / m[i];
-            float y = 0.0f;
+        const bool blend = cb.x == 1;
+        constexpr int U = 2;  // pixels per thread and round, loads issued first
+        for (int i0 = cb.z + threadIdx.x; i0 < cb.w; i0 += U * NT) {
+            // the parameters stay in LDS (broadcast reads) rather than being hoisted into
+            // E * 2 * PM registers
+            __asm__ volatile("" ::: "memory");
+            double wl[U];
+            float fl[U], vr[U], fc[U], mv[U][E];
 #pragma unroll
-            for (int k = 0; k < PM; k++) {
-                if (k < p1) {
-                    const float t = S.design[(size_t)k * S.N + i] * spf[k];
-                    y = (k == 0) ? t : t + y;
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * NT;
+                const bool in = i < cb.w;
+                wl[u] = in ? S.wl[i] : lo;
+                fl[u] = in ? S.flux[i] : 0.0f;
+                vr[u] = in ? S.var[i] : 1.0f;
+                fc[u] = (in && blend) ? S.fac[i] : 0.0f;
+#pragma unroll
+                for (int e = 0; e < E; e++) mv[u][e] = (in && ok[e]) ? m[e][i] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (i0 + u * NT >= cb.w) break;
+                float a[PM], ap[PM];
+                design_row<PM>(chunk_xhat(wl[u], lo, hi), p1, a);
+                if (blend) design_row<PM>(chunk_xhat(wl[u], plo, phi), p1, ap);
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    if (!ok[e]) continue;
+                    float p = dot_na<PM>(a, spf[e], p1);
+                    if (blend) {
+                        const float prev = dot_na<PM>(ap, spf[e] + PM, p1);
+                        p = prev * fc[u] + p * (1.0f - fc[u]);
+                    }
+                    const float r = fl[u] - p * mv[u][e];
+                    acc[e] += (double)((r * r) / vr[u]);
                 }
             }
-            const float r = y - bq;
-            acc[0] += (double)((r / S.var[i]) * r);
+        }
+    } else if (S.cont_kind == CONT_LINEAR) {
+        for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
+            __asm__ volatile("" ::: "memory");
+            float d[PM];
+#pragma unroll
+            for (int k = 0; k < PM; k++) d[k] = k < p1 ? S.design[(size_t)k * S.N + i] : 0.0f;
+            const float fl = S.flux[i], var = S.var[i];
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                if (!ok[e]) continue;
+                const float bq = fl / m[e][i];
+                float y = 0.0f;
+#pragma unroll
+                for (int k = 0; k < PM; k++) {
+                    if (k < p1) {
+                        const float t = d[k] * spf[e][k];
+                        y = (k == 0) ? t : t + y;
+                    }
+                }
+                const float r = y - bq;
+                acc[e] += (double)((r / var) * r);
+            }
         }
     } else if (S.cont_kind == CONT_FIXED) {
         for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
-            const float r = S.flux[i] - m[i] * S.fixedc[i];
-            acc[0] += (double)((r / S.var[i]) * r);
+            const float fl = S.flux[i], var = S.var[i], fc = S.fixedc[i];
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                if (!ok[e]) continue;
+                const float r = fl - m[e][i] * fc;
+                acc[e] += (double)((r / var) * r);
+            }
         }
     } else {
-        const float mean = P[0];
         for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
-            const float r = S.flux[i] - m[i] * mean;
-            acc[0] += (double)((r / S.var[i]) * r);
+            const float fl = S.flux[i], var = S.var[i];
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                if (!ok[e]) continue;
+                const float r = fl - m[e][i] * spf[e][0];
+                acc[e] += (double)((r / var) * r);
+            }
         }
     }
-    block_sum<1>(acc, 1, red);
-    if (threadIdx.x == 0) cpart[ev.cp_off + bt.y] = acc[0];
+    const double tot = group_sums<E>(acc, red);
+    const int t = threadIdx.x;
+    if (t < E && t < bd.y) {
+        const DevEval& ev = evals[bd.x + t];
+        if (ev.status == ST_OK) cpart[ev.cp_off + bd.z] = tot;
+    }
 }
 
 __global__ void k_final(const DevStar* __restrict__ stars, const DevEval* __restrict__ evals,
@@ -871,8 +1017,8 @@ __global__ void k_sort_scatter(const DevEval* __restrict__ evals, int n, int* __
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
                        int* sort_hist, int* perm, int n_model,
-                       const int2* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
-                       const int2* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
+                       const int4* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
+                       const int4* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int pmax, int mode,
                        hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st) {
     if (n_evals <= 0) return hipSuccess;
@@ -891,22 +1037,27 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
                            n_evals, tbound, tb_stride, model, mode);
     if (ev_model1) (void)hipEventRecord(ev_model1, st);
     if (n_mom > 0) {
-        if (pmax <= 12)
-            hipLaunchKernelGGL(k_moments<12>, dim3(n_mom), dim3(NT), 0, st, stars, evals,
+        static_assert(mom_group(10) == 4 && mom_group(16) == 2 && mom_group(P1_MAX) == 1, "");
+        if (pmax <= 10)
+            hipLaunchKernelGGL((k_moments<10, 4>), dim3(n_mom), dim3(NT), 0, st, stars, evals,
+                               mblk_mom, model, mpart);
+        else if (pmax <= 16)
+            hipLaunchKernelGGL((k_moments<16, 2>), dim3(n_mom), dim3(NT), 0, st, stars, evals,
                                mblk_mom, model, mpart);
         else
-            hipLaunchKernelGGL(k_moments<P1_MAX>, dim3(n_mom), dim3(NT), 0, st, stars, evals,
+            hipLaunchKernelGGL((k_moments<P1_MAX, 1>), dim3(n_mom), dim3(NT), 0, st, stars, evals,
                                mblk_mom, model, mpart);
     }
     if (n_solve > 0)
         hipLaunchKernelGGL(k_solve, dim3((n_solve + 127) / 128), dim3(128), 0, st, stars, evals,
                            solve_items, n_solve, mpart, pf);
     if (n_chi2 > 0) {
-        if (pmax <= 12)
-            hipLaunchKernelGGL(k_chi2<12>, dim3(n_chi2), dim3(NT), 0, st, stars, evals, mblk_chi2,
-                               model, pf, cpart);
+        static_assert(chi2_group(16) == 8 && chi2_group(P1_MAX) == 4, "");
+        if (pmax <= 16)
+            hipLaunchKernelGGL((k_chi2<16, 8>), dim3(n_chi2), dim3(NT), 0, st, stars, evals,
+                               mblk_chi2, model, pf, cpart);
         else
-            hipLaunchKernelGGL(k_chi2<P1_MAX>, dim3(n_chi2), dim3(NT), 0, st, stars, evals,
+            hipLaunchKernelGGL((k_chi2<P1_MAX, 4>), dim3(n_chi2), dim3(NT), 0, st, stars, evals,
                                mblk_chi2, model, pf, cpart);
     }
     hipLaunchKernelGGL(k_final, dim3((n_evals + 127) / 128), dim3(128), 0, st, stars, evals,
