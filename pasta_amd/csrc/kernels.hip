@@ -199,18 +199,35 @@ __device__ __forceinline__ void unpack8(const u32x4 v, float u[8]) {
 // neighbours read row 0 with weight 0 (their weight is 0 in the reference too).
 constexpr int OOB_OFFSET = 0x7ffffff0;
 
-__device__ __forceinline__ void acc8(float (&acc)[8], const u32x4 v, float fw, long long x0,
-                                     bool fast, int Wq) {
+template <bool FAST>
+__device__ __forceinline__ void acc8(float (&acc)[8], const u32x4 v, float fw, long long x0, int Wq) {
     float u[8];
     unpack8(v, u);
-    if (fast) {
 #pragma unroll
-        for (int t = 0; t < 8; t++) acc[t] = fmaf(u[t], fw, acc[t]);
-    } else {
+    for (int t = 0; t < 8; t++) {
+        const float q = (FAST || x0 + t < Wq) ? u[t] : u[t] / 65535.0f;  // tail quirk :978
+        acc[t] = fmaf(q, fw, acc[t]);
+    }
+}
+
+template <bool FAST>
+__device__ __forceinline__ void gather_rows(const DevGrid& G, const DevEval& ev, int voff, long long x0,
+                                            float (&acc)[8]) {
+    const int nrec = (int)(G.pitch * 2);
+#pragma unroll 1
+    for (int j0 = 0; j0 < 64; j0 += 16) {
+        u32x4 v[16];
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const float q = (x0 + t < Wq) ? u[t] : u[t] / 65535.0f;  // tail quirk :978
-            acc[t] = fmaf(q, fw, acc[t]);
+        for (int jj = 0; jj < 16; jj++) {
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)ev.rowp[j0 + jj], 0, nrec, 0x00020000);
+            v[jj] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
+        }
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) {
+            const float fw = __builtin_bit_cast(
+                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, ev.fw[j0 + jj])));
+            acc8<FAST>(acc, v[jj], fw, x0, G.Wq);
         }
     }
 }
@@ -220,31 +237,16 @@ __device__ __forceinline__ int islot(int k) { return k + ((k >> 4) << 2); }
 template <bool PAD>
 __device__ __forceinline__ void interp_groups(const DevGrid& G, const DevEval& ev, long long g0,
                                               int ngroups, float* dst) {
-    const int nrec = (int)(G.pitch * 2);
     for (int gg = threadIdx.x; gg < ngroups; gg += NT) {
         const long long x0 = g0 + 8LL * gg;
         const bool inside = x0 >= 0 && x0 < G.W;
         const int voff = inside ? (int)(2 * x0) : OOB_OFFSET;
-        const bool fast = x0 + 7 < G.Wq;  // all 8 pixels before the n % 128 tail
         float acc[8];
 #pragma unroll
         for (int t = 0; t < 8; t++) acc[t] = 0.0f;
-#pragma unroll 1
-        for (int j0 = 0; j0 < 64; j0 += 16) {
-            u32x4 v[16];
-#pragma unroll
-            for (int jj = 0; jj < 16; jj++) {
-                const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                    (void*)ev.rowp[j0 + jj], 0, nrec, 0x00020000);
-                v[jj] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
-            }
-#pragma unroll
-            for (int jj = 0; jj < 16; jj++) {
-                const float fw = __builtin_bit_cast(
-                    float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, ev.fw[j0 + jj])));
-                acc8(acc, v[jj], fw, x0, fast, G.Wq);
-            }
-        }
+        // all 8 pixels of every lane's group before the n % 128 tail: no quirk handling
+        if (__all(x0 + 7 < G.Wq)) gather_rows<true>(G, ev, voff, x0, acc);
+        else gather_rows<false>(G, ev, voff, x0, acc);
         if (x0 + 8 > G.W) {
 #pragma unroll
             for (int t = 0; t < 8; t++)
