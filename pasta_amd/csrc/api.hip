@@ -31,7 +31,7 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
                        int* sort_hist, int* perm, int n_model,
                        const int4* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
                        const int4* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
-                       double* cpart, double* chi2, int* status, int pmax, int mode,
+                       double* cpart, double* chi2, int* status, int pmax, int p1u, int mode,
                        hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st);
 hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, int n, double* pos,
                              double* vel, const double* bpos, hipStream_t st);
@@ -811,7 +811,7 @@ int pasta_star_eval_bytes(const pasta_star* s, int64_t* bytes) {
 // ---------------------------------------------------------------------------
 struct Batch {
     int mode = 0;  // 0: resample to the observed pixels; 1: synthetic-grid output
-    int n_evals = 0, n_stars = 0, pmax = 1;
+    int n_evals = 0, n_stars = 0, pmax = 1, p1u = 0;
     pasta_grid* grid = nullptr;
     std::vector<pasta_star*> stars;
     std::vector<int> first_eval, count;
@@ -856,6 +856,15 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
         ne += counts[s];
         B.pmax = std::max(B.pmax, stars[s]->dev.p1);
     }
+    // one parameter count shared by every chunk / linear star: compile-time kernels
+    B.p1u = 0;
+    for (int s = 0; s < B.n_stars; s++) {
+        const DevStar& D = stars[s]->dev;
+        if (D.cont_kind != CONT_CHUNK && D.cont_kind != CONT_LINEAR) continue;
+        if (B.p1u == 0) B.p1u = D.p1;
+        else if (B.p1u != D.p1) { B.p1u = -1; break; }
+    }
+    if (B.p1u < 0) B.p1u = 0;
     B.n_evals = ne;
     B.h_evals.assign(ne, DevEval{});
     int64_t mo = 0, mp = 0, cp = 0, pf = 0;
@@ -930,7 +939,7 @@ static int batch_launch(Batch& B) {
     HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p,
                        B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.n_bm, B.d_bmom.p, B.n_bmom, B.d_solve.p, B.n_solve, B.d_bchi2.p,
                        B.n_bchi2, B.d_model.p, B.d_mpart.p, B.d_pf.p, B.d_cpart.p, B.d_chi2.p,
-                       B.d_status.p, B.pmax, B.mode, B.eager_events ? B.ev[0] : nullptr,
+                       B.d_status.p, B.pmax, B.p1u, B.mode, B.eager_events ? B.ev[0] : nullptr,
                        B.eager_events ? B.ev[1] : nullptr, B.stream));
     return PASTA_OK;
 }

@@ -6,7 +6,8 @@
 
 namespace pasta {
 
-constexpr int TILE = 1792;        // synthetic pixels per model-kernel workgroup
+constexpr int TILE = 1776;        // synthetic pixels per model-kernel workgroup: with both
+                                  // halos at most 7 x 256 rotation outputs (whole MFMA tiles)
 constexpr int NT = 256;           // threads per workgroup (4 wavefronts)
 constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
 constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)

@@ -579,8 +579,10 @@ __device__ __forceinline__ float dot_na(const float (&a)[PM], const float* b, in
 
 // Continuum moments of E evaluations of one star over one block of observed pixels:
 // block descriptor (first evaluation, count, moment block, 0).  The design row of a
-// pixel is formed once and used for every evaluation of the group.
-template <int PM, int E>
+// pixel is formed once and used for every evaluation of the group.  P1F > 0 fixes the
+// parameter count per chunk at compile time (every chunk / linear star of the batch has
+// P1F parameters), so the per-parameter guards fold away.
+template <int PM, int E, int P1F>
 __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ stars,
                                                 const DevEval* __restrict__ evals,
                                                 const int4* __restrict__ blocks,
@@ -605,7 +607,7 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
     for (int k = 0; k < E * PM; k++) acc[k] = 0.0;
     int np = 1;
     if (S.cont_kind == CONT_CHUNK) {
-        np = S.p1;
+        np = P1F > 0 ? P1F : S.p1;
         const double lo = S.chunk_lohi[2 * mb.x], hi = S.chunk_lohi[2 * mb.x + 1];
         // U pixels per thread and round: every load of a round is issued before the
         // first is used
@@ -641,7 +643,7 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
             }
         }
     } else if (S.cont_kind == CONT_LINEAR) {
-        np = S.p1;
+        np = P1F > 0 ? P1F : S.p1;
         for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
             double ad[PM];
 #pragma unroll
@@ -719,7 +721,7 @@ __global__ void k_solve(const DevStar* __restrict__ stars, const DevEval* __rest
 // descriptor (first evaluation, count, chi2 block, 0); the star's blocks are (kind,
 // chunk, i0, i1) with kind 1 where two chunk polynomials blend.  Design rows are formed
 // once per pixel; each evaluation's continuum parameters are read from LDS.
-template <int PM, int E>
+template <int PM, int E, int P1F>
 __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
                                              const DevEval* __restrict__ evals,
                                              const int4* __restrict__ blocks,
@@ -744,7 +746,7 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
     double acc[E];
 #pragma unroll
     for (int e = 0; e < E; e++) acc[e] = 0.0;
-    const int p1 = S.p1;
+    const int p1 = P1F > 0 ? P1F : S.p1;
     if (S.cont_kind == CONT_CHUNK || S.cont_kind == CONT_LINEAR || S.cont_kind == CONT_CONSTANT) {
         const int c = S.cont_kind == CONT_CHUNK ? cb.y : 0;
         const int np = S.cont_kind == CONT_CONSTANT ? 1 : p1;
@@ -1021,7 +1023,7 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
                        int* sort_hist, int* perm, int n_model,
                        const int4* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
                        const int4* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
-                       double* cpart, double* chi2, int* status, int pmax, int mode,
+                       double* cpart, double* chi2, int* status, int pmax, int p1u, int mode,
                        hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st) {
     if (n_evals <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64), 0, st, G, stars, labels, evals, rot, tbound,
@@ -1040,27 +1042,31 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
     if (ev_model1) (void)hipEventRecord(ev_model1, st);
     if (n_mom > 0) {
         static_assert(mom_group(10) == 4 && mom_group(16) == 2 && mom_group(P1_MAX) == 1, "");
-        if (pmax <= 10)
-            hipLaunchKernelGGL((k_moments<10, 4>), dim3(n_mom), dim3(NT), 0, st, stars, evals,
-                               mblk_mom, model, mpart);
-        else if (pmax <= 16)
-            hipLaunchKernelGGL((k_moments<16, 2>), dim3(n_mom), dim3(NT), 0, st, stars, evals,
-                               mblk_mom, model, mpart);
-        else
-            hipLaunchKernelGGL((k_moments<P1_MAX, 1>), dim3(n_mom), dim3(NT), 0, st, stars, evals,
-                               mblk_mom, model, mpart);
+#define PASTA_MOM(PM, E, F)                                                                  \
+    hipLaunchKernelGGL((k_moments<PM, E, F>), dim3(n_mom), dim3(NT), 0, st, stars, evals,     \
+                       mblk_mom, model, mpart)
+        if (p1u == 5) PASTA_MOM(10, 4, 5);
+        else if (p1u == 9) PASTA_MOM(10, 4, 9);
+        else if (p1u == 11) PASTA_MOM(16, 2, 11);
+        else if (pmax <= 10) PASTA_MOM(10, 4, 0);
+        else if (pmax <= 16) PASTA_MOM(16, 2, 0);
+        else PASTA_MOM(P1_MAX, 1, 0);
+#undef PASTA_MOM
     }
     if (n_solve > 0)
         hipLaunchKernelGGL(k_solve, dim3((n_solve + 127) / 128), dim3(128), 0, st, stars, evals,
                            solve_items, n_solve, mpart, pf);
     if (n_chi2 > 0) {
         static_assert(chi2_group(16) == 8 && chi2_group(P1_MAX) == 4, "");
-        if (pmax <= 16)
-            hipLaunchKernelGGL((k_chi2<16, 8>), dim3(n_chi2), dim3(NT), 0, st, stars, evals,
-                               mblk_chi2, model, pf, cpart);
-        else
-            hipLaunchKernelGGL((k_chi2<P1_MAX, 4>), dim3(n_chi2), dim3(NT), 0, st, stars, evals,
-                               mblk_chi2, model, pf, cpart);
+#define PASTA_CHI2(PM, E, F)                                                                  \
+    hipLaunchKernelGGL((k_chi2<PM, E, F>), dim3(n_chi2), dim3(NT), 0, st, stars, evals,        \
+                       mblk_chi2, model, pf, cpart)
+        if (p1u == 5) PASTA_CHI2(16, 8, 5);
+        else if (p1u == 9) PASTA_CHI2(16, 8, 9);
+        else if (p1u == 11) PASTA_CHI2(16, 8, 11);
+        else if (pmax <= 16) PASTA_CHI2(16, 8, 0);
+        else PASTA_CHI2(P1_MAX, 4, 0);
+#undef PASTA_CHI2
     }
     hipLaunchKernelGGL(k_final, dim3((n_evals + 127) / 128), dim3(128), 0, st, stars, evals,
                        n_evals, cpart, chi2, status);
