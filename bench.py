@@ -131,6 +131,96 @@ def run_plain(pasta, synth, interp, wlg, spec, iters=200, P=100, seed=2000):
                 chi2=res.chi2, wl=wl, flux=flux, var=var)
 
 
+def serving_requests(pasta, synth, interp, wlg, spec, n, seed=5000):
+    """Serving-config requests: lengths lognormal (median 80,000, clipped 20,000-200,000),
+    labels uniform in the grid, vsini U(0,300), RV U(-150,150), SNR U(30,300), continuum
+    modes cycling chunk(10, 8) / linear (30 smooth columns) / constant."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(n):
+        npx = int(np.clip(rng.lognormal(np.log(80_000), 0.5), 20_000, 200_000))
+        wl = synth.observed_wavelengths(spec, npx, 600)
+        lab = [rng.uniform(4000, 10000), rng.uniform(-1.0, 0.5), rng.uniform(3.0, 5.0),
+               rng.uniform(0, 300), rng.uniform(-150, 150)]
+        disp = pasta.FixedResolutionDispersion(wl, R_INSTR, wlg)
+        flux, var = synth.observe(interp.produce_model(disp, *lab), wl, rng.uniform(30, 300),
+                                  seed + 1 + k)
+        mode = ("chunk", "linear", "constant")[k % 3]
+        reqs.append(dict(wl=wl, disp=disp, flux=flux, var=var, mode=mode, n=npx,
+                         design=synth.smooth_design(npx, 30, seed + k) if mode == "linear" else None))
+    return reqs
+
+
+def _continuum(pasta, r):
+    if r["mode"] == "chunk":
+        return pasta.ChunkContinuumFitter(r["wl"], 10, 8, 0.2)
+    if r["mode"] == "linear":
+        return pasta.LinearModelContinuumFitter(r["design"])
+    return pasta.ConstantContinuum()
+
+
+def pct(x, q):
+    return float(np.percentile(np.asarray(x), q)) if len(x) else None
+
+
+def run_serving(pasta, interp, reqs, rate, n_req, P, iters, iters_per_step, n_plain, seed):
+    """Poisson arrivals at `rate` req/s of the first n_req requests into the server (continuous
+    batching), then the first n_plain of the same arrivals through the plain path, one fit at
+    a time in arrival order.  Latency = finish - scheduled arrival."""
+    rng = np.random.default_rng(seed)
+    arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    settings = pasta.PSOSettings(P, iters)
+    t0 = time.perf_counter()
+    stars = [pasta.Star(interp, reqs[k]["disp"], _continuum(pasta, reqs[k]), reqs[k]["flux"],
+                        reqs[k]["var"]) for k in range(n_req)]
+    prep_ms = (time.perf_counter() - t0) / n_req * 1e3
+    srv = pasta.FitServer(interp, max_stars=1024, max_particles=P, iters_per_step=iters_per_step)
+    try:
+        ids, lat, sub, steps, busy = {}, {}, 0, 0, 0
+        t0 = time.perf_counter()
+        while len(lat) < n_req:
+            now = time.perf_counter() - t0
+            while sub < n_req and arr[sub] <= now:
+                ids[sub] = srv.submit(stars[sub], settings, (0.0, 300.0), (-150.0, 150.0), seed=sub)
+                sub += 1
+            active, waiting = srv.step()
+            steps += 1
+            busy = max(busy, active)
+            now = time.perf_counter() - t0
+            for k, rid in ids.items():
+                if k not in lat and srv.poll(rid) is not None:
+                    lat[k] = now - arr[k]
+            if active == 0 and waiting == 0 and sub < n_req and len(lat) == sub:
+                time.sleep(max(0.0, arr[sub] - (time.perf_counter() - t0)))
+        makespan = time.perf_counter() - t0
+        evals = srv.stats()[0]
+    finally:
+        srv.close()
+    lat_s = [lat[k] for k in range(n_req)]
+    # the plain path, same arrivals, one fit at a time: the service time of each of the
+    # first n_plain requests is measured; FIFO latencies over all n_req arrivals follow
+    # from finish_k = max(arrival_k, finish_(k-1)) + service_(k mod n_plain)
+    from pasta_amd import _lib
+    n_plain = min(n_plain, n_req)
+    pstars = [pasta.Star(interp, reqs[k]["disp"], _continuum(pasta, reqs[k]), reqs[k]["flux"],
+                         reqs[k]["var"]) for k in range(n_plain)]
+    st = pasta.api._settings_c(settings, (0.0, 300.0), (-150.0, 150.0), None)
+    res = _lib.FitResult()
+    service = []
+    for k in range(n_plain):
+        t0 = time.perf_counter()
+        _lib.check(_lib.lib.pasta_fit(pstars[k]._h, C.byref(st), k, None, C.byref(res), None), "fit")
+        service.append(time.perf_counter() - t0)
+    plain_lat, fin = [], 0.0
+    for k in range(n_req):
+        fin = max(arr[k], fin) + service[k % n_plain]
+        plain_lat.append(fin - arr[k])
+    return dict(rate=rate, requests=n_req, p50_s=pct(lat_s, 50), p99_s=pct(lat_s, 99),
+                evals_per_s=evals / makespan, makespan_s=makespan, steps=steps, max_active=busy,
+                star_prep_ms=prep_ms, plain_service_ms=1e3 * float(np.mean(service)),
+                plain_measured=n_plain, plain_p50_s=pct(plain_lat, 50), plain_p99_s=pct(plain_lat, 99))
+
+
 def cpu_baseline(interp, spec, plain, budget_s):
     """The oracle on the host cores over a bounded sample of the plain star's evaluations."""
     from oracle import oracle as O
@@ -186,6 +276,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rates", default="50,200,500")
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="arrival window per rate (requests = rate x seconds, at most 1000)")
+    ap.add_argument("--serve-particles", type=int, default=32)
+    ap.add_argument("--serve-iters", type=int, default=100)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="also run one plain copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -227,6 +323,22 @@ def main():
         plain = run_plain(pasta, synth, interp, wlg, spec, iters=args.plain_iters)
         log(f"[bench] plain: {plain['evals_per_s']:,.0f} evals/s, fit {plain['fit_latency_s']:.2f} s, "
             f"label {np.round(plain['label'], 3).tolist()} true {np.round(plain['true'], 3).tolist()}")
+
+    serving = []
+    if not args.no_serving:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        counts = [int(min(1000, max(20, r * args.serve_seconds))) for r in rates]
+        t0 = time.perf_counter()
+        reqs = serving_requests(pasta, synth, interp, wlg, spec, max(counts))
+        log(f"[bench] {len(reqs)} serving requests generated in {time.perf_counter() - t0:.1f} s")
+        for k, (r, n) in enumerate(zip(rates, counts)):
+            sv = run_serving(pasta, interp, reqs, r, n, args.serve_particles, args.serve_iters,
+                             2, 40, 6000 + k)
+            serving.append(sv)
+            log(f"[bench] serving {r:g} req/s: {n} requests, p50 {sv['p50_s']:.3f} s, p99 "
+                f"{sv['p99_s']:.3f} s, {sv['evals_per_s']:,.0f} evals/s, max {sv['max_active']} "
+                f"stars in flight; plain path one at a time (service {sv['plain_service_ms']:.1f} ms): "
+                f"p50 {sv['plain_p50_s']:.3f} s, p99 {sv['plain_p99_s']:.3f} s")
 
     cpu = None
     if not args.no_cpu_baseline and plain is not None:
@@ -296,6 +408,16 @@ def main():
             "one_core_sample": f"{cpu['n1']} evaluations on 1 thread",
         }
         out["parity_full_size"] = {"max_rel_chi2": cpu["parity_max_rel"], "n": cpu["n_parity"]}
+    if serving:
+        out["serving"] = {
+            "workload": f"Poisson arrivals, lognormal N (median 80k, 20k-200k), continuum modes "
+                        f"chunk(10,8)/linear(30 columns)/constant, PSO {args.serve_particles} "
+                        f"particles x {args.serve_iters} iterations per request; latency from "
+                        f"scheduled arrival to result; each request's star (upload, continuum "
+                        f"factorisation) is prepared before the arrivals start (star_prep_ms)",
+            "rates": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in sv.items()}
+                      for sv in serving],
+        }
     log(f"[bench] total {time.perf_counter() - t_all:.1f} s")
     print(json.dumps(out), flush=True)
 

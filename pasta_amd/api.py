@@ -30,7 +30,7 @@ __all__ = [
     "WlGrid", "InMemInterpolator", "PSOSettings", "Label", "OptimizationResult",
     "NoConvolutionDispersion", "FixedResolutionDispersion", "VariableResolutionDispersion",
     "ChunkContinuumFitter", "LinearModelContinuumFitter", "FixedContinuum", "ConstantContinuum",
-    "FixConstraint", "RangeConstraint", "get_vsini_kernel", "InMemSingleFitter",
+    "FixConstraint", "RangeConstraint", "get_vsini_kernel", "InMemSingleFitter", "FitServer",
 ]
 
 
@@ -585,3 +585,59 @@ class InMemSingleFitter:
         if allow_nan is False and st.any():
             raise ValueError(_lib.STATUS_TEXT.get(int(st[st != 0][0])))
         return out.tolist()
+
+
+class FitServer:
+    """Continuous-batching fit server over a resident grid (pasta_server_*).
+
+    Requests are admitted between steps while slots and HBM allow; every admitted
+    star's particles run in one captured PSO-iteration graph; finished stars are
+    retired at once.  A result is identical alone and in any mix of requests."""
+
+    def __init__(self, interpolator, max_stars=512, max_particles=64, iters_per_step=1,
+                 hbm_budget_bytes=0):
+        cfg = _lib.ServerConfig(int(max_stars), int(max_particles), int(iters_per_step),
+                                int(hbm_budget_bytes), 0)
+        h = C.c_void_p()
+        _raise(lib.pasta_server_create(interpolator._h, C.byref(cfg), C.byref(h)), "server")
+        self._h, self._interp = h, interpolator
+
+    def submit(self, star, settings, vsini_range, rv_range, seed=0, constraints=None):
+        """Queue a fit of `star` (a Star; the server takes it over).  Returns the request id."""
+        s = _settings_c(settings, vsini_range, rv_range, constraints)
+        rid = C.c_uint64()
+        _raise(lib.pasta_server_submit(self._h, star._h, C.byref(s), int(seed), C.byref(rid)),
+               "submit")
+        star.release()
+        return rid.value
+
+    def step(self):
+        """Admit, run iters_per_step PSO iterations of every admitted star, retire.
+        Returns (active stars, waiting requests)."""
+        a, w = C.c_int(), C.c_int()
+        _raise(lib.pasta_server_step(self._h, C.byref(a), C.byref(w)), "step")
+        return a.value, w.value
+
+    def poll(self, request_id):
+        """OptimizationResult of a finished request, else None."""
+        done = C.c_int()
+        r = _lib.FitResult()
+        _raise(lib.pasta_server_poll(self._h, int(request_id), C.byref(done), C.byref(r)), "poll")
+        return OptimizationResult._from_c(r) if done.value else None
+
+    def cancel(self, request_id):
+        _raise(lib.pasta_server_cancel(self._h, int(request_id)), "cancel")
+
+    def stats(self):
+        """(evaluations, requests finished, iterations, HBM bytes in use)."""
+        out = (C.c_int64 * 4)()
+        _raise(lib.pasta_server_stats(self._h, out), "stats")
+        return tuple(out)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.pasta_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

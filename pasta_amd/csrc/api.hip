@@ -29,9 +29,9 @@ namespace pasta {
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
                        int* sort_hist, int* perm, int n_model,
-                       const int4* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
-                       const int4* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
-                       double* cpart, double* chi2, int* status, int pmax, int p1u, int mode,
+                       const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
+                       const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
+                       double* cpart, double* chi2, int* status, int mode,
                        hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st);
 hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, int n, double* pos,
                              double* vel, const double* bpos, hipStream_t st);
@@ -811,7 +811,8 @@ int pasta_star_eval_bytes(const pasta_star* s, int64_t* bytes) {
 // ---------------------------------------------------------------------------
 struct Batch {
     int mode = 0;  // 0: resample to the observed pixels; 1: synthetic-grid output
-    int n_evals = 0, n_stars = 0, pmax = 1, p1u = 0;
+    int n_evals = 0, n_stars = 0;
+    std::vector<ContClass> cls;
     pasta_grid* grid = nullptr;
     std::vector<pasta_star*> stars;
     std::vector<int> first_eval, count;
@@ -850,21 +851,10 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
     B.count = counts;
     B.first_eval.assign(B.n_stars, 0);
     int ne = 0;
-    B.pmax = 1;
     for (int s = 0; s < B.n_stars; s++) {
         B.first_eval[s] = ne;
         ne += counts[s];
-        B.pmax = std::max(B.pmax, stars[s]->dev.p1);
     }
-    // one parameter count shared by every chunk / linear star: compile-time kernels
-    B.p1u = 0;
-    for (int s = 0; s < B.n_stars; s++) {
-        const DevStar& D = stars[s]->dev;
-        if (D.cont_kind != CONT_CHUNK && D.cont_kind != CONT_LINEAR) continue;
-        if (B.p1u == 0) B.p1u = D.p1;
-        else if (B.p1u != D.p1) { B.p1u = -1; break; }
-    }
-    if (B.p1u < 0) B.p1u = 0;
     B.n_evals = ne;
     B.h_evals.assign(ne, DevEval{});
     int64_t mo = 0, mp = 0, cp = 0, pf = 0;
@@ -894,17 +884,36 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
                 solve.push_back(make_int2(e, 0));
         }
     }
-    // continuum blocks: groups of evaluations of one star share each block of pixels
-    const int gm = mom_group(B.pmax), gc = chi2_group(B.pmax);
+    // continuum blocks, per class of stars (continuum kind, parameter count): groups of
+    // evaluations of one star share each block of pixels
+    B.cls.clear();
+    std::vector<int> cls_of(B.n_stars);
     for (int s = 0; s < B.n_stars; s++) {
         const DevStar& D = stars[s]->dev;
-        const int e0 = B.first_eval[s], e1 = e0 + counts[s];
-        for (int e = e0; e < e1; e += gm)
-            for (int b = 0; b < D.n_mblocks; b++)
-                bmom.push_back(make_int4(e, std::min(gm, e1 - e), b, 0));
-        for (int e = e0; e < e1; e += gc)
-            for (int b = 0; b < D.n_cblocks; b++)
-                bchi2.push_back(make_int4(e, std::min(gc, e1 - e), b, 0));
+        const int p1 = D.cont_kind == CONT_CONSTANT ? 1 : (D.cont_kind == CONT_FIXED ? 0 : D.p1);
+        int c = 0;
+        while (c < (int)B.cls.size() && !(B.cls[c].kind == D.cont_kind && B.cls[c].p1 == p1)) c++;
+        if (c == (int)B.cls.size()) B.cls.push_back(ContClass{D.cont_kind, p1, 0, 0, 0, 0});
+        cls_of[s] = c;
+    }
+    for (int c = 0; c < (int)B.cls.size(); c++) {
+        ContClass& K = B.cls[c];
+        const int gm = mom_variant(K.kind, K.p1).e, gc = chi2_variant(K.kind, K.p1).e;
+        K.mom_off = (int)bmom.size();
+        K.chi2_off = (int)bchi2.size();
+        for (int s = 0; s < B.n_stars; s++) {
+            if (cls_of[s] != c) continue;
+            const DevStar& D = stars[s]->dev;
+            const int e0 = B.first_eval[s], e1 = e0 + counts[s];
+            for (int e = e0; e < e1; e += gm)
+                for (int b = 0; b < D.n_mblocks; b++)
+                    bmom.push_back(make_int4(e, std::min(gm, e1 - e), b, 0));
+            for (int e = e0; e < e1; e += gc)
+                for (int b = 0; b < D.n_cblocks; b++)
+                    bchi2.push_back(make_int4(e, std::min(gc, e1 - e), b, 0));
+        }
+        K.n_mom = (int)bmom.size() - K.mom_off;
+        K.n_chi2 = (int)bchi2.size() - K.chi2_off;
     }
     // model blocks: tile-major over the evaluations (sorted on the device each launch)
     if ((int64_t)max_tiles * ne > 0x7fffffff) return fail(PASTA_ERR_ARG, "batch too large");
@@ -937,9 +946,10 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
 static int batch_launch(Batch& B) {
     DevGrid G = B.grid->dev();
     HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p,
-                       B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.n_bm, B.d_bmom.p, B.n_bmom, B.d_solve.p, B.n_solve, B.d_bchi2.p,
-                       B.n_bchi2, B.d_model.p, B.d_mpart.p, B.d_pf.p, B.d_cpart.p, B.d_chi2.p,
-                       B.d_status.p, B.pmax, B.p1u, B.mode, B.eager_events ? B.ev[0] : nullptr,
+                       B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.n_bm, B.d_bmom.p,
+                       B.d_bchi2.p, B.cls.data(), (int)B.cls.size(), B.d_solve.p, B.n_solve,
+                       B.d_model.p, B.d_mpart.p, B.d_pf.p, B.d_cpart.p, B.d_chi2.p,
+                       B.d_status.p, B.mode, B.eager_events ? B.ev[0] : nullptr,
                        B.eager_events ? B.ev[1] : nullptr, B.stream));
     return PASTA_OK;
 }
@@ -1249,6 +1259,7 @@ int pasta_fit(pasta_star* s, const pasta_fit_settings* st, uint64_t seed, const 
 struct PsoBatch {
     Batch B;
     std::vector<DevPso> h_pso;
+    std::vector<double> h_pos, h_vel;  // initial positions / velocities of every evaluation
 };
 
 static int pso_batch_setup(PsoBatch& PB, pasta_grid* g, const std::vector<pasta_star*>& stars,
@@ -1259,7 +1270,10 @@ static int pso_batch_setup(PsoBatch& PB, pasta_grid* g, const std::vector<pasta_
     for (auto* st : sts) counts.push_back(st->num_particles);
     CHK(batch_build(B, g, stars, counts, 0));
     const int ns = (int)stars.size();
-    std::vector<double> pos((size_t)B.n_evals * 5), vel((size_t)B.n_evals * 5);
+    std::vector<double>& pos = PB.h_pos;
+    std::vector<double>& vel = PB.h_vel;
+    pos.assign((size_t)B.n_evals * 5, 0.0);
+    vel.assign((size_t)B.n_evals * 5, 0.0);
     PB.h_pso.assign(ns, DevPso{});
     *max_it = 0;
     for (int s = 0; s < ns; s++) {
@@ -1415,7 +1429,9 @@ struct pasta_server {
     std::deque<Request*> waiting;
     std::vector<Request*> running;
     std::map<uint64_t, std::unique_ptr<Request>> all;
-    std::unique_ptr<PsoBatch> pb;
+    std::unique_ptr<PsoBatch> pb;  // kept across rebuilds: its buffers keep their capacity
+    Batch fin;                     // evaluation of finished stars' best labels (reused)
+    bool active = false;           // pb holds a captured iteration of the running set
     bool dirty = false;
     int64_t evals_done = 0, finished = 0, iters_run = 0;
     double t_model_last = 0, t_model_sum = 0, t_iter_last = 0, t_iter_sum = 0, t_n = 0;
@@ -1473,7 +1489,7 @@ int pasta_server_submit(pasta_server* sv, pasta_star* s, const pasta_fit_setting
 
 // Download the PSO state of every running, started star from the current batch.
 static int server_save_state(pasta_server* sv) {
-    if (!sv->pb) return PASTA_OK;
+    if (!sv->active) return PASTA_OK;
     Batch& B = sv->pb->B;
     std::vector<DevPso> ps(B.n_stars);
     HIPCHK(hipMemcpy(ps.data(), B.d_pso.p, sizeof(DevPso) * B.n_stars, hipMemcpyDeviceToHost));
@@ -1498,9 +1514,15 @@ static int server_save_state(pasta_server* sv) {
 // Build a batch for the running set; started stars resume from their saved state,
 // new stars get their initial population evaluated and sorted.
 static int server_rebuild(pasta_server* sv) {
-    sv->pb.reset();
+    sv->active = false;
     if (sv->running.empty()) return PASTA_OK;
-    auto pb = std::make_unique<PsoBatch>();
+    if (!sv->pb) sv->pb = std::make_unique<PsoBatch>();
+    PsoBatch& PB = *sv->pb;
+    Batch& B = PB.B;
+    if (B.graph) {
+        HIPCHK(hipGraphExecDestroy(B.graph));
+        B.graph = nullptr;
+    }
     std::vector<pasta_star*> stars;
     std::vector<const pasta_fit_settings*> sts;
     std::vector<uint64_t> seeds;
@@ -1510,31 +1532,39 @@ static int server_rebuild(pasta_server* sv) {
         seeds.push_back(r->seed);
     }
     int max_it = 0;
-    CHK(pso_batch_setup(*pb, sv->g, stars, sts, seeds, false, &max_it));
-    Batch& B = pb->B;
-    bool any_new = false;
+    CHK(pso_batch_setup(PB, sv->g, stars, sts, seeds, false, &max_it));
+    bool any_new = false, any_started = false;
+    std::vector<double> bpos(PB.h_pos.size(), 0.0), bcost(B.n_evals, 0.0);
     for (int s = 0; s < B.n_stars; s++) {
         Request* r = sv->running[s];
-        int f0 = B.first_eval[s], P = B.count[s];
+        const int f0 = B.first_eval[s], P = B.count[s];
         if (r->started) {
             DevPso p = r->pso;
             p.first_eval = f0;
-            pb->h_pso[s] = p;
-            HIPCHK(hipMemcpy(B.d_labels.p + 5 * f0, r->pos.data(), 40 * P, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(B.d_vel.p + 5 * f0, r->vel.data(), 40 * P, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(B.d_bpos.p + 5 * f0, r->bpos.data(), 40 * P, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(B.d_bcost.p + f0, r->bcost.data(), 8 * P, hipMemcpyHostToDevice));
+            PB.h_pso[s] = p;
+            std::copy(r->pos.begin(), r->pos.end(), PB.h_pos.begin() + 5 * (size_t)f0);
+            std::copy(r->vel.begin(), r->vel.end(), PB.h_vel.begin() + 5 * (size_t)f0);
+            std::copy(r->bpos.begin(), r->bpos.end(), bpos.begin() + 5 * (size_t)f0);
+            std::copy(r->bcost.begin(), r->bcost.end(), bcost.begin() + f0);
             for (int q = 0; q < P; q++) B.h_evals[f0 + q].active = 0;  // not part of the init pass
+            any_started = true;
         } else {
             any_new = true;
         }
     }
-    CHK(B.d_pso.upload(pb->h_pso));
+    if (any_started) {  // one upload per array for every resumed star
+        CHK(B.d_labels.upload(PB.h_pos));
+        CHK(B.d_vel.upload(PB.h_vel));
+        CHK(B.d_bpos.upload(bpos));
+        CHK(B.d_bcost.upload(bcost));
+    }
+    CHK(B.d_pso.upload(PB.h_pso));
     CHK(B.d_evals.upload(B.h_evals));
-    if (any_new) CHK(pso_first_eval(*pb));
-    for (auto& e : B.ev) HIPCHK(hipEventCreate(&e));
-    CHK(pso_capture(*pb));
-    sv->pb = std::move(pb);
+    if (any_new) CHK(pso_first_eval(PB));
+    for (auto& e : B.ev)
+        if (!e) HIPCHK(hipEventCreate(&e));
+    CHK(pso_capture(PB));
+    sv->active = true;
     return PASTA_OK;
 }
 
@@ -1554,7 +1584,7 @@ static int server_retire(pasta_server* sv) {
         fs.push_back(sv->running[s]->star);
         for (int d = 0; d < 5; d++) lab.push_back(ps[s].gbest[d]);
     }
-    Batch F;
+    Batch& F = sv->fin;
     CHK(batch_build(F, sv->g, fs, std::vector<int>(fs.size(), 1), 0));
     std::vector<double> c(fs.size());
     std::vector<int> st(fs.size());
@@ -1611,7 +1641,7 @@ int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
         CHK(server_rebuild(sv));
         sv->dirty = false;
     }
-    if (sv->pb) {
+    if (sv->active) {
         Batch& B = sv->pb->B;
         for (int it = 0; it < sv->cfg.iters_per_step; it++) {
             if (!sv->cfg.timing) {
@@ -1636,7 +1666,10 @@ int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
         sv->iters_run += sv->cfg.iters_per_step;
         sv->evals_done += (int64_t)B.n_evals * sv->cfg.iters_per_step;
         CHK(server_retire(sv));
-        if (sv->dirty && sv->running.empty()) { sv->pb.reset(); sv->dirty = false; }
+        if (sv->dirty && sv->running.empty()) {
+            sv->active = false;
+            sv->dirty = false;
+        }
     }
     if (active) *active = (int)sv->running.size();
     if (waiting) *waiting = (int)sv->waiting.size();

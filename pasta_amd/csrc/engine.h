@@ -15,13 +15,37 @@ constexpr int CH_MAX = 32;        // max chunks
 constexpr int P1_MAX = 32;        // max continuum parameters per chunk / linear columns
 constexpr int BLK = 4096;         // observed pixels per reduction block
 
-// Evaluations of one star that one continuum-moment / chi2 workgroup handles together
-// (they share the star's design rows), by the largest parameter count in the batch.
-constexpr int mom_group(int pmax) { return pmax <= 10 ? 4 : (pmax <= 16 ? 2 : 1); }
-constexpr int chi2_group(int pmax) { return pmax <= 16 ? 8 : 4; }
-
 enum DispKind : int { DISP_NONE = 0, DISP_FIXED = 1, DISP_VARIABLE = 2 };
 enum ContKind : int { CONT_CHUNK = 0, CONT_LINEAR = 1, CONT_FIXED = 2, CONT_CONSTANT = 3 };
+
+// Continuum-stage kernel variant for one class of stars (one continuum kind and
+// parameter count): pm = register slots per design row, e = evaluations of one star
+// per workgroup (they share its design rows), f = parameter count fixed at compile
+// time (0: read from the star).
+struct KVariant {
+    int pm, e, f;
+};
+__host__ __device__ constexpr KVariant mom_variant(int kind, int p1) {
+    return kind == CONT_CONSTANT ? KVariant{1, 8, 0}
+           : p1 == 5             ? KVariant{10, 4, 5}
+           : p1 == 9             ? KVariant{10, 4, 9}
+           : p1 == 11            ? KVariant{16, 2, 11}
+           : p1 <= 10            ? KVariant{10, 4, 0}
+           : p1 <= 16            ? KVariant{16, 2, 0}
+                                 : KVariant{32, 1, 0};
+}
+__host__ __device__ constexpr KVariant chi2_variant(int kind, int p1) {
+    return (kind == CONT_CONSTANT || kind == CONT_FIXED) ? KVariant{1, 8, 0}
+           : (p1 == 5 || p1 == 9 || p1 == 11)           ? KVariant{16, 8, p1}
+           : p1 <= 16                                    ? KVariant{16, 8, 0}
+                                                         : KVariant{32, 4, 0};
+}
+
+// Stars of one (continuum kind, parameter count) class in a batch: their moment and chi2
+// blocks are [mom_off, mom_off + n_mom) and [chi2_off, chi2_off + n_chi2) of the tables.
+struct ContClass {
+    int kind, p1, mom_off, n_mom, chi2_off, n_chi2;
+};
 
 struct DevGrid {
     const uint16_t* spectra;  // n_models rows of `pitch` uint16 (pitch % 8 == 0)

@@ -580,7 +580,7 @@ __device__ __forceinline__ float dot_na(const float (&a)[PM], const float* b, in
 // Continuum moments of E evaluations of one star over one block of observed pixels:
 // block descriptor (first evaluation, count, moment block, 0).  The design row of a
 // pixel is formed once and used for every evaluation of the group.  P1F > 0 fixes the
-// parameter count per chunk at compile time (every chunk / linear star of the batch has
+// parameter count per chunk at compile time (the launch covers one class of stars with
 // P1F parameters), so the per-parameter guards fold away.
 template <int PM, int E, int P1F>
 __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ stars,
@@ -1021,9 +1021,9 @@ __global__ void k_sort_scatter(const DevEval* __restrict__ evals, int n, int* __
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
                        int* sort_hist, int* perm, int n_model,
-                       const int4* mblk_mom, int n_mom, const int2* solve_items, int n_solve,
-                       const int4* mblk_chi2, int n_chi2, float* model, double* mpart, float* pf,
-                       double* cpart, double* chi2, int* status, int pmax, int p1u, int mode,
+                       const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
+                       const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
+                       double* cpart, double* chi2, int* status, int mode,
                        hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st) {
     if (n_evals <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64), 0, st, G, stars, labels, evals, rot, tbound,
@@ -1040,32 +1040,42 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
         hipLaunchKernelGGL(k_model, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot, perm,
                            n_evals, tbound, tb_stride, model, mode);
     if (ev_model1) (void)hipEventRecord(ev_model1, st);
-    if (n_mom > 0) {
-        static_assert(mom_group(10) == 4 && mom_group(16) == 2 && mom_group(P1_MAX) == 1, "");
+    for (int c = 0; c < n_cls; c++) {
+        const ContClass& K = cls[c];
+        if (K.n_mom <= 0) continue;
+        const KVariant v = mom_variant(K.kind, K.p1);
+        const int4* blk = bmom + K.mom_off;
 #define PASTA_MOM(PM, E, F)                                                                  \
-    hipLaunchKernelGGL((k_moments<PM, E, F>), dim3(n_mom), dim3(NT), 0, st, stars, evals,     \
-                       mblk_mom, model, mpart)
-        if (p1u == 5) PASTA_MOM(10, 4, 5);
-        else if (p1u == 9) PASTA_MOM(10, 4, 9);
-        else if (p1u == 11) PASTA_MOM(16, 2, 11);
-        else if (pmax <= 10) PASTA_MOM(10, 4, 0);
-        else if (pmax <= 16) PASTA_MOM(16, 2, 0);
-        else PASTA_MOM(P1_MAX, 1, 0);
+    if (v.pm == PM && v.e == E && v.f == F)                                                   \
+        hipLaunchKernelGGL((k_moments<PM, E, F>), dim3(K.n_mom), dim3(NT), 0, st, stars, evals, \
+                           blk, model, mpart);
+        PASTA_MOM(1, 8, 0)
+        PASTA_MOM(10, 4, 5)
+        PASTA_MOM(10, 4, 9)
+        PASTA_MOM(16, 2, 11)
+        PASTA_MOM(10, 4, 0)
+        PASTA_MOM(16, 2, 0)
+        PASTA_MOM(32, 1, 0)
 #undef PASTA_MOM
     }
     if (n_solve > 0)
         hipLaunchKernelGGL(k_solve, dim3((n_solve + 127) / 128), dim3(128), 0, st, stars, evals,
                            solve_items, n_solve, mpart, pf);
-    if (n_chi2 > 0) {
-        static_assert(chi2_group(16) == 8 && chi2_group(P1_MAX) == 4, "");
+    for (int c = 0; c < n_cls; c++) {
+        const ContClass& K = cls[c];
+        if (K.n_chi2 <= 0) continue;
+        const KVariant v = chi2_variant(K.kind, K.p1);
+        const int4* blk = bchi2 + K.chi2_off;
 #define PASTA_CHI2(PM, E, F)                                                                  \
-    hipLaunchKernelGGL((k_chi2<PM, E, F>), dim3(n_chi2), dim3(NT), 0, st, stars, evals,        \
-                       mblk_chi2, model, pf, cpart)
-        if (p1u == 5) PASTA_CHI2(16, 8, 5);
-        else if (p1u == 9) PASTA_CHI2(16, 8, 9);
-        else if (p1u == 11) PASTA_CHI2(16, 8, 11);
-        else if (pmax <= 16) PASTA_CHI2(16, 8, 0);
-        else PASTA_CHI2(P1_MAX, 4, 0);
+    if (v.pm == PM && v.e == E && v.f == F)                                                    \
+        hipLaunchKernelGGL((k_chi2<PM, E, F>), dim3(K.n_chi2), dim3(NT), 0, st, stars, evals,   \
+                           blk, model, pf, cpart);
+        PASTA_CHI2(1, 8, 0)
+        PASTA_CHI2(16, 8, 5)
+        PASTA_CHI2(16, 8, 9)
+        PASTA_CHI2(16, 8, 11)
+        PASTA_CHI2(16, 8, 0)
+        PASTA_CHI2(32, 4, 0)
 #undef PASTA_CHI2
     }
     hipLaunchKernelGGL(k_final, dim3((n_evals + 127) / 128), dim3(128), 0, st, stars, evals,
