@@ -28,6 +28,13 @@ namespace pasta {
 // in distinct LDS banks.  Logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KROT_MAX + 55.
 constexpr int ILOG = 4160;
 constexpr int ILDS = ILOG + ILOG / 4;                               // 5200 slots
+// Rotation kernels of up to KB_MAX taps run on bf16 matrix cores with a three-way split:
+// the tile is kept as three bf16 arrays of ILOGB elements (logical extent <= 7 +
+// (TILE + 1 + 2*HD_MAX) + KB_MAX + 71), the taps as three split parts in two copies
+// (even / odd alignment) of packed bf16 pairs, TW_MAX pairs each.
+constexpr int KB_MAX = 640;
+constexpr int ILOGB = 2752;
+constexpr int TW_MAX = (32 * ((KB_MAX + 15 + 31) / 32) + 32) / 2;  // 352
 constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 8 + 16 + 8;           // 2080
 constexpr double C_KMS = 299792.0;   // src/convolve_rv.rs:440,547
 constexpr double LN10 = 2.302585092994045684017991454684364208;
@@ -234,7 +241,27 @@ __device__ __forceinline__ void gather_rows(const DevGrid& G, const DevEval& ev,
 
 __device__ __forceinline__ int islot(int k) { return k + ((k >> 4) << 2); }
 
-template <bool PAD>
+// Three-way bf16 split x = hi + mid + lo (each round-to-nearest; both differences are
+// exact in f32), so that products of two split values keep f32 accuracy when the
+// three smallest cross terms are dropped (relative error <= ~2^-23).
+__device__ __forceinline__ void split3(float x, unsigned short& h, unsigned short& m,
+                                       unsigned short& l) {
+    const __bf16 bh = (__bf16)x;
+    const float r = x - (float)bh;
+    const __bf16 bm = (__bf16)r;
+    const __bf16 bl = (__bf16)(r - (float)bm);
+    h = __builtin_bit_cast(unsigned short, bh);
+    m = __builtin_bit_cast(unsigned short, bm);
+    l = __builtin_bit_cast(unsigned short, bl);
+}
+
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+// OUT_F32: dst[k]; OUT_PAD: dst[islot(k)]; OUT_BF3: three bf16 arrays of ILOGB elements
+// (hi, mid, lo) starting at (unsigned short*)dst.
+enum InterpOut { OUT_F32 = 0, OUT_PAD = 1, OUT_BF3 = 2 };
+
+template <int OUT>
 __device__ __forceinline__ void interp_groups(const DevGrid& G, const DevEval& ev, long long g0,
                                               int ngroups, float* dst) {
     for (int gg = threadIdx.x; gg < ngroups; gg += NT) {
@@ -252,9 +279,25 @@ __device__ __forceinline__ void interp_groups(const DevGrid& G, const DevEval& e
             for (int t = 0; t < 8; t++)
                 if (x0 + t >= G.W) acc[t] = 0.0f;
         }
-        float* d = dst + (PAD ? islot(8 * gg) : 8 * gg);
-        *(f4*)d = f4{acc[0], acc[1], acc[2], acc[3]};
-        *(f4*)(d + 4) = f4{acc[4], acc[5], acc[6], acc[7]};
+        if (OUT == OUT_BF3) {
+            u16x8 h, m, l;
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                unsigned short a, b, c;
+                split3(acc[t], a, b, c);
+                h[t] = a;
+                m[t] = b;
+                l[t] = c;
+            }
+            unsigned short* d = (unsigned short*)dst + 8 * gg;
+            *(u16x8*)d = h;
+            *(u16x8*)(d + ILOGB) = m;
+            *(u16x8*)(d + 2 * ILOGB) = l;
+        } else {
+            float* d = dst + (OUT == OUT_PAD ? islot(8 * gg) : 8 * gg);
+            *(f4*)d = f4{acc[0], acc[1], acc[2], acc[3]};
+            *(f4*)(d + 4) = f4{acc[4], acc[5], acc[6], acc[7]};
+        }
     }
 }
 
@@ -355,6 +398,86 @@ __device__ __forceinline__ void conv_rot_mfma(const float* Ipad, const float* hp
     }
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4b __attribute__((ext_vector_type(4)));
+
+// Rotational broadening for K <= KB_MAX on v_mfma_f32_16x16x32_bf16, the same Toeplitz
+// product as conv_rot_mfma with 32 contraction steps per instruction.  Each operand is
+// split into bf16 hi + mid + lo and the six products hi*hi, hi*mid, mid*hi, hi*lo,
+// lo*hi, mid*mid are accumulated in f32 (the dropped terms are below 2^-23 relative).
+// Operand layout (checked on gfx950): A lane l holds A[l%16][8(l/16)+t], B lane l holds
+// B[8(l/16)+t][l%16], t < 8, D as for the f32 instruction.  Lane (m, kk) of group c
+// takes j = 32c + 8kk + t: its A operands are I[16a + 32c + 8kk .. +7] (one 16-byte read
+// per split part), its B operands the padded taps tpad[32c + 8kk + 16 - n .. +7]
+// (tpad[16 + u] = tr[u]), read as four packed pairs from the copy whose pairs start at
+// the parity of that index: tw[part][p][q] = (tpad[2q + p], tpad[2q + p + 1]).
+__device__ __forceinline__ void conv_rot_bf16(const unsigned short* Ib3, const unsigned int* tw,
+                                              int K, int np, int off, float* out, bool mask,
+                                              long long x0, int W) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nblk = (np + 15) >> 4;
+    const int tiles = (nblk + 15) >> 4;
+    const int C = (K + 15 + 31) >> 5;
+    const int m = lane & 15, kk = lane >> 4;
+    const int s0 = 8 * kk + 16 - m;  // tpad index of this lane's first B operand, group 0
+    const unsigned int* twp = tw + (s0 & 1) * TW_MAX + (s0 >> 1);
+    for (int t0 = wv; t0 < tiles; t0 += 8) {
+        const int t1 = t0 + 4;
+        const bool has1 = t1 < tiles;
+        const int a0 = min(16 * t0 + m, nblk - 1), a1 = min(16 * t1 + m, nblk - 1);
+        const unsigned short* ia0 = Ib3 + 16 * a0 + 8 * kk;
+        const unsigned short* ia1 = Ib3 + 16 * a1 + 8 * kk;
+        f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int c = 0; c < C; c++) {
+            bf16x8 bh, bm, bl;
+            {
+                const unsigned int* q = twp + 16 * c;
+                const u32x4b vh = {q[0], q[1], q[2], q[3]};
+                const u32x4b vm = {q[2 * TW_MAX], q[2 * TW_MAX + 1], q[2 * TW_MAX + 2], q[2 * TW_MAX + 3]};
+                const u32x4b vl = {q[4 * TW_MAX], q[4 * TW_MAX + 1], q[4 * TW_MAX + 2], q[4 * TW_MAX + 3]};
+                bh = __builtin_bit_cast(bf16x8, vh);
+                bm = __builtin_bit_cast(bf16x8, vm);
+                bl = __builtin_bit_cast(bf16x8, vl);
+            }
+            const unsigned short* p0 = ia0 + 32 * c;
+            const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const u16x8*)p0);
+            const bf16x8 am = __builtin_bit_cast(bf16x8, *(const u16x8*)(p0 + ILOGB));
+            const bf16x8 al = __builtin_bit_cast(bf16x8, *(const u16x8*)(p0 + 2 * ILOGB));
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c0, 0, 0, 0);
+            if (has1) {
+                const unsigned short* p1 = ia1 + 32 * c;
+                const bf16x8 xh = __builtin_bit_cast(bf16x8, *(const u16x8*)p1);
+                const bf16x8 xm = __builtin_bit_cast(bf16x8, *(const u16x8*)(p1 + ILOGB));
+                const bf16x8 xl = __builtin_bit_cast(bf16x8, *(const u16x8*)(p1 + 2 * ILOGB));
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xm, bm, c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, bh, c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, bl, c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xm, bh, c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, bm, c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, bh, c1, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int p = 16 * (16 * t0 + 4 * kk + i) + m;
+            if (p >= off && p < np) {
+                const long long x = x0 + (p - off);
+                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c0[i];
+            }
+            p = 16 * (16 * t1 + 4 * kk + i) + m;
+            if (has1 && p >= off && p < np) {
+                const long long x = x0 + (p - off);
+                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c1[i];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_model: one workgroup per (evaluation, synthetic tile)
 // ---------------------------------------------------------------------------
@@ -364,10 +487,17 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
                                               const int* __restrict__ perm, int n_evals,
                                               const int* __restrict__ tbound, int tb_stride,
                                               float* __restrict__ model, int mode) {
-    __shared__ float Ib[ILDS];
+    // Rotation stage storage, shared by its two forms: f32 (padded tile Ib, reversed taps
+    // hrev: zero, then the taps from index 16, zeros after) or bf16 x 3 (tile arrays Ib3,
+    // split tap pairs tw).  Ib also holds the instrumental-broadening output.
+    static_assert(3 * ILOGB * 2 + 6 * TW_MAX * 4 <= (ILDS + KROT_MAX + 48) * 4, "LDS union");
+    __shared__ __align__(16) float lds_rot[ILDS + KROT_MAX + 48];
     __shared__ float Rb[RLDS];
-    __shared__ float hrev[KROT_MAX + 48];  // zero, then reversed taps from index 16, zeros after
     __shared__ float grev[2 * HD_MAX + 1 + 8];
+    float* const Ib = lds_rot;
+    float* const hrev = lds_rot + ILDS;
+    unsigned short* const Ib3 = (unsigned short*)lds_rot;
+    unsigned int* const tw = (unsigned int*)(lds_rot + (3 * ILOGB) / 2);
 
     // tile-major over evaluations sorted by neighbour rows: the workgroups resident on
     // one XCD work on one tile of evaluations that gather overlapping rows
@@ -404,7 +534,24 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     const int nI = nR + 2 * hr;
     const bool mask = S.disp_kind != DISP_NONE;
 
-    if (K > 0) {
+    const bool bf3 = K > 0 && K <= KB_MAX;
+    if (bf3) {
+        // tw[part][parity][q] = (tpad[2q + parity], tpad[2q + parity + 1]), part = hi/mid/lo
+        const float* h = rot + (size_t)bt.x * KROT_MAX;
+        const int nq = 16 * ((K + 46) >> 5) + 8;  // every pair the MFMA groups read
+        for (int i = tid; i < 2 * nq; i += NT) {
+            const int par = i >= nq, q = i - par * nq;
+            unsigned short e[2][3];
+#pragma unroll
+            for (int z = 0; z < 2; z++) {
+                const int u = 2 * q + par + z - 16;
+                split3((u >= 0 && u < K) ? h[K - 1 - u] : 0.0f, e[z][0], e[z][1], e[z][2]);
+            }
+#pragma unroll
+            for (int part = 0; part < 3; part++)
+                tw[(2 * part + par) * TW_MAX + q] = (unsigned int)e[0][part] | ((unsigned int)e[1][part] << 16);
+        }
+    } else if (K > 0) {
         const float* h = rot + (size_t)bt.x * KROT_MAX;
         const int Kp = 16 * ((K + 30) >> 4) + 16;  // every index the MFMA steps read
         for (int i = tid; i < Kp; i += NT) {
@@ -421,11 +568,24 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     // pixel; `off` is the first pixel's logical index.  With rotation the tile goes to
     // Ib (padded slots), else straight into Rb, read from R = Rb + off.
     float* R = Rb;
-    if (K > 0) {
+    if (bf3) {
         const long long g0 = (rlo - hr) & ~7LL;
         const int off = (int)(rlo - hr - g0);
         const int ng = (off + nI + 7) >> 3;
-        interp_groups<true>(G, ev, g0, ng, Ib);
+        interp_groups<OUT_BF3>(G, ev, g0, ng, Ib);
+        if (tid < 64) {  // the MFMA groups read past the tile
+            Ib3[8 * ng + tid] = 0;
+            Ib3[ILOGB + 8 * ng + tid] = 0;
+            Ib3[2 * ILOGB + 8 * ng + tid] = 0;
+        }
+        __syncthreads();
+        // stage 2: rotational broadening, outputs p = off + q for R index q
+        conv_rot_bf16(Ib3, tw, K, off + nR, off, Rb, mask, rlo, G.W);
+    } else if (K > 0) {
+        const long long g0 = (rlo - hr) & ~7LL;
+        const int off = (int)(rlo - hr - g0);
+        const int ng = (off + nI + 7) >> 3;
+        interp_groups<OUT_PAD>(G, ev, g0, ng, Ib);
         if (tid < 48) Ib[islot(8 * ng + tid)] = 0.0f;  // the MFMA steps read past the tile
         __syncthreads();
         // stage 2: rotational broadening, outputs p = off + q for R index q
@@ -434,7 +594,7 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
         const long long g0 = rlo & ~7LL;
         const int off = (int)(rlo - g0);
         const int ng = (off + nR + 7) >> 3;
-        interp_groups<false>(G, ev, g0, ng, Rb);
+        interp_groups<OUT_F32>(G, ev, g0, ng, Rb);
         R = Rb + off;
     }
     if (tid < 16) R[nR + tid] = 0.0f;
