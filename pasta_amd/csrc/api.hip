@@ -794,7 +794,7 @@ int pasta_star_destroy(pasta_star* s) {
 
 static int64_t star_eval_bytes(const pasta_star* s) {
     const DevStar& D = s->dev;
-    int64_t b = (int64_t)D.N * 4 + sizeof(DevEval) + KROT_MAX * 4 + 5 * 8 * 4;
+    int64_t b = (int64_t)D.N * 4 + sizeof(DevEval) + ROT_STRIDE * 4 + 5 * 8 * 4;
     b += (int64_t)D.n_mblocks * std::max(1, D.p1) * 8 + (int64_t)D.n_cblocks * 8 + D.n_params * 4;
     b += (int64_t)(D.n_tiles + D.n_mblocks + D.n_cblocks + std::max(1, D.n_chunks)) * 8;
     return b;
@@ -925,7 +925,7 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
     CHK(B.d_stars.upload(ds));
     CHK(B.d_evals.upload(B.h_evals));
     CHK(B.d_labels.alloc((size_t)ne * 5));
-    CHK(B.d_rot.alloc((size_t)ne * KROT_MAX));
+    CHK(B.d_rot.alloc((size_t)ne * ROT_STRIDE));
     CHK(B.d_model.alloc((size_t)std::max<int64_t>(mo, 1)));
     CHK(B.d_mpart.alloc((size_t)std::max<int64_t>(mp, 1)));
     CHK(B.d_cpart.alloc((size_t)std::max<int64_t>(cp, 1)));

@@ -11,6 +11,12 @@ constexpr int TILE = 1776;        // synthetic pixels per model-kernel workgroup
 constexpr int NT = 256;           // threads per workgroup (4 wavefronts)
 constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
 constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)
+// Rotation kernels of up to KB_MAX taps run on bf16 matrix cores with a three-way operand
+// split; their taps are kept as 3 split parts x 2 alignments x TW_MAX packed bf16 pairs.
+constexpr int KB_MAX = 640;
+constexpr int TW_MAX = (32 * ((KB_MAX + 15 + 31) / 32) + 32) / 2;  // 352
+// per-evaluation rotation buffer: f32 taps, then the split pairs
+constexpr int ROT_STRIDE = KROT_MAX + 6 * TW_MAX;
 constexpr int CH_MAX = 32;        // max chunks
 constexpr int P1_MAX = 32;        // max continuum parameters per chunk / linear columns
 constexpr int BLK = 4096;         // observed pixels per reduction block

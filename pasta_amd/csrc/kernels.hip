@@ -28,13 +28,9 @@ namespace pasta {
 // in distinct LDS banks.  Logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KROT_MAX + 55.
 constexpr int ILOG = 4160;
 constexpr int ILDS = ILOG + ILOG / 4;                               // 5200 slots
-// Rotation kernels of up to KB_MAX taps run on bf16 matrix cores with a three-way split:
-// the tile is kept as three bf16 arrays of ILOGB elements (logical extent <= 7 +
-// (TILE + 1 + 2*HD_MAX) + KB_MAX + 71), the taps as three split parts in two copies
-// (even / odd alignment) of packed bf16 pairs, TW_MAX pairs each.
-constexpr int KB_MAX = 640;
+// For the bf16 rotation the tile is kept as three bf16 arrays of ILOGB elements
+// (logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KB_MAX + 71).
 constexpr int ILOGB = 2752;
-constexpr int TW_MAX = (32 * ((KB_MAX + 15 + 31) / 32) + 32) / 2;  // 352
 constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 8 + 16 + 8;           // 2080
 constexpr double C_KMS = 299792.0;   // src/convolve_rv.rs:440,547
 constexpr double LN10 = 2.302585092994045684017991454684364208;
@@ -76,6 +72,20 @@ __device__ __forceinline__ int obs_lower_bound(const double* __restrict__ fidx, 
         else lo = mid + 1;
     }
     return lo;
+}
+
+// Three-way bf16 split x = hi + mid + lo (each round-to-nearest; both differences are
+// exact in f32), so that products of two split values keep f32 accuracy when the
+// three smallest cross terms are dropped (relative error <= ~2^-23).
+__device__ __forceinline__ void split3(float x, unsigned short& h, unsigned short& m,
+                                       unsigned short& l) {
+    const __bf16 bh = (__bf16)x;
+    const float r = x - (float)bh;
+    const __bf16 bm = (__bf16)r;
+    const __bf16 bl = (__bf16)(r - (float)bm);
+    h = __builtin_bit_cast(unsigned short, bh);
+    m = __builtin_bit_cast(unsigned short, bm);
+    l = __builtin_bit_cast(unsigned short, bl);
 }
 
 // ---------------------------------------------------------------------------
@@ -158,7 +168,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     }
     const int K = s_K;
     if (K > 0) {
-        float* h = rot + (size_t)e * KROT_MAX;
+        float* h = rot + (size_t)e * ROT_STRIDE;
         const double eps = 0.6;
         const double vrot = vsini / C_KMS;
         const double mid = ((double)(K - 1) * dvelo) / 2.0;
@@ -181,6 +191,26 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
         __syncthreads();
         const float s = s_sum;
         for (int i = lane; i < K; i += 64) h[i] = h[i] / s;
+        if (K <= KB_MAX) {
+            // reversed, zero-padded taps tpad[16 + u] = h[K - 1 - u], split into bf16
+            // hi/mid/lo, as pairs tw[part][parity][q] = (tpad[2q + parity], tpad[2q + parity + 1])
+            __syncthreads();
+            unsigned int* tw = (unsigned int*)(h + KROT_MAX);
+            const int nq = 16 * ((K + 46) >> 5) + 8;  // every pair the MFMA groups read
+            for (int i = lane; i < 2 * nq; i += 64) {
+                const int par = i >= nq, q = i - par * nq;
+                unsigned short v[2][3];
+#pragma unroll
+                for (int z = 0; z < 2; z++) {
+                    const int u = 2 * q + par + z - 16;
+                    split3((u >= 0 && u < K) ? h[K - 1 - u] : 0.0f, v[z][0], v[z][1], v[z][2]);
+                }
+#pragma unroll
+                for (int part = 0; part < 3; part++)
+                    tw[(2 * part + par) * TW_MAX + q] =
+                        (unsigned int)v[0][part] | ((unsigned int)v[1][part] << 16);
+            }
+        }
     }
 }
 
@@ -240,20 +270,6 @@ __device__ __forceinline__ void gather_rows(const DevGrid& G, const DevEval& ev,
 }
 
 __device__ __forceinline__ int islot(int k) { return k + ((k >> 4) << 2); }
-
-// Three-way bf16 split x = hi + mid + lo (each round-to-nearest; both differences are
-// exact in f32), so that products of two split values keep f32 accuracy when the
-// three smallest cross terms are dropped (relative error <= ~2^-23).
-__device__ __forceinline__ void split3(float x, unsigned short& h, unsigned short& m,
-                                       unsigned short& l) {
-    const __bf16 bh = (__bf16)x;
-    const float r = x - (float)bh;
-    const __bf16 bm = (__bf16)r;
-    const __bf16 bl = (__bf16)(r - (float)bm);
-    h = __builtin_bit_cast(unsigned short, bh);
-    m = __builtin_bit_cast(unsigned short, bm);
-    l = __builtin_bit_cast(unsigned short, bl);
-}
 
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
@@ -535,24 +551,11 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     const bool mask = S.disp_kind != DISP_NONE;
 
     const bool bf3 = K > 0 && K <= KB_MAX;
-    if (bf3) {
-        // tw[part][parity][q] = (tpad[2q + parity], tpad[2q + parity + 1]), part = hi/mid/lo
-        const float* h = rot + (size_t)bt.x * KROT_MAX;
-        const int nq = 16 * ((K + 46) >> 5) + 8;  // every pair the MFMA groups read
-        for (int i = tid; i < 2 * nq; i += NT) {
-            const int par = i >= nq, q = i - par * nq;
-            unsigned short e[2][3];
-#pragma unroll
-            for (int z = 0; z < 2; z++) {
-                const int u = 2 * q + par + z - 16;
-                split3((u >= 0 && u < K) ? h[K - 1 - u] : 0.0f, e[z][0], e[z][1], e[z][2]);
-            }
-#pragma unroll
-            for (int part = 0; part < 3; part++)
-                tw[(2 * part + par) * TW_MAX + q] = (unsigned int)e[0][part] | ((unsigned int)e[1][part] << 16);
-        }
+    if (bf3) {  // split taps prepared per evaluation by k_prep
+        const u32x4* src = (const u32x4*)(rot + (size_t)bt.x * ROT_STRIDE + KROT_MAX);
+        for (int i = tid; i < 6 * TW_MAX / 4; i += NT) ((u32x4*)tw)[i] = src[i];
     } else if (K > 0) {
-        const float* h = rot + (size_t)bt.x * KROT_MAX;
+        const float* h = rot + (size_t)bt.x * ROT_STRIDE;
         const int Kp = 16 * ((K + 30) >> 4) + 16;  // every index the MFMA steps read
         for (int i = tid; i < Kp; i += NT) {
             const int u = i - 16;

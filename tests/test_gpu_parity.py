@@ -91,6 +91,23 @@ def test_interpolate_and_convolve_matches_oracle(small, gpu_small):
         assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) < RTOL, lab
 
 
+def test_rotation_paths_match_oracle(small, gpu_small):
+    """Rotation kernels of up to 640 taps run on bf16 matrix cores (three-way split),
+    wider ones on f32 matrix cores: vsini values on both sides of the switch (K ~ 626,
+    648 taps on this grid) and far beyond it (K ~ 1100, 1840)."""
+    pasta, wlg, interp = gpu_small
+    d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
+    labs = np.array([[6123.0, -0.3, 4.21, v, 5.0] for v in (340.0, 352.0, 600.0, 1000.0)])
+    dvelo = np.log(10.0) * small.spec.log10_step
+    ks = [len(O.rotation_kernel(v, dvelo)) for v in labs[:, 3]]
+    assert ks[0] <= 640 < ks[1] and ks[3] > 1500, ks
+    print(f"rotation paths: taps {ks}", flush=True)
+    out = interp.produce_model_bulk(d, labs)
+    for k, lab in enumerate(labs):
+        ref = O.produce_model(small.og, small.odisp, lab)
+        assert _rel(out[k], ref) < RTOL, (lab.tolist(), _rel(out[k], ref))
+
+
 def _cont_pair(small, pasta, kind):
     n = len(small.wl)
     if kind == "chunk":
