@@ -38,6 +38,10 @@ extern "C" {
 #define PASTA_ST_INDEX 6      /* resample reads past the model segment (the reference panics) */
 #define PASTA_ST_PSO 7        /* src/particleswarm.rs:267-275 (particle left the bounds) */
 #define PASTA_ST_SOLVE 8      /* continuum normal equations not positive definite */
+#define PASTA_ST_SIGN 9       /* compute_uncertainty: chi2 - target has one sign at both ends of
+                                 the search bracket (argmin BrentRoot's error, fitting.rs:538-545) */
+#define PASTA_ST_CLAMP 10     /* compute_uncertainty: the search bound cannot be clamped into the
+                                 grid (bounds.rs:13-16 via fitting.rs:532) */
 
 const char *pasta_last_error(void);
 int pasta_version(void);
@@ -189,6 +193,24 @@ int pasta_fit(pasta_star *s, const pasta_fit_settings *st, uint64_t seed, const 
  * every star in one device-driven, hipGraph-captured PSO iteration. */
 int pasta_fit_bulk(pasta_star *const *stars, int n_stars, const pasta_fit_settings *st,
                    const uint64_t *seeds, pasta_fit_result *out, double *trace);
+
+/* ----------------------------------------------------------- uncertainty --
+ * SingleFitter::compute_uncertainty (src/fitting.rs:482-568; Python entry and
+ * default search radius (2000, 0.3, 0.3, 40, 40) at src/lib.rs:1545-1568): for every
+ * label i, the distances below (out[2i]) and above (out[2i+1]) `label` at which
+ * chi2 crosses best * (1 + sqrt(2/n)), n = 4 R (wl_last - wl_first) / (wl_first +
+ * wl_last), found by Brent root searches (argmin BrentRoot, tolerance 1e-3 of the
+ * search radius).  status[j] = 0, or the PASTA_ST_* reason that side has no value
+ * (the reference's None).  search_radius NULL = the defaults.  Returns
+ * PASTA_ERR_MODEL when the model at `label` itself fails (the reference raises). */
+int pasta_compute_uncertainty(pasta_star *s, double spec_res, const double label[5],
+                              const double search_radius[5], double out[10], int status[10]);
+/* compute_uncertainty_bulk (src/lib.rs:1570-1600): n stars, labels n x 5, out and
+ * status n x 10; all searches advance together, one batched evaluation per round.  A
+ * star whose best model fails gets that status on all ten sides. */
+int pasta_compute_uncertainty_bulk(pasta_star *const *stars, int n, double spec_res,
+                                   const double *labels, const double *search_radius,
+                                   double *out, int *status);
 
 /* ---------------------------------------------------------------- server --
  * Continuous batching of fit requests over a resident grid: requests are

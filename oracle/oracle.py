@@ -57,6 +57,8 @@ class _PsoSettings(C.Structure):
 
 COST_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, _dp, _dp)
 
+SCALAR_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_double))
+
 _sig = {
     "or_grid_from_labels": (C.c_void_p, [_dp, C.c_int]),
     "or_grid_free": (None, [C.c_void_p]),
@@ -111,6 +113,9 @@ _sig = {
     "or_fit": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, C.POINTER(_Bounds),
                          C.POINTER(_PsoSettings), C.c_uint64, _dp, C.c_int, _dp, _dp, _fp,
                          C.POINTER(C.c_long)]),
+    "or_brent_root": (C.c_int, [SCALAR_FN, C.c_void_p, C.c_double, C.c_double, C.c_double, _dp, _ip]),
+    "or_compute_uncertainty": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp,
+                                         C.c_double, _dp, _dp, _dp, _ip]),
 }
 for _n, (_r, _a) in _sig.items():
     _f = getattr(_L, _n)
@@ -438,3 +443,32 @@ def cost_batch(grid, disp, cont, flux, var, labels, n_threads=1):
     _chk(_L.or_cost_batch(C.byref(M), disp._h, cont._h, _p(flux, C.c_float), _p(var, C.c_float),
                           _p(lab, C.c_double), len(lab), n_threads, _p(out, C.c_double)), "cost_batch")
     return out
+
+
+def brent_root(fn, lo, hi, tol):
+    """argmin's BrentRoot restated: root of fn on [lo, hi]; returns (root, n_evals)."""
+    def _cb(ctx, x, out):
+        try:
+            out[0] = float(fn(x))
+            return 0
+        except Exception:
+            return -7
+    cb = SCALAR_FN(_cb)
+    r = C.c_double()
+    n = C.c_int()
+    _chk(_L.or_brent_root(cb, None, lo, hi, tol, C.byref(r), C.byref(n)), "brent_root")
+    return r.value, n.value
+
+
+def compute_uncertainty(grid, disp, cont, flux, var, spec_res, label, search_radius=None):
+    """(out[5,2] lower/upper distances, status[5,2]) as compute_uncertainty (fitting.rs:482-568)."""
+    flux, var = _f(flux), _f(var)
+    M = grid.model()
+    out = np.zeros(10)
+    st = np.zeros(10, np.int32)
+    sr = None if search_radius is None else _d(search_radius)
+    _chk(_L.or_compute_uncertainty(C.byref(M), disp._h, cont._h, _p(flux, C.c_float),
+                                   _p(var, C.c_float), spec_res, _p(_d(label), C.c_double),
+                                   None if sr is None else _p(sr, C.c_double), _p(out, C.c_double),
+                                   _p(st, C.c_int)), "compute_uncertainty")
+    return out.reshape(5, 2), st.reshape(5, 2)

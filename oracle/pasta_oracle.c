@@ -1627,3 +1627,141 @@ int or_cost_batch(const or_model *M, const or_disp *d, const or_cont *c, const f
     fit_ctx fc = {M, d, c, flux, var, n_threads};
     return model_cost_fn(&fc, n, labels, chi2);
 }
+
+/* ======================================================================
+ * Uncertainty -- fitting.rs:333-372 (UncertaintyCostFunction), 482-568
+ * (compute_uncertainty); the root finder is argmin 0.10.0's BrentRoot.
+ * ====================================================================== */
+
+int or_brent_root(or_scalar_fn fn, void *ctx, double min, double max, double tol, double *root,
+                  int *n_evals) {
+    const double eps = 2.220446049250313e-16;
+    double a = min, b = max, c = max, d = NAN, e = NAN, fa, fb, fc;
+    int rc, n = 0;
+    if ((rc = fn(ctx, a, &fa))) return rc;
+    n++;
+    if ((rc = fn(ctx, b, &fb))) return rc;
+    n++;
+    if (fa * fb > 0.0) return OR_ERR_SIGN;
+    fc = fb;
+    double best = b, best_cost = fabs(fb);
+    for (;;) {
+        if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) {
+            c = a;
+            fc = fa;
+            d = b - a;
+            e = d;
+        }
+        if (fabs(fc) < fabs(fb)) {
+            a = b; b = c; c = a;
+            fa = fb; fb = fc; fc = fa;
+        }
+        const double tol1 = 2.0 * eps * fabs(b) + 0.5 * tol;
+        const double xm = 0.5 * (c - b);
+        if (fabs(xm) <= tol1 || fb == 0.0) {
+            if (fabs(fb) < best_cost) { best = b; best_cost = fabs(fb); }
+            break;
+        }
+        if (fabs(e) >= tol1 && fabs(fa) > fabs(fb)) {
+            const double s = fb / fa;
+            double p, q;
+            if (a == c) {
+                p = 2.0 * xm * s;
+                q = 1.0 - s;
+            } else {
+                const double qq = fa / fc, r = fb / fc;
+                p = s * (2.0 * xm * qq * (qq - r) - (b - a) * (r - 1.0));
+                q = (qq - 1.0) * (r - 1.0) * (s - 1.0);
+            }
+            if (p > 0.0) q = -q;
+            p = fabs(p);
+            const double min1 = 3.0 * xm * q - fabs(tol1 * q), min2 = fabs(e * q);
+            if (2.0 * p < (min1 < min2 ? min1 : min2)) {
+                e = d;
+                d = p / q;
+            } else {
+                d = xm;
+                e = d;
+            }
+        } else {
+            d = xm;
+            e = d;
+        }
+        a = b;
+        fa = fb;
+        b += fabs(d) > tol1 ? d : (xm > 0.0 ? tol1 : -tol1);
+        if ((rc = fn(ctx, b, &fb))) return rc;
+        n++;
+        if (fabs(fb) < best_cost) { best = b; best_cost = fabs(fb); }
+    }
+    *root = best;
+    if (n_evals) *n_evals = n;
+    return OR_OK;
+}
+
+typedef struct {
+    const or_model *M;
+    const or_disp *d;
+    const or_cont *c;
+    const float *flux, *var;
+    double label[5];
+    int index;
+    double target, radius;
+    float *synth;
+} unc_ctx;
+
+/* UncertaintyCostFunction::cost (fitting.rs:350-371) */
+static int unc_cost(void *vctx, double x, double *f) {
+    unc_ctx *u = (unc_ctx *)vctx;
+    double lab[5];
+    memcpy(lab, u->label, sizeof(lab));
+    lab[u->index] += x * u->radius;
+    double chi2;
+    int rc = or_produce_model(u->M, u->d, lab, u->synth);
+    if (!rc) rc = or_fit_continuum(u->c, u->flux, u->var, u->synth, NULL, &chi2);
+    if (rc) return rc;
+    *f = chi2 - u->target;
+    return OR_OK;
+}
+
+int or_compute_uncertainty(const or_model *M, const or_disp *d, const or_cont *c,
+                           const float *flux, const float *var, double spec_res,
+                           const double label[5], const double *search_radius, double out[10],
+                           int status[10]) {
+    static const double def[5] = {2000.0, 0.3, 0.3, 40.0, 40.0};
+    double sr[5];
+    memcpy(sr, search_radius ? search_radius : def, sizeof(sr));
+    sr[0] *= label[0];
+    sr[3] = fmax(label[3] * sr[3], 100.0);
+    unc_ctx u = {M, d, c, flux, var, {0}, 0, 0.0, 0.0, NULL};
+    memcpy(u.label, label, sizeof(u.label));
+    u.synth = malloc(sizeof(float) * d->n_obs);
+    double best_chi2;
+    int rc = or_produce_model(M, d, label, u.synth);
+    if (!rc) rc = or_fit_continuum(c, flux, var, u.synth, NULL, &best_chi2);
+    if (rc) { free(u.synth); return rc; }
+    const double first = d->wl[0], last = d->wl[d->n_obs - 1];
+    const double n = 4.0 * spec_res * (last - first) / (first + last);
+    u.target = best_chi2 * (1.0 + sqrt(2.0 / n));
+    /* SingleBounds over the grid with vsini (0, 1e4) and RV (-1e3, 1e3) (fitting.rs:519) */
+    or_bounds b = {M->g, 0.0, 1e4, -1e3, 1e3, {0, 0, 0, 0, 0}, {0}, {0}};
+    for (int i = 0; i < 5; i++) {
+        u.index = i;
+        u.radius = sr[i];
+        for (int right = 0; right < 2; right++) {
+            double bl[5], bound, root;
+            memcpy(bl, label, sizeof(bl));
+            bl[i] += right ? sr[i] : -sr[i];
+            int r = or_bounds_clamp(&b, bl, i, &bound);
+            if (!r) {
+                const double resc = (bound - label[i]) / sr[i];
+                r = right ? or_brent_root(unc_cost, &u, 0.0, fmin(resc, 1.0), 1e-3, &root, NULL)
+                          : or_brent_root(unc_cost, &u, resc, 0.0, 1e-3, &root, NULL);
+            }
+            status[2 * i + right] = r;
+            out[2 * i + right] = r ? NAN : (right ? root * sr[i] : -root * sr[i]);
+        }
+    }
+    free(u.synth);
+    return OR_OK;
+}

@@ -31,6 +31,7 @@ extern "C" {
 #define OR_ERR_INDEX -6       /* resample reads past the end of the segment (reference panics) */
 #define OR_ERR_ARG -7         /* bad argument / length mismatch */
 #define OR_ERR_PSO -8         /* particle left bounds (particleswarm.rs:267-275) */
+#define OR_ERR_SIGN -9        /* BrentRoot: f(min) and f(max) have the same sign */
 
 /* ---------------- grid geometry (interpolate.rs:58-670) ---------------- */
 typedef struct or_grid {
@@ -190,6 +191,23 @@ int or_fit(const or_model *M, const or_disp *d, const or_cont *c, const float *f
            const float *var, const or_bounds *b, const or_pso_settings *s, uint64_t seed,
            const double *init_pos, int n_threads, double best[5], double *best_cost,
            float *params, long *n_evals);
+
+/* ---------------- uncertainty (fitting.rs:333-372,482-568) ---------------- */
+/* BrentRoot of argmin 0.10.0 (a dependency absent from the reference tree), restated from
+ * the Brent / zbrent root finder it implements: brackets [min, max], tolerance tol, returns
+ * the iterate with the smallest |f| (argmin's best parameter).  Parity of the iterate
+ * sequence with argmin is unpinned; roots are pinned by closed forms. */
+typedef int (*or_scalar_fn)(void *ctx, double x, double *f);
+int or_brent_root(or_scalar_fn fn, void *ctx, double min, double max, double tol, double *root,
+                  int *n_evals);
+/* compute_uncertainty: out[2*i] = lower, out[2*i+1] = upper distance of label i to the
+ * chi2 = best * (1 + sqrt(2/n)) crossing; status[2*i+k] = 0 or an OR_ERR_* code (the
+ * reference's Err, None in Python).  search_radius NULL = the Python defaults
+ * (2000, 0.3, 0.3, 40, 40) (lib.rs:1554).  Returns an error if the best model fails. */
+int or_compute_uncertainty(const or_model *M, const or_disp *d, const or_cont *c,
+                           const float *flux, const float *var, double spec_res,
+                           const double label[5], const double *search_radius, double out[10],
+                           int status[10]);
 
 #ifdef __cplusplus
 }

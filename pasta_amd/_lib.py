@@ -22,6 +22,8 @@ STATUS_TEXT = {
     6: "resample index past the end of the model",
     7: "particle is not within bounds",
     8: "continuum solve failed",
+    9: "chi2 - target has the same sign at both ends of the search bracket",
+    10: "search bound outside the grid",
 }
 
 
@@ -112,6 +114,8 @@ SIGNATURES = {
     "pasta_vsini_kernel": [_P, C.c_double, _fp, C.c_int, _ip],
     "pasta_fit": [_P, C.POINTER(FitSettings), C.c_uint64, _dp, C.POINTER(FitResult), _dp],
     "pasta_fit_bulk": [_PP, C.c_int, C.POINTER(FitSettings), _u64p, C.POINTER(FitResult), _dp],
+    "pasta_compute_uncertainty": [_P, C.c_double, _dp, _dp, _dp, _ip],
+    "pasta_compute_uncertainty_bulk": [_PP, C.c_int, C.c_double, _dp, _dp, _dp, _ip],
     "pasta_server_create": [_P, C.POINTER(ServerConfig), _PP],
     "pasta_server_destroy": [_P],
     "pasta_server_submit": [_P, _P, C.POINTER(FitSettings), C.c_uint64, _u64p],

@@ -31,6 +31,7 @@ __all__ = [
     "NoConvolutionDispersion", "FixedResolutionDispersion", "VariableResolutionDispersion",
     "ChunkContinuumFitter", "LinearModelContinuumFitter", "FixedContinuum", "ConstantContinuum",
     "FixConstraint", "RangeConstraint", "get_vsini_kernel", "InMemSingleFitter", "FitServer",
+    "LabelUncertainties",
 ]
 
 
@@ -107,6 +108,25 @@ class OptimizationResult:
         lab = Label(*r.label[:5])
         return OptimizationResult(lab, r.continuum_params[: r.n_params], r.chi2, r.iterations,
                                   r.time_s)
+
+
+class LabelUncertainties:
+    """Lower / upper uncertainty per label (src/lib.rs:170-210): (lower, upper), each a
+    float or None where the search found no crossing."""
+    NAMES = ("teff", "m", "logg", "vsini", "rv")
+
+    def __init__(self, out, status):
+        out = np.asarray(out, np.float64).reshape(5, 2)
+        status = np.asarray(status).reshape(5, 2)
+        for i, name in enumerate(self.NAMES):
+            setattr(self, name, tuple(None if status[i, k] else float(out[i, k]) for k in range(2)))
+        self.status = status
+
+    def to_dict(self):
+        return {n: getattr(self, n) for n in self.NAMES}
+
+    def to_json(self):
+        return json.dumps(self.to_dict())
 
 
 class _Constraint:
@@ -577,6 +597,35 @@ class InMemSingleFitter:
             for k, o in enumerate(out):
                 o._trace = tr[k]
         return out
+
+    def compute_uncertainty(self, interpolator, observed_flux, observed_var, spec_res, label,
+                            search_radius=None):
+        """Uncertainties with the chi2 landscape method (src/lib.rs:1545-1568)."""
+        star = self._star(interpolator, observed_flux, observed_var)
+        out = np.zeros(10)
+        st = np.zeros(10, np.int32)
+        sr = None if search_radius is None else f64(search_radius)
+        _raise(lib.pasta_compute_uncertainty(star._h, float(spec_res), ptr(f64(label), C.c_double),
+                                             None if sr is None else ptr(sr, C.c_double),
+                                             ptr(out, C.c_double), ptr(st, C.c_int)),
+               "compute_uncertainty")
+        return LabelUncertainties(out, st)
+
+    def compute_uncertainty_bulk(self, interpolator, observed_fluxes, observed_vars, spec_res,
+                                 labels, search_radius=None):
+        """compute_uncertainty for many spectra in one batched search (src/lib.rs:1570-1600)."""
+        stars = [self._star(interpolator, f, v) for f, v in zip(observed_fluxes, observed_vars)]
+        n = len(stars)
+        lab = f64(labels).reshape(n, 5)
+        out = np.zeros((n, 10))
+        st = np.zeros((n, 10), np.int32)
+        sr = None if search_radius is None else f64(search_radius)
+        arr = (C.c_void_p * n)(*[s._h for s in stars])
+        _raise(lib.pasta_compute_uncertainty_bulk(arr, n, float(spec_res), ptr(lab, C.c_double),
+                                                  None if sr is None else ptr(sr, C.c_double),
+                                                  ptr(out, C.c_double), ptr(st, C.c_int)),
+               "compute_uncertainty_bulk")
+        return [LabelUncertainties(out[k], st[k]) for k in range(n)]
 
     def chi2(self, interpolator, observed_flux, observed_var, labels, allow_nan=None):
         """chi2 at each label vector (src/lib.rs:1603-1627): NaN on failure unless allow_nan=False."""

@@ -1401,6 +1401,190 @@ int pasta_fit_bulk(pasta_star* const* stars, int n_stars, const pasta_fit_settin
 }
 
 // ---------------------------------------------------------------------------
+// compute_uncertainty (src/fitting.rs:333-372,482-568): Brent root searches of
+// chi2(label + x * radius) - target on both sides of every label, all searches of
+// all stars advanced together, one batched chi2 evaluation per round.
+// ---------------------------------------------------------------------------
+
+// argmin 0.10.0's BrentRoot (Brent / zbrent): state machine fed one function value
+// at a time.  best = iterate with the smallest |f| (argmin's best parameter).
+struct BrentSearch {
+    double a = 0, b = 0, c = 0, d = NAN, e = NAN, fa = 0, fb = 0, fc = 0, tol = 1e-3;
+    double best = 0, best_cost = 0, x = 0;
+    int state = 0;  // 0: wants f(a), 1: wants f(b), 2: wants f(x), 3: done, 4: failed
+    int err = 0;
+    void start(double lo, double hi, double t) {
+        a = lo; b = hi; c = hi; d = e = NAN; tol = t; state = 0; x = a;
+    }
+    void fail_with(int code) { state = 4; err = code; }
+    void feed(double f) {
+        if (state == 0) { fa = f; state = 1; x = b; return; }
+        if (state == 1) {
+            fb = f;
+            if (fa * fb > 0.0) { fail_with(PASTA_ST_SIGN); return; }
+            fc = fb;
+            best = b; best_cost = std::fabs(fb);
+            state = 2;
+            plan();
+            return;
+        }
+        fb = f;
+        if (std::fabs(fb) < best_cost) { best = b; best_cost = std::fabs(fb); }
+        plan();
+    }
+    // the part of an iteration before its evaluation: converge, or choose the next b
+    void plan() {
+        const double eps = 2.220446049250313e-16;
+        if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) {
+            c = a; fc = fa; d = b - a; e = d;
+        }
+        if (std::fabs(fc) < std::fabs(fb)) {
+            a = b; b = c; c = a;
+            fa = fb; fb = fc; fc = fa;
+        }
+        const double tol1 = 2.0 * eps * std::fabs(b) + 0.5 * tol;
+        const double xm = 0.5 * (c - b);
+        if (std::fabs(xm) <= tol1 || fb == 0.0) {
+            if (std::fabs(fb) < best_cost) { best = b; best_cost = std::fabs(fb); }
+            state = 3;
+            return;
+        }
+        if (std::fabs(e) >= tol1 && std::fabs(fa) > std::fabs(fb)) {
+            const double sg = fb / fa;
+            double p, q;
+            if (a == c) {
+                p = 2.0 * xm * sg;
+                q = 1.0 - sg;
+            } else {
+                const double qq = fa / fc, r = fb / fc;
+                p = sg * (2.0 * xm * qq * (qq - r) - (b - a) * (r - 1.0));
+                q = (qq - 1.0) * (r - 1.0) * (sg - 1.0);
+            }
+            if (p > 0.0) q = -q;
+            p = std::fabs(p);
+            const double min1 = 3.0 * xm * q - std::fabs(tol1 * q), min2 = std::fabs(e * q);
+            if (2.0 * p < std::min(min1, min2)) { e = d; d = p / q; }
+            else { d = xm; e = d; }
+        } else {
+            d = xm; e = d;
+        }
+        a = b;
+        fa = fb;
+        b += std::fabs(d) > tol1 ? d : (xm > 0.0 ? tol1 : -tol1);
+        x = b;
+    }
+    bool wants() const { return state <= 2; }
+};
+
+static int uncertainty_impl(pasta_star* const* stars, int n, double spec_res, const double* labels,
+                            const double* search_radius, double* out, int* status,
+                            std::vector<int>& bst) {
+    if (!stars || n <= 0 || !labels || !out || !status) return fail(PASTA_ERR_ARG, "bad arguments");
+    if (!(spec_res > 0.0)) return fail(PASTA_ERR_ARG, "spec_res must be positive");
+    pasta_grid* g = stars[0]->grid;
+    for (int k = 0; k < n; k++) {
+        if (!stars[k] || stars[k]->grid != g) return fail(PASTA_ERR_ARG, "stars must share one grid");
+        if (stars[k]->cont_kind < 0) return fail(PASTA_ERR_ARG, "star has no continuum model");
+    }
+    static const double def_radius[5] = {2000.0, 0.3, 0.3, 40.0, 40.0};  // src/lib.rs:1554
+    const double* r0 = search_radius ? search_radius : def_radius;
+    std::lock_guard<std::mutex> lk(g->mu);
+    Batch B;
+    // best chi2 of every star at its label
+    std::vector<pasta_star*> all(stars, stars + n);
+    CHK(batch_build(B, g, all, std::vector<int>(n, 1), 0));
+    std::vector<double> best(n);
+    bst.assign(n, 0);
+    CHK(batch_eval(B, labels, best.data(), bst.data()));
+    // SingleBounds over the grid, vsini (0, 1e4), RV (-1e3, 1e3) (src/fitting.rs:519)
+    Bounds bounds{};
+    bounds.vsini_lo = 0.0; bounds.vsini_hi = 1e4; bounds.rv_lo = -1e3; bounds.rv_hi = 1e3;
+    std::vector<BrentSearch> bs((size_t)n * 10);
+    std::vector<double> target(n), radius((size_t)n * 5);
+    for (int k = 0; k < n; k++) {
+        const double* lab = labels + 5 * (size_t)k;
+        double* sr = &radius[5 * (size_t)k];
+        for (int i = 0; i < 5; i++) sr[i] = r0[i];
+        sr[0] *= lab[0];
+        sr[3] = std::max(lab[3] * sr[3], 100.0);
+        const DevStar& D = stars[k]->dev;
+        const double nn = 4.0 * spec_res * (D.wlN - D.wl0) / (D.wl0 + D.wlN);
+        target[k] = best[k] * (1.0 + std::sqrt(2.0 / nn));
+        for (int i = 0; i < 5; i++)
+            for (int right = 0; right < 2; right++) {
+                BrentSearch& S = bs[10 * (size_t)k + 2 * i + right];
+                if (bst[k]) { S.fail_with(bst[k]); continue; }
+                double bl[5], bound;
+                std::memcpy(bl, lab, sizeof(bl));
+                bl[i] += right ? sr[i] : -sr[i];
+                if (bounds_clamp(g->geom, bounds, bl, i, &bound)) { S.fail_with(PASTA_ST_CLAMP); continue; }
+                const double resc = (bound - lab[i]) / sr[i];
+                if (right) S.start(0.0, std::min(resc, 1.0), 1e-3);
+                else S.start(resc, 0.0, 1e-3);
+            }
+    }
+    // rounds: every search that wants a value gets it from one batched evaluation
+    for (int round = 0; round < 10000; round++) {
+        std::vector<pasta_star*> rs;
+        std::vector<int> cnt;
+        std::vector<double> lab;
+        std::vector<size_t> who;
+        for (int k = 0; k < n; k++) {
+            int c = 0;
+            for (int j = 0; j < 10; j++) {
+                const BrentSearch& S = bs[10 * (size_t)k + j];
+                if (!S.wants()) continue;
+                const int i = j / 2;
+                double l[5];
+                std::memcpy(l, labels + 5 * (size_t)k, sizeof(l));
+                l[i] += S.x * radius[5 * (size_t)k + i];
+                lab.insert(lab.end(), l, l + 5);
+                who.push_back(10 * (size_t)k + j);
+                c++;
+            }
+            if (c) { rs.push_back(stars[k]); cnt.push_back(c); }
+        }
+        if (who.empty()) break;
+        CHK(batch_build(B, g, rs, cnt, 0));
+        std::vector<double> c2(who.size());
+        std::vector<int> st(who.size());
+        CHK(batch_eval(B, lab.data(), c2.data(), st.data()));
+        for (size_t q = 0; q < who.size(); q++) {
+            BrentSearch& S = bs[who[q]];
+            const int k = (int)(who[q] / 10);
+            if (st[q]) S.fail_with(st[q]);
+            else S.feed(c2[q] - target[k]);
+        }
+    }
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < 10; j++) {
+            const BrentSearch& S = bs[10 * (size_t)k + j];
+            const double sr = radius[5 * (size_t)k + j / 2];
+            const bool ok = S.state == 3;
+            status[10 * (size_t)k + j] = ok ? 0 : (S.state == 4 ? S.err : PASTA_ST_SIGN);
+            out[10 * (size_t)k + j] = ok ? ((j & 1) ? S.best * sr : -S.best * sr) : NAN;
+        }
+    return PASTA_OK;
+}
+
+int pasta_compute_uncertainty_bulk(pasta_star* const* stars, int n, double spec_res,
+                                   const double* labels, const double* search_radius,
+                                   double* out, int* status) {
+    std::vector<int> bst;
+    return uncertainty_impl(stars, n, spec_res, labels, search_radius, out, status, bst);
+}
+
+int pasta_compute_uncertainty(pasta_star* s, double spec_res, const double label[5],
+                              const double search_radius[5], double out[10], int status[10]) {
+    if (!s || !label) return fail(PASTA_ERR_ARG, "bad arguments");
+    pasta_star* const arr[1] = {s};
+    std::vector<int> bst;
+    CHK(uncertainty_impl(arr, 1, spec_res, label, search_radius, out, status, bst));
+    if (bst[0]) return fail(PASTA_ERR_MODEL, status_text(bst[0]));  // the reference raises
+    return PASTA_OK;
+}
+
+// ---------------------------------------------------------------------------
 // server: continuous batching of fit requests over the resident grid
 // ---------------------------------------------------------------------------
 #include <deque>

@@ -70,7 +70,7 @@ def test_python_api_surface():
                  "VariableResolutionDispersion", "ChunkContinuumFitter",
                  "LinearModelContinuumFitter", "FixedContinuum", "ConstantContinuum",
                  "FixConstraint", "RangeConstraint", "get_vsini_kernel", "InMemSingleFitter",
-                 "FitServer"]:
+                 "FitServer", "LabelUncertainties"]:
         assert hasattr(pasta, name), name
     s = pasta.PSOSettings(10, 20)
     assert (s.inertia_factor, s.cognitive_factor, s.social_factor, s.delta) == (
