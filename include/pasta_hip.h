@@ -139,6 +139,12 @@ int pasta_star_eval_bytes(const pasta_star *s, int64_t *bytes);
  * parameters.  Returns PASTA_ERR_MODEL if any status is non-zero. */
 int pasta_eval_chi2(pasta_star *s, const double *labels, int P, double *chi2, int *status,
                     float *params);
+/* chi2 of n stars with one label vector each (labels n x 5), in one batched launch:
+ * the evaluation behind chi2_bulk (src/lib.rs:1297-1338), which builds each star from
+ * its own wavelengths with NoConvolution and ChunkFitter(5, 8, 0.2).  chi2[k] is NaN
+ * and status[k] the PASTA_ST_* reason for a failed star; returns PASTA_ERR_MODEL then. */
+int pasta_eval_chi2_multi(pasta_star *const *stars, int n, const double *labels, double *chi2,
+                          int *status);
 /* produce_model / produce_model_bulk (src/interpolate.rs:1068-1090, src/lib.rs:993-1011,
  * 1064-1089): labels P x 5 -> out P x n_obs */
 int pasta_produce_model(pasta_grid *g, const pasta_disp *d, const double *labels, int P,

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpasta_hip.so")
 
 PASTA_OK = 0
+PASTA_ERR_MODEL = -4
 STATUS_TEXT = {
     0: "ok",
     1: "labels out of the grid bounds",
@@ -115,6 +116,7 @@ SIGNATURES = {
     "pasta_fit": [_P, C.POINTER(FitSettings), C.c_uint64, _dp, C.POINTER(FitResult), _dp],
     "pasta_fit_bulk": [_PP, C.c_int, C.POINTER(FitSettings), _u64p, C.POINTER(FitResult), _dp],
     "pasta_compute_uncertainty": [_P, C.c_double, _dp, _dp, _dp, _ip],
+    "pasta_eval_chi2_multi": [_PP, C.c_int, _dp, _dp, _ip],
     "pasta_compute_uncertainty_bulk": [_PP, C.c_int, C.c_double, _dp, _dp, _dp, _ip],
     "pasta_server_create": [_P, C.POINTER(ServerConfig), _PP],
     "pasta_server_destroy": [_P],

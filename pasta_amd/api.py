@@ -464,6 +464,25 @@ class InMemInterpolator:
                                 ptr(out, C.c_float), ptr(st, C.c_int))
         return out, st
 
+    def chi2_bulk(self, observed_wavelength, observed_flux, observed_var, labels, progress=None,
+                  allow_nan=None):
+        """chi2 of many spectra, one label each, with NoConvolution and ChunkFitter(5, 8, 0.2)
+        as the reference hard-codes (src/lib.rs:1297-1338); one batched GPU launch.  A
+        failed model gives NaN with allow_nan=True, else raises."""
+        wlg = self.wavelength
+        stars = [Star(self, NoConvolutionDispersion(wl, wlg), ChunkContinuumFitter(wl, 5, 8, 0.2),
+                      f, v) for wl, f, v in zip(observed_wavelength, observed_flux, observed_var)]
+        n = len(stars)
+        lab = f64(labels).reshape(n, 5)
+        out = np.zeros(n)
+        st = np.zeros(n, np.int32)
+        arr = (C.c_void_p * n)(*[s._h for s in stars])
+        rc = lib.pasta_eval_chi2_multi(arr, n, ptr(lab, C.c_double), ptr(out, C.c_double),
+                                       ptr(st, C.c_int))
+        if rc and not (allow_nan and rc == _lib.PASTA_ERR_MODEL):
+            _raise(rc, "chi2_bulk")
+        return out.tolist()
+
     def get_fitter(self, dispersion, continuum_fitter, settings, vsini_range, rv_range):
         return InMemSingleFitter(self, dispersion, continuum_fitter, settings, vsini_range, rv_range)
 

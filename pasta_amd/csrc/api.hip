@@ -1018,6 +1018,28 @@ int pasta_eval_chi2(pasta_star* s, const double* labels, int P, double* chi2, in
     return PASTA_OK;
 }
 
+int pasta_eval_chi2_multi(pasta_star* const* stars, int n, const double* labels, double* chi2,
+                          int* status) {
+    if (!stars || n <= 0 || !labels || !chi2) return fail(PASTA_ERR_ARG, "bad arguments");
+    pasta_grid* g = stars[0]->grid;
+    for (int k = 0; k < n; k++) {
+        if (!stars[k] || stars[k]->grid != g) return fail(PASTA_ERR_ARG, "stars must share one grid");
+        if (stars[k]->cont_kind < 0) return fail(PASTA_ERR_ARG, "star has no continuum model");
+    }
+    std::lock_guard<std::mutex> lk(g->mu);
+    Batch B;
+    CHK(batch_build(B, g, std::vector<pasta_star*>(stars, stars + n), std::vector<int>(n, 1), 0));
+    std::vector<int> st(n);
+    CHK(batch_eval(B, labels, chi2, st.data()));
+    int bad = 0;
+    for (int k = 0; k < n; k++) {
+        if (status) status[k] = st[k];
+        if (st[k] && !bad) bad = st[k];
+    }
+    if (bad) return fail(PASTA_ERR_MODEL, status_text(bad));
+    return PASTA_OK;
+}
+
 static int model_only(pasta_grid* g, const pasta_disp* d, const double* labels5, int P,
                       float* out, int* status, int mode) {
     if (!g || !labels5 || P <= 0 || !out) return fail(PASTA_ERR_ARG, "bad arguments");

@@ -180,3 +180,31 @@ def test_results_are_deterministic(small, gpu_small):
     c, _, _ = star.chi2(labs[::-1].copy())
     assert np.array_equal(a, b)
     assert np.array_equal(a, c[::-1])
+
+
+def test_chi2_bulk_matches_oracle(small, gpu_small):
+    """chi2_bulk (src/lib.rs:1297-1338): per-spectrum wavelengths, NoConvolution and
+    ChunkFitter(5, 8, 0.2) hard-coded, one label each, in one batched launch."""
+    from pasta_amd import synth
+    pasta, wlg, interp = gpu_small
+    wls = [small.wl, synth.observed_wavelengths(small.spec, 9_000, 450),
+           synth.observed_wavelengths(small.spec, 12_345, 500)]
+    labs = np.array([[6123.0, -0.3, 4.21, 10.0, 20.0], [5600.0, 0.1, 4.5, 80.0, -60.0],
+                     [6800.0, -0.8, 3.7, 150.0, 100.0]])
+    obs = []
+    for k, wl in enumerate(wls):
+        od = O.Dispersion(small.og, O.DISP_NONE, wl)
+        obs.append(synth.observe(O.produce_model(small.og, od, labs[k]), wl, 100.0, 300 + k))
+    got = interp.chi2_bulk(wls, [o[0] for o in obs], [o[1] for o in obs], labs)
+    for k, wl in enumerate(wls):
+        od = O.Dispersion(small.og, O.DISP_NONE, wl)
+        oc = O.Continuum("chunk", len(wl), wl=wl, n_chunks=5, p_order=8, overlap=0.2)
+        ref, _ = O.cost(small.og, od, oc, obs[k][0], obs[k][1], labs[k])
+        assert abs(got[k] - ref) / ref < RTOL, (k, got[k], ref)
+    # a failing label: NaN with allow_nan, else an error
+    bad = labs.copy()
+    bad[1, 0] = 20000.0
+    out = interp.chi2_bulk(wls, [o[0] for o in obs], [o[1] for o in obs], bad, allow_nan=True)
+    assert np.isnan(out[1]) and not np.isnan(out[0])
+    with pytest.raises(Exception):
+        interp.chi2_bulk(wls, [o[0] for o in obs], [o[1] for o in obs], bad)
