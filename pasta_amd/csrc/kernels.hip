@@ -427,6 +427,33 @@ typedef unsigned int u32x4b __attribute__((ext_vector_type(4)));
 // per split part), its B operands the padded taps tpad[32c + 8kk + 16 - n .. +7]
 // (tpad[16 + u] = tr[u]), read as four packed pairs from the copy whose pairs start at
 // the parity of that index: tw[part][p][q] = (tpad[2q + p], tpad[2q + p + 1]).
+__device__ __forceinline__ void load_a3(const unsigned short* p, bf16x8& h, bf16x8& m, bf16x8& l) {
+    h = __builtin_bit_cast(bf16x8, *(const u16x8*)p);
+    m = __builtin_bit_cast(bf16x8, *(const u16x8*)(p + ILOGB));
+    l = __builtin_bit_cast(bf16x8, *(const u16x8*)(p + 2 * ILOGB));
+}
+
+__device__ __forceinline__ void load_b3(const unsigned int* q, bf16x8& h, bf16x8& m, bf16x8& l) {
+    const u32x4b vh = {q[0], q[1], q[2], q[3]};
+    const u32x4b vm = {q[2 * TW_MAX], q[2 * TW_MAX + 1], q[2 * TW_MAX + 2], q[2 * TW_MAX + 3]};
+    const u32x4b vl = {q[4 * TW_MAX], q[4 * TW_MAX + 1], q[4 * TW_MAX + 2], q[4 * TW_MAX + 3]};
+    h = __builtin_bit_cast(bf16x8, vh);
+    m = __builtin_bit_cast(bf16x8, vm);
+    l = __builtin_bit_cast(bf16x8, vl);
+}
+
+// c += the six split products of one 32-step group (smallest terms first)
+__device__ __forceinline__ f4 mfma6(const bf16x8& ah, const bf16x8& am, const bf16x8& al,
+                                    const bf16x8& bh, const bf16x8& bm, const bf16x8& bl, f4 c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+    return c;
+}
+
 __device__ __forceinline__ void conv_rot_bf16(const unsigned short* Ib3, const unsigned int* tw,
                                               int K, int np, int off, float* out, bool mask,
                                               long long x0, int W) {
@@ -444,38 +471,21 @@ __device__ __forceinline__ void conv_rot_bf16(const unsigned short* Ib3, const u
         const unsigned short* ia0 = Ib3 + 16 * a0 + 8 * kk;
         const unsigned short* ia1 = Ib3 + 16 * a1 + 8 * kk;
         f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int c = 0; c < C; c++) {
-            bf16x8 bh, bm, bl;
-            {
-                const unsigned int* q = twp + 16 * c;
-                const u32x4b vh = {q[0], q[1], q[2], q[3]};
-                const u32x4b vm = {q[2 * TW_MAX], q[2 * TW_MAX + 1], q[2 * TW_MAX + 2], q[2 * TW_MAX + 3]};
-                const u32x4b vl = {q[4 * TW_MAX], q[4 * TW_MAX + 1], q[4 * TW_MAX + 2], q[4 * TW_MAX + 3]};
-                bh = __builtin_bit_cast(bf16x8, vh);
-                bm = __builtin_bit_cast(bf16x8, vm);
-                bl = __builtin_bit_cast(bf16x8, vl);
+        if (has1) {
+            for (int c = 0; c < C; c++) {
+                bf16x8 bh, bm, bl, ah, am, al, xh, xm, xl;
+                load_b3(twp + 16 * c, bh, bm, bl);
+                load_a3(ia0 + 32 * c, ah, am, al);
+                load_a3(ia1 + 32 * c, xh, xm, xl);
+                c0 = mfma6(ah, am, al, bh, bm, bl, c0);
+                c1 = mfma6(xh, xm, xl, bh, bm, bl, c1);
             }
-            const unsigned short* p0 = ia0 + 32 * c;
-            const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const u16x8*)p0);
-            const bf16x8 am = __builtin_bit_cast(bf16x8, *(const u16x8*)(p0 + ILOGB));
-            const bf16x8 al = __builtin_bit_cast(bf16x8, *(const u16x8*)(p0 + 2 * ILOGB));
-            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, c0, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c0, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c0, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, c0, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, c0, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c0, 0, 0, 0);
-            if (has1) {
-                const unsigned short* p1 = ia1 + 32 * c;
-                const bf16x8 xh = __builtin_bit_cast(bf16x8, *(const u16x8*)p1);
-                const bf16x8 xm = __builtin_bit_cast(bf16x8, *(const u16x8*)(p1 + ILOGB));
-                const bf16x8 xl = __builtin_bit_cast(bf16x8, *(const u16x8*)(p1 + 2 * ILOGB));
-                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xm, bm, c1, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, bh, c1, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, bl, c1, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xm, bh, c1, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, bm, c1, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, bh, c1, 0, 0, 0);
+        } else {
+            for (int c = 0; c < C; c++) {
+                bf16x8 bh, bm, bl, ah, am, al;
+                load_b3(twp + 16 * c, bh, bm, bl);
+                load_a3(ia0 + 32 * c, ah, am, al);
+                c0 = mfma6(ah, am, al, bh, bm, bl, c0);
             }
         }
 #pragma unroll
