@@ -28,7 +28,8 @@
 namespace pasta {
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
-                       int* sort_hist, int* perm, int n_model,
+                       int* sort_hist, int* perm, int* pair_eq, int2* items, int* n_items,
+                       int n_model,
                        const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
                        const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int mode,
@@ -822,7 +823,8 @@ struct Batch {
     DBuf<DevEval> d_evals;
     DBuf<double> d_labels, d_mpart, d_cpart, d_chi2;
     DBuf<float> d_rot, d_model, d_pf;
-    DBuf<int> d_status, d_hist, d_perm, d_tbound;
+    DBuf<int> d_status, d_hist, d_perm, d_tbound, d_pair_eq, d_nitems;
+    DBuf<int2> d_items;
     int tb_stride = 1;
     DBuf<int4> d_bmom, d_bchi2;
     DBuf<int2> d_solve;
@@ -936,6 +938,9 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
     CHK(B.d_tbound.alloc((size_t)ne * B.tb_stride));
     CHK(B.d_hist.alloc((size_t)g->geom.n_models + 1));
     CHK(B.d_perm.alloc(ne));
+    CHK(B.d_pair_eq.alloc(ne));
+    CHK(B.d_items.alloc(ne));
+    CHK(B.d_nitems.alloc(1));
     CHK(B.d_bmom.upload(bmom));
     CHK(B.d_solve.upload(solve));
     CHK(B.d_bchi2.upload(bchi2));
@@ -946,7 +951,8 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
 static int batch_launch(Batch& B) {
     DevGrid G = B.grid->dev();
     HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p,
-                       B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.n_bm, B.d_bmom.p,
+                       B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.d_pair_eq.p, B.d_items.p,
+                       B.d_nitems.p, B.n_bm, B.d_bmom.p,
                        B.d_bchi2.p, B.cls.data(), (int)B.cls.size(), B.d_solve.p, B.n_solve,
                        B.d_model.p, B.d_mpart.p, B.d_pf.p, B.d_cpart.p, B.d_chi2.p,
                        B.d_status.p, B.mode, B.eager_events ? B.ev[0] : nullptr,

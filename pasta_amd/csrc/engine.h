@@ -96,7 +96,7 @@ struct DevEval {
     int active;      // 0 = skip (finished / failed star in the batched path)
     int key;         // storage row of the first present neighbour: groups evaluations
                      // that gather the same rows (n_models for failed ones)
-    int pad1;
+    int rows_hash;   // hash of idx[]: evaluations with equal rows can share their gathers
     double shift_log;
     long long m_off;  // offset of this evaluation's resampled model in the model buffer
     long long mp_off; // offset of its continuum-moment partials (doubles)

@@ -150,6 +150,9 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
         ev.status = st;
         ev.krot = krot;
         int key = G.geom->n_models;
+        unsigned int hsh = 2166136261u;  // FNV-1a over the neighbour rows: fast reject in pairing
+        for (int j = 0; j < 64; j++) hsh = (hsh ^ (unsigned int)ev.idx[j]) * 16777619u;
+        ev.rows_hash = (int)hsh;
         if (st == ST_OK)
             for (int j = 0; j < 64; j++)
                 if (ev.idx[j] >= 0) { key = ev.idx[j]; break; }
@@ -272,50 +275,6 @@ __device__ __forceinline__ void gather_rows(const DevGrid& G, const DevEval& ev,
 __device__ __forceinline__ int islot(int k) { return k + ((k >> 4) << 2); }
 
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-
-// OUT_F32: dst[k]; OUT_PAD: dst[islot(k)]; OUT_BF3: three bf16 arrays of ILOGB elements
-// (hi, mid, lo) starting at (unsigned short*)dst.
-enum InterpOut { OUT_F32 = 0, OUT_PAD = 1, OUT_BF3 = 2 };
-
-template <int OUT>
-__device__ __forceinline__ void interp_groups(const DevGrid& G, const DevEval& ev, long long g0,
-                                              int ngroups, float* dst) {
-    for (int gg = threadIdx.x; gg < ngroups; gg += NT) {
-        const long long x0 = g0 + 8LL * gg;
-        const bool inside = x0 >= 0 && x0 < G.W;
-        const int voff = inside ? (int)(2 * x0) : OOB_OFFSET;
-        float acc[8];
-#pragma unroll
-        for (int t = 0; t < 8; t++) acc[t] = 0.0f;
-        // all 8 pixels of every lane's group before the n % 128 tail: no quirk handling
-        if (__all(x0 + 7 < G.Wq)) gather_rows<true>(G, ev, voff, x0, acc);
-        else gather_rows<false>(G, ev, voff, x0, acc);
-        if (x0 + 8 > G.W) {
-#pragma unroll
-            for (int t = 0; t < 8; t++)
-                if (x0 + t >= G.W) acc[t] = 0.0f;
-        }
-        if (OUT == OUT_BF3) {
-            u16x8 h, m, l;
-#pragma unroll
-            for (int t = 0; t < 8; t++) {
-                unsigned short a, b, c;
-                split3(acc[t], a, b, c);
-                h[t] = a;
-                m[t] = b;
-                l[t] = c;
-            }
-            unsigned short* d = (unsigned short*)dst + 8 * gg;
-            *(u16x8*)d = h;
-            *(u16x8*)(d + ILOGB) = m;
-            *(u16x8*)(d + 2 * ILOGB) = l;
-        } else {
-            float* d = dst + (OUT == OUT_PAD ? islot(8 * gg) : 8 * gg);
-            *(f4*)d = f4{acc[0], acc[1], acc[2], acc[3]};
-            *(f4*)(d + 4) = f4{acc[4], acc[5], acc[6], acc[7]};
-        }
-    }
-}
 
 // out[q] = sum_u tr[u] * in[q + u] for q in [0, n); 8 outputs per thread with a
 // sliding register window.  tr is zero-padded to a multiple of 8 and `in` has
@@ -505,115 +464,195 @@ __device__ __forceinline__ void conv_rot_bf16(const unsigned short* Ib3, const u
 }
 
 // ---------------------------------------------------------------------------
-// k_model: one workgroup per (evaluation, synthetic tile)
+// k_model: one workgroup per (work item, synthetic tile).  A work item is one
+// evaluation, or two evaluations that gather the same 64 neighbour rows: their
+// interpolation shares every 16-byte gather and u16 conversion (one conversion and two
+// FMAs per pixel-row for the pair), then each runs its own rotation, instrumental
+// broadening and resample.  Each evaluation's arithmetic is the same as alone.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restrict__ stars,
-                                              const DevEval* __restrict__ evals,
-                                              const float* __restrict__ rot,
-                                              const int* __restrict__ perm, int n_evals,
-                                              const int* __restrict__ tbound, int tb_stride,
-                                              float* __restrict__ model, int mode) {
-    // Rotation stage storage, shared by its two forms: f32 (padded tile Ib, reversed taps
-    // hrev: zero, then the taps from index 16, zeros after) or bf16 x 3 (tile arrays Ib3,
-    // split tap pairs tw).  Ib also holds the instrumental-broadening output.
-    static_assert(3 * ILOGB * 2 + 6 * TW_MAX * 4 <= (ILDS + KROT_MAX + 48) * 4, "LDS union");
-    __shared__ __align__(16) float lds_rot[ILDS + KROT_MAX + 48];
-    __shared__ float Rb[RLDS];
-    __shared__ float grev[2 * HD_MAX + 1 + 8];
-    float* const Ib = lds_rot;
-    float* const hrev = lds_rot + ILDS;
-    unsigned short* const Ib3 = (unsigned short*)lds_rot;
-    unsigned int* const tw = (unsigned int*)(lds_rot + (3 * ILOGB) / 2);
+struct TileJob {
+    const DevEval* ev;
+    const DevStar* S;
+    int e;           // evaluation
+    int ilo, ihi;    // observed pixels of the tile (mode 0)
+    long long clo;   // first synthetic pixel of the convolved range C
+    long long rlo;   // first pixel of the rotation output R (clo - hd)
+    int nC, hd, K, nR;
+    bool mask;
+    int kind;        // 0 no rotation (tile into Rb), 1 bf16 rotation, 2 f32 rotation
+    long long g0;    // first interpolated 8-pixel group (multiple of 8)
+    int off, ng;     // logical index of rlo - K/2 (or rlo); groups interpolated
+};
 
-    // tile-major over evaluations sorted by neighbour rows: the workgroups resident on
-    // one XCD work on one tile of evaluations that gather overlapping rows
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = b / n_evals;
-    const int2 bt = make_int2(perm[b - tile * n_evals], tile);
-    const DevEval& ev = evals[bt.x];
-    if (ev.status != ST_OK) return;
+__device__ __forceinline__ bool tile_job(const DevGrid& G, const DevStar* stars, const DevEval* evals,
+                                         int e, int tile, const int* tbound, int tb_stride, int mode,
+                                         TileJob& J) {
+    const DevEval& ev = evals[e];
+    if (ev.status != ST_OK) return false;
     const DevStar& S = stars[ev.star];
-    if (tile >= S.n_tiles) return;
-    const int tid = threadIdx.x;
+    if (tile >= S.n_tiles) return false;
+    J.ev = &ev;
+    J.S = &S;
+    J.e = e;
     const double sl = ev.shift_log;
-    const long long t0 = (long long)(ev.tile_base + bt.y) * TILE;
-    int ilo = 0, ihi = 0;
-    long long clo, chi;
+    const long long t0 = (long long)(ev.tile_base + tile) * TILE;
+    long long chi;
     if (mode == 0) {
-        const int* tb = tbound + (size_t)bt.x * tb_stride + bt.y;
-        ilo = __builtin_amdgcn_readfirstlane(tb[0]);
-        ihi = __builtin_amdgcn_readfirstlane(tb[1]);
-        if (ilo >= ihi) return;
-        clo = jfloor(S.fidx[ilo] + sl);
-        chi = jfloor(S.fidx[ihi - 1] + sl) + 1;
+        const int* tb = tbound + (size_t)e * tb_stride + tile;
+        J.ilo = __builtin_amdgcn_readfirstlane(tb[0]);
+        J.ihi = __builtin_amdgcn_readfirstlane(tb[1]);
+        if (J.ilo >= J.ihi) return false;
+        J.clo = jfloor(S.fidx[J.ilo] + sl);
+        chi = jfloor(S.fidx[J.ihi - 1] + sl) + 1;
     } else {
-        if (t0 >= G.W) return;
-        clo = t0;
+        if (t0 >= G.W) return false;
+        J.ilo = J.ihi = 0;
+        J.clo = t0;
         chi = (t0 + TILE < G.W ? t0 + TILE : G.W) - 1;
     }
-    const int nC = (int)(chi - clo + 1);
-    const int hd = (S.disp_kind == DISP_NONE) ? 0 : (S.Kd - 1) / 2;
-    const int K = ev.krot;
-    const int hr = K > 0 ? K / 2 : 0;
-    const long long rlo = clo - hd;
-    const int nR = nC + 2 * hd;
-    const int nI = nR + 2 * hr;
-    const bool mask = S.disp_kind != DISP_NONE;
+    J.nC = (int)(chi - J.clo + 1);
+    J.hd = (S.disp_kind == DISP_NONE) ? 0 : (S.Kd - 1) / 2;
+    J.K = ev.krot;
+    const int hr = J.K > 0 ? J.K / 2 : 0;
+    J.rlo = J.clo - J.hd;
+    J.nR = J.nC + 2 * J.hd;
+    J.mask = S.disp_kind != DISP_NONE;
+    J.kind = J.K == 0 ? 0 : (J.K <= KB_MAX ? 1 : 2);
+    const long long first = J.rlo - hr;
+    J.g0 = first & ~7LL;
+    J.off = (int)(first - J.g0);
+    J.ng = (J.off + J.nR + 2 * hr + 7) >> 3;
+    return true;
+}
 
-    const bool bf3 = K > 0 && K <= KB_MAX;
-    if (bf3) {  // split taps prepared per evaluation by k_prep
-        const u32x4* src = (const u32x4*)(rot + (size_t)bt.x * ROT_STRIDE + KROT_MAX);
-        for (int i = tid; i < 6 * TW_MAX / 4; i += NT) ((u32x4*)tw)[i] = src[i];
-    } else if (K > 0) {
-        const float* h = rot + (size_t)bt.x * ROT_STRIDE;
+// store one interpolated 8-pixel group (local group index gl) in the job's tile layout
+__device__ __forceinline__ void store_group(const TileJob& J, int gl, const float (&acc)[8],
+                                            float* Ib, float* Rb) {
+    if (J.kind == 1) {
+        u16x8 h, m, l;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            unsigned short a, b, c;
+            split3(acc[t], a, b, c);
+            h[t] = a;
+            m[t] = b;
+            l[t] = c;
+        }
+        unsigned short* d = (unsigned short*)Ib + 8 * gl;
+        *(u16x8*)d = h;
+        *(u16x8*)(d + ILOGB) = m;
+        *(u16x8*)(d + 2 * ILOGB) = l;
+    } else {
+        float* d = J.kind == 2 ? Ib + islot(8 * gl) : Rb + 8 * gl;
+        *(f4*)d = f4{acc[0], acc[1], acc[2], acc[3]};
+        *(f4*)(d + 4) = f4{acc[4], acc[5], acc[6], acc[7]};
+    }
+}
+
+// interpolated values of the group at pixel x0 for one evaluation
+__device__ __forceinline__ void gather_one(const DevGrid& G, const DevEval& ev, long long x0,
+                                           float (&acc)[8]) {
+    const bool inside = x0 >= 0 && x0 < G.W;
+    const int voff = inside ? (int)(2 * x0) : OOB_OFFSET;
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc[t] = 0.0f;
+    // all 8 pixels of every lane's group before the n % 128 tail: no quirk handling
+    if (__all(x0 + 7 < G.Wq)) gather_rows<true>(G, ev, voff, x0, acc);
+    else gather_rows<false>(G, ev, voff, x0, acc);
+    if (x0 + 8 > G.W) {
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            if (x0 + t >= G.W) acc[t] = 0.0f;
+    }
+}
+
+// two evaluations with the same rows: one gather and conversion, two weight sets
+template <bool FAST>
+__device__ __forceinline__ void gather_rows2(const DevGrid& G, const DevEval& ea, const DevEval& eb,
+                                             int voff, long long x0, float (&aa)[8], float (&ab)[8]) {
+    const int nrec = (int)(G.pitch * 2);
+#pragma unroll 1
+    for (int j0 = 0; j0 < 64; j0 += 8) {
+        u32x4 v[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; jj++) {
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)ea.rowp[j0 + jj], 0, nrec, 0x00020000);
+            v[jj] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
+        }
+#pragma unroll
+        for (int jj = 0; jj < 8; jj++) {
+            const float fa = __builtin_bit_cast(
+                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, ea.fw[j0 + jj])));
+            const float fb = __builtin_bit_cast(
+                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, eb.fw[j0 + jj])));
+            float u[8];
+            unpack8(v[jj], u);
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const float q = (FAST || x0 + t < G.Wq) ? u[t] : u[t] / 65535.0f;  // tail quirk :978
+                aa[t] = fmaf(q, fa, aa[t]);
+                ab[t] = fmaf(q, fb, ab[t]);
+            }
+        }
+    }
+}
+
+// taps of a job into LDS (and the zero padding the MFMA steps read past its tile); issued
+// before the interpolation so that their latency overlaps it
+__device__ __forceinline__ void load_taps(const TileJob& J, const float* rot, float* Ib,
+                                          float* hrev, unsigned int* tw, float* grev,
+                                          bool with_tw = true) {
+    const int tid = threadIdx.x;
+    const DevStar& S = *J.S;
+    if (J.kind == 1) {  // split taps prepared per evaluation by k_prep
+        if (with_tw) {
+            const u32x4* src = (const u32x4*)(rot + (size_t)J.e * ROT_STRIDE + KROT_MAX);
+            for (int i = tid; i < 6 * TW_MAX / 4; i += NT) ((u32x4*)tw)[i] = src[i];
+        }
+        unsigned short* Ib3 = (unsigned short*)Ib;
+        if (tid < 64) {  // the MFMA groups read past the tile
+            Ib3[8 * J.ng + tid] = 0;
+            Ib3[ILOGB + 8 * J.ng + tid] = 0;
+            Ib3[2 * ILOGB + 8 * J.ng + tid] = 0;
+        }
+    } else if (J.kind == 2) {
+        const float* h = rot + (size_t)J.e * ROT_STRIDE;
+        const int K = J.K;
         const int Kp = 16 * ((K + 30) >> 4) + 16;  // every index the MFMA steps read
         for (int i = tid; i < Kp; i += NT) {
             const int u = i - 16;
             hrev[i] = (u >= 0 && u < K) ? h[K - 1 - u] : 0.0f;
         }
+        if (tid < 48) Ib[islot(8 * J.ng + tid)] = 0.0f;  // the MFMA steps read past the tile
     }
     if (S.disp_kind == DISP_FIXED) {
         const int Kp = (S.Kd + 7) & ~7;
         for (int i = tid; i < Kp; i += NT) grev[i] = i < S.Kd ? S.dkern[S.Kd - 1 - i] : 0.0f;
     }
+}
 
-    // stage 1: interpolation.  Groups start at the multiple of 8 at or below the first
-    // pixel; `off` is the first pixel's logical index.  With rotation the tile goes to
-    // Ib (padded slots), else straight into Rb, read from R = Rb + off.
+// rotation, instrumental broadening and resample of a job whose interpolated tile and
+// taps are in LDS (stored but not yet synchronised)
+__device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, const float* rot,
+                                           float* model, int mode, float* Ib, float* Rb,
+                                           float* hrev, unsigned int* tw, float* grev) {
+    const int tid = threadIdx.x;
+    const DevStar& S = *J.S;
+    __syncthreads();
+    // rotational broadening, outputs p = off + q for R index q
     float* R = Rb;
-    if (bf3) {
-        const long long g0 = (rlo - hr) & ~7LL;
-        const int off = (int)(rlo - hr - g0);
-        const int ng = (off + nI + 7) >> 3;
-        interp_groups<OUT_BF3>(G, ev, g0, ng, Ib);
-        if (tid < 64) {  // the MFMA groups read past the tile
-            Ib3[8 * ng + tid] = 0;
-            Ib3[ILOGB + 8 * ng + tid] = 0;
-            Ib3[2 * ILOGB + 8 * ng + tid] = 0;
-        }
-        __syncthreads();
-        // stage 2: rotational broadening, outputs p = off + q for R index q
-        conv_rot_bf16(Ib3, tw, K, off + nR, off, Rb, mask, rlo, G.W);
-    } else if (K > 0) {
-        const long long g0 = (rlo - hr) & ~7LL;
-        const int off = (int)(rlo - hr - g0);
-        const int ng = (off + nI + 7) >> 3;
-        interp_groups<OUT_PAD>(G, ev, g0, ng, Ib);
-        if (tid < 48) Ib[islot(8 * ng + tid)] = 0.0f;  // the MFMA steps read past the tile
-        __syncthreads();
-        // stage 2: rotational broadening, outputs p = off + q for R index q
-        conv_rot_mfma(Ib, hrev, K, off + nR, off, Rb, mask, rlo, G.W);
-    } else {
-        const long long g0 = rlo & ~7LL;
-        const int off = (int)(rlo - g0);
-        const int ng = (off + nR + 7) >> 3;
-        interp_groups<OUT_F32>(G, ev, g0, ng, Rb);
-        R = Rb + off;
-    }
-    if (tid < 16) R[nR + tid] = 0.0f;
+    if (J.kind == 1)
+        conv_rot_bf16((const unsigned short*)Ib, tw, J.K, J.off + J.nR, J.off, Rb, J.mask, J.rlo, G.W);
+    else if (J.kind == 2)
+        conv_rot_mfma(Ib, hrev, J.K, J.off + J.nR, J.off, Rb, J.mask, J.rlo, G.W);
+    else
+        R = Rb + J.off;
+    if (tid < 16) R[J.nR + tid] = 0.0f;
     __syncthreads();
 
-    // stage 3: instrumental broadening -> C (reuses Ib)
+    // instrumental broadening -> C (reuses Ib)
+    const int nC = J.nC, hd = J.hd;
     const float* Cb = R;
     if (S.disp_kind == DISP_FIXED) {
         conv_tile(R, grev, S.Kd, nC, Ib, false, 0, 0);
@@ -622,7 +661,7 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     } else if (S.disp_kind == DISP_VARIABLE) {
         const int Kd = S.Kd;
         for (int q = tid; q < nC; q += NT) {
-            const long long x = clo + q;
+            const long long x = J.clo + q;
             float c;
             if (x < hd || x >= G.W - hd) {
                 c = R[q + hd];
@@ -638,18 +677,141 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
         __syncthreads();
     }
 
-    // stage 4: RV shift + resample (f32-quantised weight, convolve_rv.rs:569-573)
-    float* out = model + ev.m_off;
+    // RV shift + resample (f32-quantised weight, convolve_rv.rs:569-573)
+    float* out = model + J.ev->m_off;
     if (mode == 1) {
-        for (int q = tid; q < nC; q += NT) out[clo + q] = Cb[q];
+        for (int q = tid; q < nC; q += NT) out[J.clo + q] = Cb[q];
         return;
     }
-    for (int i = ilo + tid; i < ihi; i += NT) {
+    const double sl = J.ev->shift_log;
+    for (int i = J.ilo + tid; i < J.ihi; i += NT) {
         const double idx = S.fidx[i] + sl;
         const long long j = jfloor(idx);
         const float w = (float)idx - (float)j;
-        const float a = Cb[j - clo], c = Cb[j + 1 - clo];
+        const float a = Cb[j - J.clo], c = Cb[j + 1 - J.clo];
         out[i] = (1.0f - w) * a + w * c;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restrict__ stars,
+                                              const DevEval* __restrict__ evals,
+                                              const float* __restrict__ rot,
+                                              const int2* __restrict__ items,
+                                              const int* __restrict__ n_items, int n_evals,
+                                              const int* __restrict__ tbound, int tb_stride,
+                                              float* __restrict__ model, int mode) {
+    // Rotation stage storage, shared by its two forms: f32 (padded tile Ib, reversed taps
+    // hrev: zero, then the taps from index 16, zeros after) or bf16 x 3 (tile arrays,
+    // split tap pairs tw).  Ib also holds the instrumental-broadening output.
+    static_assert(3 * ILOGB * 2 + 6 * TW_MAX * 4 <= (ILDS + KROT_MAX + 48) * 4, "LDS union");
+    __shared__ __align__(16) float lds_rot[ILDS + KROT_MAX + 48];
+    __shared__ float Rb[RLDS];
+    __shared__ float grev[2 * HD_MAX + 1 + 8];
+    float* const Ib = lds_rot;
+    float* const hrev = lds_rot + ILDS;
+    unsigned int* const tw = (unsigned int*)(lds_rot + (3 * ILOGB) / 2);
+
+    // tile-major over the work items (in neighbour-row order): the workgroups resident on
+    // one XCD work on one tile of evaluations that gather overlapping rows
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = b / n_evals;
+    const int k = b - tile * n_evals;
+    if (k >= __builtin_amdgcn_readfirstlane(*n_items)) return;
+    const int2 it = items[k];
+    TileJob A;
+    const bool hasA = tile_job(G, stars, evals, it.x, tile, tbound, tb_stride, mode, A);
+    bool hasB = false;
+    long long bg0 = 0;
+    int bng = 0;
+    const DevEval* bev = nullptr;
+    if (it.y >= 0) {
+        TileJob T;
+        hasB = tile_job(G, stars, evals, it.y, tile, tbound, tb_stride, mode, T);
+        bg0 = T.g0;
+        bng = T.ng;
+        bev = T.ev;
+    }
+    const long long pg0 = min(A.g0, bg0);
+    const int png = (int)((max(A.g0 + 8LL * A.ng, bg0 + 8LL * bng) - pg0) >> 3);
+    if (hasA) load_taps(A, rot, Ib, hrev, tw, grev);
+    if (hasA && hasB && png <= 2 * NT) {
+        // shared interpolation: A's groups go to LDS at once, B's wait in registers, and
+        // so do B's split taps (read now, stored after A's rotation)
+        const bool btw = bev->krot > 0 && bev->krot <= KB_MAX;
+        u32x4 tkeep[3];
+        {
+            const u32x4* src = (const u32x4*)(rot + (size_t)it.y * ROT_STRIDE + KROT_MAX);
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const int i = threadIdx.x + r * NT;
+                tkeep[r] = (btw && i < 6 * TW_MAX / 4) ? src[i] : u32x4{0u, 0u, 0u, 0u};
+            }
+        }
+        float keep[2][8];
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int gg = threadIdx.x + r * NT;
+            float aa[8], ab[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) { aa[t] = 0.0f; ab[t] = 0.0f; }
+            if (gg < png) {
+                const long long x0 = pg0 + 8LL * gg;
+                const bool inside = x0 >= 0 && x0 < G.W;
+                const int voff = inside ? (int)(2 * x0) : OOB_OFFSET;
+                if (__all(x0 + 7 < G.Wq)) gather_rows2<true>(G, *A.ev, *bev, voff, x0, aa, ab);
+                else gather_rows2<false>(G, *A.ev, *bev, voff, x0, aa, ab);
+                if (x0 + 8 > G.W) {
+#pragma unroll
+                    for (int t = 0; t < 8; t++)
+                        if (x0 + t >= G.W) { aa[t] = 0.0f; ab[t] = 0.0f; }
+                }
+                const long long ga = (x0 - A.g0) >> 3;
+                if (ga >= 0 && ga < A.ng) store_group(A, (int)ga, aa, Ib, Rb);
+            }
+#pragma unroll
+            for (int t = 0; t < 8; t++) keep[r][t] = ab[t];
+        }
+        finish_job(G, A, rot, model, mode, Ib, Rb, hrev, tw, grev);
+        __syncthreads();
+        TileJob B;
+        tile_job(G, stars, evals, it.y, tile, tbound, tb_stride, mode, B);
+        if (btw) {
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const int i = threadIdx.x + r * NT;
+                if (i < 6 * TW_MAX / 4) ((u32x4*)tw)[i] = tkeep[r];
+            }
+        }
+        load_taps(B, rot, Ib, hrev, tw, grev, false);
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int gg = threadIdx.x + r * NT;
+            const long long gb = (pg0 + 8LL * gg - B.g0) >> 3;
+            if (gg < png && gb >= 0 && gb < B.ng) store_group(B, (int)gb, keep[r], Ib, Rb);
+        }
+        finish_job(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
+        return;
+    }
+    // one evaluation at a time
+    if (hasA) {
+        for (int gg = threadIdx.x; gg < A.ng; gg += NT) {
+            float acc[8];
+            gather_one(G, *A.ev, A.g0 + 8LL * gg, acc);
+            store_group(A, gg, acc, Ib, Rb);
+        }
+        finish_job(G, A, rot, model, mode, Ib, Rb, hrev, tw, grev);
+    }
+    if (hasB) {
+        if (hasA) __syncthreads();
+        TileJob B;
+        tile_job(G, stars, evals, it.y, tile, tbound, tb_stride, mode, B);
+        load_taps(B, rot, Ib, hrev, tw, grev);
+        for (int gg = threadIdx.x; gg < B.ng; gg += NT) {
+            float acc[8];
+            gather_one(G, *B.ev, B.g0 + 8LL * gg, acc);
+            store_group(B, gg, acc, Ib, Rb);
+        }
+        finish_job(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
     }
 }
 
@@ -1191,9 +1353,95 @@ __global__ void k_sort_scatter(const DevEval* __restrict__ evals, int n, int* __
 // ---------------------------------------------------------------------------
 // host-side launchers (called from the driver in api.hip)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// pairing: adjacent evaluations in neighbour-row order that gather the same 64 rows
+// for the same synthetic tiles become one work item of k_model
+// ---------------------------------------------------------------------------
+__global__ void k_pair_eq(const DevEval* __restrict__ evals, const int* __restrict__ perm, int n,
+                          int* __restrict__ eq) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int r = 0;
+    if (i + 1 < n) {
+        const DevEval& a = evals[perm[i]];
+        const DevEval& b = evals[perm[i + 1]];
+        if (a.status == ST_OK && b.status == ST_OK && a.tile_base == b.tile_base &&
+            a.key == b.key && a.rows_hash == b.rows_hash) {
+            r = 1;
+            for (int j = 0; j < 64; j++)
+                if (a.idx[j] != b.idx[j]) { r = 0; break; }
+        }
+    }
+    eq[i] = r;
+}
+
+// inclusive block scans over 1024 threads (op: max or sum)
+template <bool MAX>
+__device__ __forceinline__ int block_scan(int v, int* wsum /* 16 */) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v = MAX ? max(v, u) : v + u;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        int x = lane < 16 ? wsum[lane] : (MAX ? -1 : 0);
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int u = __shfl_up(x, o, 64);
+            if (lane >= o) x = MAX ? max(x, u) : x + u;
+        }
+        if (lane < 16) wsum[lane] = x;
+    }
+    __syncthreads();
+    const int prev = w > 0 ? wsum[w - 1] : (MAX ? -1 : 0);
+    const int r = MAX ? max(v, prev) : v + prev;
+    __syncthreads();
+    return r;
+}
+
+// Greedy pairing along each run of equal neighbour rows: positions 0-1, 2-3, ... of a
+// run share; items[k] = (first evaluation, second or -1), in neighbour-row order.
+__global__ __launch_bounds__(1024) void k_pair_items(const int* __restrict__ perm,
+                                                     const int* __restrict__ eq, int n,
+                                                     int2* __restrict__ items,
+                                                     int* __restrict__ n_items) {
+    __shared__ int wsum[16];
+    __shared__ int s_first[1024];
+    __shared__ int s_carry[3];  // run start of the chunk's last element, its pairing flag, items
+    const int tid = threadIdx.x;
+    if (tid == 0) { s_carry[0] = 0; s_carry[1] = 0; s_carry[2] = 0; }
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + tid;
+        const bool valid = i < n;
+        const int mark = (valid && (i == 0 || !eq[i - 1])) ? i : -1;
+        const int start = max(block_scan<true>(mark, wsum), s_carry[0]);
+        const bool first = valid && eq[i] && (((i - start) & 1) == 0);
+        s_first[tid] = first;
+        __syncthreads();
+        const bool second = tid > 0 ? s_first[tid - 1] != 0 : s_carry[1] != 0;
+        const int emit = (valid && !second) ? 1 : 0;
+        const int incl = block_scan<false>(emit, wsum);
+        const int k = s_carry[2] + incl - emit;
+        if (emit) items[k] = make_int2(perm[i], first ? perm[i + 1] : -1);
+        __syncthreads();
+        if (tid == 1023) {
+            s_carry[0] = start;
+            s_carry[1] = first;
+            s_carry[2] = k + emit;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *n_items = s_carry[2];
+}
+
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
-                       int* sort_hist, int* perm, int n_model,
+                       int* sort_hist, int* perm, int* pair_eq, int2* items, int* n_items,
+                       int n_model,
                        const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
                        const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int mode,
@@ -1208,10 +1456,14 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, sort_hist, nb);
     hipLaunchKernelGGL(k_sort_scatter, dim3((n_evals + 255) / 256), dim3(256), 0, st, evals,
                        n_evals, sort_hist, perm);
+    hipLaunchKernelGGL(k_pair_eq, dim3((n_evals + 255) / 256), dim3(256), 0, st, evals, perm,
+                       n_evals, pair_eq);
+    hipLaunchKernelGGL(k_pair_items, dim3(1), dim3(1024), 0, st, perm, pair_eq, n_evals, items,
+                       n_items);
     if (ev_model0) (void)hipEventRecord(ev_model0, st);
     if (n_model > 0)
-        hipLaunchKernelGGL(k_model, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot, perm,
-                           n_evals, tbound, tb_stride, model, mode);
+        hipLaunchKernelGGL(k_model, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot, items,
+                           n_items, n_evals, tbound, tb_stride, model, mode);
     if (ev_model1) (void)hipEventRecord(ev_model1, st);
     for (int c = 0; c < n_cls; c++) {
         const ContClass& K = cls[c];
