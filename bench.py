@@ -242,7 +242,7 @@ def cpu_baseline(interp, spec, plain, budget_s):
     log(f"[bench] oracle set-up {time.perf_counter() - t0:.1f} s")
     threads = min(len(os.sched_getaffinity(0)), 16)
     rng = np.random.default_rng(77)
-    pool = 8192
+    pool = 16384
     labels = np.stack([rng.uniform(4200, 9800, pool), rng.uniform(-0.95, 0.45, pool),
                        rng.uniform(3.1, 4.9, pool), rng.uniform(0, 300, pool),
                        rng.uniform(-150, 150, pool)], axis=1)
