@@ -739,7 +739,11 @@ static int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, c
         D.n_tiles = (g->W + TILE - 1) / TILE;
     }
     if (flux) CHK(dev_copy(s.get(), flux, n, &D.flux));
-    if (var) CHK(dev_copy(s.get(), var, n, &D.var));
+    if (var) {  // chi2 terms are r * r * (1 / var): one multiply per evaluation and pixel
+        std::vector<float> iv(n);
+        for (int i = 0; i < n; i++) iv[i] = 1.0f / var[i];
+        CHK(dev_copy(s.get(), iv.data(), n, &D.ivar));
+    }
     if (c) {
         D.cont_kind = c->kind;
         D.n_chunks = c->n_chunks;

@@ -71,7 +71,7 @@ struct DevStar {
     int n_mblocks, n_cblocks, n_tiles, pad0;
     double chi2_div, wl0, wlN;
     const float* flux;
-    const float* var;
+    const float* ivar;   // 1 / variance (f32), the chi2 weight
     const double* fidx;       // float index of each observed pixel in the synthetic grid
     const double* wl;         // observed wavelengths
     const float* dkern;       // fixed: Kd taps; variable: W rows of Kd taps

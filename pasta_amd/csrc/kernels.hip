@@ -970,7 +970,7 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
 #pragma unroll
                 for (int e = 0; e < E; e++) {
                     if (!ok[e]) continue;
-                    const double y = (double)(fl[u] / mv[u][e]);
+                    const double y = (double)(fl[u] * __builtin_amdgcn_rcpf(mv[u][e]));
 #pragma unroll
                     for (int k = 0; k < PM; k++)
                         if (k < np) acc[e * PM + k] = fma(ad[k], y, acc[e * PM + k]);
@@ -987,7 +987,7 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 if (!ok[e]) continue;
-                const double y = (double)(f / m[e][i]);
+                const double y = (double)(f * __builtin_amdgcn_rcpf(m[e][i]));
 #pragma unroll
                 for (int k = 0; k < PM; k++)
                     if (k < np) acc[e * PM + k] = fma(ad[k], y, acc[e * PM + k]);
@@ -998,7 +998,7 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
             const float f = S.flux[i];
 #pragma unroll
             for (int e = 0; e < E; e++)
-                if (ok[e]) acc[e * PM] += (double)(f / m[e][i]);
+                if (ok[e]) acc[e * PM] += (double)(f * __builtin_amdgcn_rcpf(m[e][i]));
         }
     }
     const double tot = group_sums<E * PM>(acc, red);
@@ -1110,14 +1110,14 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
             // E * 2 * PM registers
             __asm__ volatile("" ::: "memory");
             double wl[U];
-            float fl[U], vr[U], fc[U], mv[U][E];
+            float fl[U], vr[U], fc[U], mv[U][E];  // vr: 1 / variance
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int i = i0 + u * NT;
                 const bool in = i < cb.w;
                 wl[u] = in ? S.wl[i] : lo;
                 fl[u] = in ? S.flux[i] : 0.0f;
-                vr[u] = in ? S.var[i] : 1.0f;
+                vr[u] = in ? S.ivar[i] : 1.0f;
                 fc[u] = (in && blend) ? S.fac[i] : 0.0f;
 #pragma unroll
                 for (int e = 0; e < E; e++) mv[u][e] = (in && ok[e]) ? m[e][i] : 0.0f;
@@ -1137,7 +1137,7 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
                         p = prev * fc[u] + p * (1.0f - fc[u]);
                     }
                     const float r = fl[u] - p * mv[u][e];
-                    acc[e] += (double)((r * r) / vr[u]);
+                    acc[e] += (double)((r * r) * vr[u]);
                 }
             }
         }
@@ -1147,7 +1147,7 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
             float d[PM];
 #pragma unroll
             for (int k = 0; k < PM; k++) d[k] = k < p1 ? S.design[(size_t)k * S.N + i] : 0.0f;
-            const float fl = S.flux[i], var = S.var[i];
+            const float fl = S.flux[i], iv = S.ivar[i];
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 if (!ok[e]) continue;
@@ -1161,27 +1161,27 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
                     }
                 }
                 const float r = y - bq;
-                acc[e] += (double)((r / var) * r);
+                acc[e] += (double)((r * iv) * r);
             }
         }
     } else if (S.cont_kind == CONT_FIXED) {
         for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
-            const float fl = S.flux[i], var = S.var[i], fc = S.fixedc[i];
+            const float fl = S.flux[i], iv = S.ivar[i], fc = S.fixedc[i];
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 if (!ok[e]) continue;
                 const float r = fl - m[e][i] * fc;
-                acc[e] += (double)((r / var) * r);
+                acc[e] += (double)((r * iv) * r);
             }
         }
     } else {
         for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
-            const float fl = S.flux[i], var = S.var[i];
+            const float fl = S.flux[i], iv = S.ivar[i];
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 if (!ok[e]) continue;
                 const float r = fl - m[e][i] * spf[e][0];
-                acc[e] += (double)((r / var) * r);
+                acc[e] += (double)((r * iv) * r);
             }
         }
     }
