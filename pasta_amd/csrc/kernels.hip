@@ -968,8 +968,7 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
 #pragma unroll
                 for (int k = 0; k < PM; k++) ad[k] = (double)a[k];
 #pragma unroll
-                for (int e = 0; e < E; e++) {
-                    if (!ok[e]) continue;
+                for (int e = 0; e < E; e++) {  // straight-line: an absent evaluation's sums are dropped
                     const double y = (double)(fl[u] * __builtin_amdgcn_rcpf(mv[u][e]));
 #pragma unroll
                     for (int k = 0; k < PM; k++)
