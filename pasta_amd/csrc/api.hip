@@ -145,7 +145,7 @@ struct pasta_cont {
     int n_chunks = 0, p1 = 1, ignore = 0, n_params = 1;
     std::vector<int> startstop;
     std::vector<double> lohi, chol, wl;
-    std::vector<float> fac, design_cm, design_rm, fixedc;
+    std::vector<float> fac, design_q, design_rm, fixedc;
     std::vector<int4> mblocks, cblocks;
     std::vector<int> mb_off;
     double chi2_div = 1.0;
@@ -587,9 +587,14 @@ int pasta_cont_create_linear(const float* design, int n, int ncols, pasta_cont**
     c->p1 = ncols;
     c->n_params = ncols;
     c->design_rm.assign(design, design + (size_t)n * ncols);
-    c->design_cm.resize((size_t)n * ncols);
+    // device layout: PT = 16 or 32 zero-padded columns of n4 = n rounded up to 4 values
+    // (the matrix-core moments read 4 pixels of one column per lane, the chi2 kernel one
+    // pixel of each column per thread)
+    const int pt = ncols <= 16 ? 16 : 32;
+    const size_t n4 = (size_t)(n + 3) & ~(size_t)3;
+    c->design_q.assign(n4 * pt, 0.0f);
     for (int i = 0; i < n; i++)
-        for (int k = 0; k < ncols; k++) c->design_cm[(size_t)k * n + i] = design[(size_t)i * ncols + k];
+        for (int k = 0; k < ncols; k++) c->design_q[(size_t)k * n4 + i] = design[(size_t)i * ncols + k];
     std::vector<double> G((size_t)ncols * ncols, 0.0);
     for (int i = 0; i < n; i++)
         for (int k = 0; k < ncols; k++)
@@ -757,7 +762,7 @@ static int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, c
         CHK(dev_copy(s.get(), c->startstop.data(), c->startstop.size(), &D.startstop));
         CHK(dev_copy(s.get(), c->chol.data(), c->chol.size(), &D.chol));
         CHK(dev_copy(s.get(), c->fac.data(), c->fac.size(), &D.fac));
-        CHK(dev_copy(s.get(), c->design_cm.data(), c->design_cm.size(), &D.design));
+        CHK(dev_copy(s.get(), c->design_q.data(), c->design_q.size(), &D.design));
         CHK(dev_copy(s.get(), c->fixedc.data(), c->fixedc.size(), &D.fixedc));
         CHK(dev_copy(s.get(), c->mblocks.data(), c->mblocks.size(), &D.mblocks));
         CHK(dev_copy(s.get(), c->mb_off.data(), c->mb_off.size(), &D.mb_off));
@@ -876,6 +881,7 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
             DevEval& ev = B.h_evals[e];
             ev.star = s;
             ev.active = 1;
+            if (D.cont_kind == CONT_LINEAR) mo = (mo + 3) & ~(int64_t)3;  // float4 reads
             ev.m_off = mo;
             mo += outlen;
             ev.mp_off = mp;

@@ -32,7 +32,8 @@ struct KVariant {
     int pm, e, f;
 };
 __host__ __device__ constexpr KVariant mom_variant(int kind, int p1) {
-    return kind == CONT_CONSTANT ? KVariant{1, 8, 0}
+    return kind == CONT_LINEAR   ? KVariant{p1 <= 16 ? 16 : 32, 64, 0}  // matrix-core kernel
+           : kind == CONT_CONSTANT ? KVariant{1, 8, 0}
            : p1 == 5             ? KVariant{10, 4, 5}
            : p1 == 9             ? KVariant{10, 4, 9}
            : p1 == 11            ? KVariant{16, 2, 11}
@@ -79,7 +80,8 @@ struct DevStar {
     const int* startstop;     // per chunk: start, stop
     const double* chol;       // per chunk (or linear): p1 x p1 lower Cholesky factor of AᵀA
     const float* fac;         // per observed pixel: overlap blend factor
-    const float* design;      // linear: p1 x N (column k contiguous)
+    const float* design;      // linear: PT = 16 (p1 <= 16) or 32 columns, column k at
+                              // k * N4 (N4 = N rounded up to 4), zero padded
     const float* fixedc;      // fixed continuum
     const int4* mblocks;      // moment blocks (chunk, i0, i1, 0)
     const int* mb_off;        // chunk c owns mblocks [mb_off[c], mb_off[c+1])

@@ -976,22 +976,6 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
                 }
             }
         }
-    } else if (S.cont_kind == CONT_LINEAR) {
-        np = P1F > 0 ? P1F : S.p1;
-        for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
-            double ad[PM];
-#pragma unroll
-            for (int k = 0; k < PM; k++) ad[k] = k < np ? (double)S.design[(size_t)k * S.N + i] : 0.0;
-            const float f = S.flux[i];
-#pragma unroll
-            for (int e = 0; e < E; e++) {
-                if (!ok[e]) continue;
-                const double y = (double)(f * __builtin_amdgcn_rcpf(m[e][i]));
-#pragma unroll
-                for (int k = 0; k < PM; k++)
-                    if (k < np) acc[e * PM + k] = fma(ad[k], y, acc[e * PM + k]);
-            }
-        }
     } else {  // constant: sum of f/m
         for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
             const float f = S.flux[i];
@@ -1007,6 +991,86 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
         if (e < bd.y && k < np) {
             const DevEval& ev = evals[bd.x + e];
             if (ev.status == ST_OK) mpart[ev.mp_off + (size_t)bd.z * np + k] = tot;
+        }
+    }
+}
+
+// Linear-continuum moments Aᵀ(f/m) on f64 matrix cores (v_mfma_f64_16x16x4f64): the
+// product of the N x p1 design with the N x E ratios y = f/m of E evaluations of one star.
+// A workgroup covers one block of observed pixels and up to 64 evaluations, 16 per
+// wavefront; a wavefront takes 16 pixels per round as 4 k-steps of 4.  Lane (j, g) =
+// (lane % 16, lane / 16) holds in k-step q the design value of parameter 16t + j and the
+// ratio of evaluation j, both at pixel i0 + 4g + q.  The design holds PT = 16 * NTL
+// columns of N4 = N rounded up to 4 values (zero where k >= p1 or i >= N), so a lane reads
+// its 4 pixels of a column as one float4.  Products of two f32 values are exact in f64 and
+// accumulate in f64, so the sums agree with a sequential f64 sum to f64 rounding.
+template <int NTL>
+__global__ __launch_bounds__(NT) void k_moments_lin(const DevStar* __restrict__ stars,
+                                                    const DevEval* __restrict__ evals,
+                                                    const int4* __restrict__ blocks,
+                                                    const float* __restrict__ model,
+                                                    double* __restrict__ mpart) {
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const int4 bd = blocks[blockIdx.x];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int e0 = wv * 16;
+    if (e0 >= bd.y) return;  // wavefront-uniform; the kernel has no barrier
+    const DevStar& S = stars[evals[bd.x].star];
+    const int4 mb = S.mblocks[bd.z];
+    const int j = lane & 15, g = lane >> 4, e = e0 + j;
+    const bool ok = e < bd.y && evals[bd.x + e].status == ST_OK;
+    const float* m = model + (ok ? evals[bd.x + e].m_off : 0);
+    const size_t n4 = (size_t)(S.N + 3) & ~(size_t)3;
+    d4 acc[2][NTL];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int t = 0; t < NTL; t++) acc[h][t] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int i0 = mb.y; i0 < mb.z; i0 += 16) {
+        const int i = i0 + 4 * g;
+        float y[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float4 a[NTL];
+#pragma unroll
+        for (int t = 0; t < NTL; t++) a[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i < mb.z) {
+#pragma unroll
+            for (int t = 0; t < NTL; t++)
+                a[t] = *reinterpret_cast<const float4*>(S.design + (size_t)(16 * t + j) * n4 + i);
+            if (i + 4 <= mb.z) {
+                const float4 f = *reinterpret_cast<const float4*>(S.flux + i);
+                if (ok) {
+                    const float4 mv = *reinterpret_cast<const float4*>(m + i);
+                    y[0] = f.x * __builtin_amdgcn_rcpf(mv.x);
+                    y[1] = f.y * __builtin_amdgcn_rcpf(mv.y);
+                    y[2] = f.z * __builtin_amdgcn_rcpf(mv.z);
+                    y[3] = f.w * __builtin_amdgcn_rcpf(mv.w);
+                }
+            } else if (ok) {
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (i + q < mb.z) y[q] = S.flux[i + q] * __builtin_amdgcn_rcpf(m[i + q]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NTL; t++) {
+            acc[0][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[t].x, (double)y[0], acc[0][t], 0, 0, 0);
+            acc[1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[t].y, (double)y[1], acc[1][t], 0, 0, 0);
+            acc[0][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[t].z, (double)y[2], acc[0][t], 0, 0, 0);
+            acc[1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[t].w, (double)y[3], acc[1][t], 0, 0, 0);
+        }
+    }
+    if (!ok) return;
+    const int np = S.p1;
+    double* out = mpart + evals[bd.x + e].mp_off + (size_t)bd.z * np;
+    // output lane layout of the f64 16x16 product: lane (j, g) register r holds row 4r + g
+    // (parameter 16t + 4r + g) of column j (evaluation j)
+#pragma unroll
+    for (int t = 0; t < NTL; t++) {
+        const d4 s = acc[0][t] + acc[1][t];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int k = 16 * t + 4 * r + g;
+            if (k < np) out[k] = s[r];
         }
     }
 }
@@ -1141,11 +1205,12 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
             }
         }
     } else if (S.cont_kind == CONT_LINEAR) {
+        const size_t n4 = (size_t)(S.N + 3) & ~(size_t)3;  // design column stride
         for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
             __asm__ volatile("" ::: "memory");
             float d[PM];
 #pragma unroll
-            for (int k = 0; k < PM; k++) d[k] = k < p1 ? S.design[(size_t)k * S.N + i] : 0.0f;
+            for (int k = 0; k < PM; k++) d[k] = k < p1 ? S.design[(size_t)k * n4 + i] : 0.0f;
             const float fl = S.flux[i], iv = S.ivar[i];
 #pragma unroll
             for (int e = 0; e < E; e++) {
@@ -1467,8 +1532,17 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
     for (int c = 0; c < n_cls; c++) {
         const ContClass& K = cls[c];
         if (K.n_mom <= 0) continue;
-        const KVariant v = mom_variant(K.kind, K.p1);
         const int4* blk = bmom + K.mom_off;
+        if (K.kind == CONT_LINEAR) {
+            if (K.p1 <= 16)
+                hipLaunchKernelGGL((k_moments_lin<1>), dim3(K.n_mom), dim3(NT), 0, st, stars,
+                                   evals, blk, model, mpart);
+            else
+                hipLaunchKernelGGL((k_moments_lin<2>), dim3(K.n_mom), dim3(NT), 0, st, stars,
+                                   evals, blk, model, mpart);
+            continue;
+        }
+        const KVariant v = mom_variant(K.kind, K.p1);
 #define PASTA_MOM(PM, E, F)                                                                  \
     if (v.pm == PM && v.e == E && v.f == F)                                                   \
         hipLaunchKernelGGL((k_moments<PM, E, F>), dim3(K.n_mom), dim3(NT), 0, st, stars, evals, \

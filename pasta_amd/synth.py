@@ -132,13 +132,15 @@ def random_labels(spec, n, seed, vsini=(0.0, 300.0), rv=(-150.0, 150.0), margin=
 
 
 def smooth_design(n, ncols, seed):
-    """Linear continuum design matrix: seeded smooth cosine columns (n x ncols)."""
+    """Linear continuum design matrix (n x ncols): a constant column and seeded-phase
+    cosines of k = 1 .. ncols-1 half periods over the range, so the columns are close to
+    orthogonal (condition number ~7 at 12 columns, ~18 at 30)."""
     rng = np.random.default_rng(seed)
     x = np.linspace(0.0, 1.0, n)
     cols = [np.ones(n)]
     for k in range(1, ncols):
         ph = rng.uniform(0, np.pi)
-        cols.append(np.cos(np.pi * k * x / 2.0 + ph))
+        cols.append(np.cos(np.pi * k * x + ph))
     return np.stack(cols, axis=1).astype(np.float32)
 
 

@@ -115,9 +115,9 @@ def _cont_pair(small, pasta, kind):
     if kind == "chunk10x8":
         return (pasta.ChunkContinuumFitter(small.wl, 10, 8, 0.2),
                 O.Continuum("chunk", n, wl=small.wl, n_chunks=10, p_order=8, overlap=0.2))
-    if kind == "linear":
+    if kind in ("linear", "linear30"):
         from pasta_amd import synth
-        dm = synth.smooth_design(n, 12, 7)
+        dm = synth.smooth_design(n, 12 if kind == "linear" else 30, 7)
         return pasta.LinearModelContinuumFitter(dm), O.Continuum("linear", n, design=dm)
     if kind == "fixed":
         x = np.linspace(-1, 1, n)
@@ -126,7 +126,7 @@ def _cont_pair(small, pasta, kind):
     return pasta.ConstantContinuum(), O.Continuum("constant", n)
 
 
-@pytest.mark.parametrize("kind", ["chunk", "chunk10x8", "linear", "fixed", "constant"])
+@pytest.mark.parametrize("kind", ["chunk", "chunk10x8", "linear", "linear30", "fixed", "constant"])
 def test_chi2_matches_oracle(small, gpu_small, kind):
     pasta, wlg, interp = gpu_small
     cont, ocont = _cont_pair(small, pasta, kind)
@@ -152,8 +152,31 @@ def test_chi2_matches_oracle(small, gpu_small, kind):
             assert abs(chi2[k] - ref) / ref < 1e-4, (lab.tolist(), chi2[k], ref)
             continue
         assert abs(chi2[k] - ref) / ref < RTOL, (kind, lab.tolist(), chi2[k], ref)
-        if kind in ("chunk", "chunk10x8", "linear", "constant"):
+        if kind in ("chunk", "chunk10x8", "linear", "linear30", "constant"):
             np.testing.assert_allclose(params[k][: len(rp)], rp, rtol=2e-4, atol=1e-3 * np.abs(rp).max())
+
+
+def test_linear_continuum_ragged(small, gpu_small):
+    """Linear continuum (continuum_fitting.rs:69-95) with 30 columns over 12,345 pixels
+    (not a whole number of pixel quads) and 70 labels (a full group of 64 evaluations and
+    a partial one), including one label outside the grid."""
+    from pasta_amd import synth
+    pasta, wlg, interp = gpu_small
+    wl = synth.observed_wavelengths(small.spec, 12_345, 500)
+    od = O.Dispersion(small.og, O.DISP_NONE, wl)
+    flux, var = synth.observe(O.produce_model(small.og, od, small.true), wl, 120.0, 41)
+    dm = synth.smooth_design(len(wl), 30, 11)
+    star = pasta.Star(interp, pasta.NoConvolutionDispersion(wl, wlg), pasta.LinearModelContinuumFitter(dm),
+                      flux, var)
+    labs = np.concatenate([[small.true], _labels_inside(small.spec, 68, 12),
+                           [[3000.0, 0.0, 4.0, 10.0, 0.0]]])
+    chi2, st, params = star.chi2(labs)
+    assert st[-1] != 0 and not st[:-1].any()
+    oc = O.Continuum("linear", len(wl), design=dm)
+    for k, lab in enumerate(labs[:-1]):
+        ref, rp = O.cost(small.og, od, oc, flux, var, lab)
+        assert abs(chi2[k] - ref) / ref < RTOL, (k, lab.tolist(), chi2[k], ref)
+        np.testing.assert_allclose(params[k][: len(rp)], rp, rtol=2e-4, atol=1e-3 * np.abs(rp).max())
 
 
 def test_chi2_errors_are_nan(small, gpu_small):
