@@ -218,6 +218,61 @@ int pasta_compute_uncertainty_bulk(pasta_star *const *stars, int n, double spec_
                                    const double *labels, const double *search_radius,
                                    double *out, int *status);
 
+/* ---------------------------------------------------------------- binary --
+ * Binary stars.  Replaces interpolate_linear (src/interpolate.rs:985-1030, Python
+ * src/lib.rs:1127-1139), produce_binary_model_norm (src/interpolate.rs:822-890,
+ * src/convolve_rv.rs:606-690, Python src/lib.rs:1013-1035) and BinaryFitter
+ * (src/fitting.rs:571-728 with BinaryBounds, src/bounds.rs:188-276; Python
+ * get_binary_fitter / fit / chi2, src/lib.rs:1177-1193,1631-1700).  Binary labels are 11
+ * values: (Teff, [M/H], logg, vsini, RV) of star 1, the same of star 2, then the light
+ * ratio.  The continuum grid `cg` holds unnormalised models on the same synthetic
+ * wavelengths; its linear interpolation weights the two normalised components. */
+/* interpolate_linear: labels P x 3 -> out P x n_pix (NaN rows + status on failure) */
+int pasta_interpolate_linear(pasta_grid *g, const double *labels, int P, float *out,
+                             int *status);
+typedef struct pasta_binary pasta_binary;
+/* An observed binary (the BinaryCostFunction fields, src/fitting.rs:571-578); c, flux and
+ * var may all be NULL for model-only use (pasta_binary_model). */
+int pasta_binary_create(pasta_grid *g, pasta_grid *cg, const pasta_disp *d, const pasta_cont *c,
+                        const float *flux, const float *var, int n, pasta_binary **out);
+int pasta_binary_destroy(pasta_binary *b);
+/* produce_binary_model_norm for labels P x 11 -> out P x n (light ratio as f32) */
+int pasta_binary_model(pasta_binary *b, const double *labels, int P, float *out, int *status);
+/* BinaryCostFunction::cost = BinaryFitter::chi2 (src/fitting.rs:581-604,708-728): labels
+ * P x 11 -> chi2 (NaN + status on failure); params (nullable) P x n_params. */
+int pasta_binary_eval_chi2(pasta_binary *b, const double *labels, int P, double *chi2,
+                           int *status, float *params);
+typedef struct pasta_binary_settings {
+    int num_particles;
+    int max_iters;
+    double inertia_factor;
+    double cognitive_factor;
+    double social_factor;
+    double delta;
+    double vsini_range[2];
+    double rv_range[2];
+    double light_ratio_range[2]; /* (0, 1) in BinaryFitter::fit (src/fitting.rs:656-661) */
+    int constraint_kind[11];     /* per binary label: 0 none, 1 fixed (a), 2 range [a, b] */
+    double constraint_a[11];
+    double constraint_b[11];
+} pasta_binary_settings;
+typedef struct pasta_binary_result {
+    double label1[5];
+    double label2[5];
+    double light_ratio;
+    double chi2;
+    int iterations;
+    int status;
+    double time_s;
+    int n_params;
+    float continuum_params[512];
+} pasta_binary_result;
+/* BinaryFitter::fit (src/fitting.rs:635-706): the PSO over the 11 labels, one batched
+ * evaluation per iteration; seed and init_pos (P x 11) as pasta_fit; trace optional
+ * (max_iters+1) x 12 of (global-best cost, position). */
+int pasta_binary_fit(pasta_binary *b, const pasta_binary_settings *st, uint64_t seed,
+                     const double *init_pos, pasta_binary_result *out, double *trace);
+
 /* ---------------------------------------------------------------- server --
  * Continuous batching of fit requests over a resident grid: requests are
  * admitted into free slots (and free HBM) between PSO iterations, every

@@ -55,6 +55,13 @@ class _PsoSettings(C.Structure):
                 ("cognitive", C.c_double), ("social", C.c_double), ("delta", C.c_double)]
 
 
+class _BinBounds(C.Structure):
+    _fields_ = [("g", C.c_void_p), ("vsini_lo", C.c_double), ("vsini_hi", C.c_double),
+                ("rv_lo", C.c_double), ("rv_hi", C.c_double), ("lr_lo", C.c_double),
+                ("lr_hi", C.c_double), ("ckind", C.c_int * 11), ("ca", C.c_double * 11),
+                ("cb", C.c_double * 11)]
+
+
 COST_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, _dp, _dp)
 
 SCALAR_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_double))
@@ -113,6 +120,21 @@ _sig = {
     "or_fit": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, C.POINTER(_Bounds),
                          C.POINTER(_PsoSettings), C.c_uint64, _dp, C.c_int, _dp, _dp, _fp,
                          C.POINTER(C.c_long)]),
+    "or_local_weights_linear": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, _ip,
+                                          _fp, _dp]),
+    "or_interpolate_linear_f32": (C.c_int, [C.c_void_p, _u16p, _fp, C.c_int, C.c_double,
+                                            C.c_double, C.c_double, _fp]),
+    "or_interpolate_linear_f64": (C.c_int, [C.c_void_p, _u16p, _fp, C.c_int, C.c_double,
+                                            C.c_double, C.c_double, _dp]),
+    "or_binary_model_norm": (C.c_int, [C.POINTER(_Model), C.POINTER(_Model), C.c_void_p, _dp, _dp,
+                                       C.c_float, _fp]),
+    "or_binary_cost": (C.c_int, [C.POINTER(_Model), C.POINTER(_Model), C.c_void_p, C.c_void_p,
+                                 _fp, _fp, _dp, _dp, _fp]),
+    "or_binbounds_outer": (C.c_int, [C.POINTER(_BinBounds), _dp, _dp]),
+    "or_binbounds_within": (C.c_int, [C.POINTER(_BinBounds), _dp]),
+    "or_binbounds_clamp": (C.c_int, [C.POINTER(_BinBounds), _dp, C.c_int, _dp]),
+    "or_pso_run_binary": (C.c_int, [C.POINTER(_BinBounds), C.POINTER(_PsoSettings), C.c_uint64,
+                                    _dp, COST_FN, C.c_void_p, _dp, _dp, _dp, C.POINTER(C.c_long)]),
     "or_brent_root": (C.c_int, [SCALAR_FN, C.c_void_p, C.c_double, C.c_double, C.c_double, _dp, _ip]),
     "or_compute_uncertainty": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp,
                                          C.c_double, _dp, _dp, _dp, _ip]),
@@ -190,6 +212,29 @@ class Grid:
                                  _p(fm, C.c_double))
         _chk(rc, "local_weights")
         return idx, w32, w64, ft.reshape(4, 4), fl.reshape(4, 4), fm
+
+    def local_weights_linear(self, teff, m, logg):
+        idx = np.zeros(8, np.int32)
+        w32 = np.zeros(8, np.float32)
+        w64 = np.zeros(8)
+        _chk(_L.or_local_weights_linear(self._h, teff, m, logg, _p(idx, C.c_int),
+                                        _p(w32, C.c_float), _p(w64, C.c_double)),
+             "local_weights_linear")
+        return idx, w32, w64
+
+    def interpolate_linear(self, teff, m, logg, f64=False):
+        fac = None if self.factors is None else _p(self.factors, C.c_float)
+        if f64:
+            out = np.zeros(self.W)
+            _chk(_L.or_interpolate_linear_f64(self._h, _p(self.spectra, C.c_uint16), fac, self.W,
+                                              teff, m, logg, _p(out, C.c_double)),
+                 "interpolate_linear")
+        else:
+            out = np.zeros(self.W, np.float32)
+            _chk(_L.or_interpolate_linear_f32(self._h, _p(self.spectra, C.c_uint16), fac, self.W,
+                                              teff, m, logg, _p(out, C.c_float)),
+                 "interpolate_linear")
+        return out
 
     def model(self):
         fac = None if self.factors is None else _p(self.factors, C.c_float)
@@ -472,3 +517,78 @@ def compute_uncertainty(grid, disp, cont, flux, var, spec_res, label, search_rad
                                    None if sr is None else _p(sr, C.c_double), _p(out, C.c_double),
                                    _p(st, C.c_int)), "compute_uncertainty")
     return out.reshape(5, 2), st.reshape(5, 2)
+
+
+def binary_model_norm(grid, cgrid, disp, labels1, labels2, light_ratio):
+    """produce_binary_model_norm (interpolate.rs:868-890): f32 model at the observed pixels."""
+    M, CM = grid.model(), cgrid.model()
+    out = np.zeros(disp.n, np.float32)
+    _chk(_L.or_binary_model_norm(C.byref(M), C.byref(CM), disp._h, _p(_d(labels1), C.c_double),
+                                 _p(_d(labels2), C.c_double), float(light_ratio),
+                                 _p(out, C.c_float)), "binary_model_norm")
+    return out
+
+
+def binary_cost(grid, cgrid, disp, cont, flux, var, labels11):
+    """BinaryCostFunction::cost (fitting.rs:581-604); returns (chi2, params)."""
+    flux, var = _f(flux), _f(var)
+    M, CM = grid.model(), cgrid.model()
+    c = C.c_double()
+    params = np.zeros(max(1, cont.n_params), np.float32)
+    _chk(_L.or_binary_cost(C.byref(M), C.byref(CM), disp._h, cont._h, _p(flux, C.c_float),
+                           _p(var, C.c_float), _p(_d(labels11), C.c_double), C.byref(c),
+                           _p(params, C.c_float)), "binary_cost")
+    return c.value, params
+
+
+def binary_bounds(grid, vsini_range, rv_range, light_ratio_range=(0.0, 1.0), constraints=()):
+    b = _BinBounds()
+    b.g = grid._h
+    b.vsini_lo, b.vsini_hi = vsini_range
+    b.rv_lo, b.rv_hi = rv_range
+    b.lr_lo, b.lr_hi = light_ratio_range
+    for (d, kind, a, bb) in constraints:
+        b.ckind[d] = kind
+        b.ca[d] = a
+        b.cb[d] = bb
+    return b
+
+
+def binary_bounds_outer(b):
+    lo, hi = np.zeros(11), np.zeros(11)
+    _L.or_binbounds_outer(C.byref(b), _p(lo, C.c_double), _p(hi, C.c_double))
+    return lo, hi
+
+
+def binary_bounds_within(b, p):
+    return bool(_L.or_binbounds_within(C.byref(b), _p(_d(p), C.c_double)))
+
+
+def binary_bounds_clamp(b, p, d):
+    out = C.c_double()
+    _chk(_L.or_binbounds_clamp(C.byref(b), _p(_d(p), C.c_double), d, C.byref(out)), "clamp")
+    return out.value
+
+
+def pso_run_binary(b, s, seed, cost_fn, init_pos=None):
+    """The 11-parameter PSO with a Python cost f(positions[n,11]) -> costs[n]."""
+    def _cb(ctx, n, pos, out):
+        try:
+            p = np.ctypeslib.as_array(pos, shape=(n, 11)).copy()
+            c = np.asarray(cost_fn(p), dtype=np.float64)
+            for i in range(n):
+                out[i] = c[i]
+            return 0
+        except Exception:
+            return -7
+    cb = COST_FN(_cb)
+    best = np.zeros(11)
+    bc = C.c_double()
+    trace = np.zeros((s.max_iters + 1, 12))
+    ne = C.c_long()
+    ipa = None if init_pos is None else _d(init_pos)
+    _chk(_L.or_pso_run_binary(C.byref(b), C.byref(s), seed,
+                              None if ipa is None else _p(ipa, C.c_double), cb, None,
+                              _p(best, C.c_double), C.byref(bc), _p(trace, C.c_double),
+                              C.byref(ne)), "pso_run_binary")
+    return best, bc.value, trace, ne.value

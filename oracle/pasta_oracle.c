@@ -1765,3 +1765,334 @@ int or_compute_uncertainty(const or_model *M, const or_disp *d, const or_cont *c
     free(u.synth);
     return OR_OK;
 }
+
+/* ======================================================================
+ * Binary stars -- interpolate.rs:168-212,536-620,796-890,985-1030,
+ * cubic.rs:6-10,246-262, convolve_rv.rs:606-703, fitting.rs:571-728,
+ * bounds.rs:188-276
+ * ====================================================================== */
+
+/* Range::find_neighbors_linear (interpolate.rs:168-212) */
+static int find_neighbors_linear(const double *v, int n, double x, int left, int right,
+                                 int out[2]) {
+    int r = right_index(v, n, x);
+    if (r >= 0) {
+        if (r < left || r > right) return OR_ERR_BOUNDS;
+        if (r == right) { out[0] = r - 1; out[1] = r; }
+        else { out[0] = r; out[1] = r + 1; }
+        return OR_OK;
+    }
+    int i = -r - 1;
+    if (i <= left || i > right) return OR_ERR_BOUNDS;
+    out[0] = i - 1; out[1] = i;
+    return OR_OK;
+}
+
+/* calculate_factors_linear (cubic.rs:6-10) */
+static void factors_linear(double x, double x0, double x1, double f[2]) {
+    f[0] = (x - x1) / (x0 - x1);
+    f[1] = (x - x0) / (x1 - x0);
+}
+
+int or_local_weights_linear(const or_grid *g, double teff, double m, double logg, int idx[8],
+                            float w[8], double w64[8]) {
+    /* get_local_grid_linear (interpolate.rs:536-620) */
+    if (!or_is_teff_logg_between_bounds(g, teff, logg)) return OR_ERR_BOUNDS;
+    if (!(m >= g->m[0] && m <= g->m[g->n_m - 1])) return OR_ERR_BOUNDS;
+    int tl, th, ll, lh, rc;
+    if ((rc = or_get_teff_index_limits_at(g, logg, &tl, &th))) return rc;
+    if ((rc = or_get_logg_index_limits_at(g, teff, &ll, &lh))) return rc;
+    int ti[2], mi[2], li[2];
+    if ((rc = find_neighbors_linear(g->teff, g->n_teff, teff, tl, th, ti))) return rc;
+    if ((rc = find_neighbors_linear(g->m, g->n_m, m, 0, g->n_m - 1, mi))) return rc;
+    if ((rc = find_neighbors_linear(g->logg, g->n_logg, logg, ll, lh, li))) return rc;
+    /* calculate_interpolation_coefficients_linear (cubic.rs:246-262): the cartesian
+     * product teff x logg x m, (a * b) * c in f64, then f32 */
+    double ft[2], fl[2], fm[2];
+    factors_linear(teff, g->teff[ti[0]], g->teff[ti[1]], ft);
+    factors_linear(logg, g->logg[li[0]], g->logg[li[1]], fl);
+    factors_linear(m, g->m[mi[0]], g->m[mi[1]], fm);
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int c = 0; c < 2; c++) {
+                int q = (a * 2 + b) * 2 + c;
+                /* find_spectrum(teff, m, logg) (model_fetchers.rs:234-236); usize
+                 * arithmetic: a logg index below the column's first panics */
+                if (li[b] < g->logg_lo[ti[a]]) return OR_ERR_INDEX;
+                int id = or_grid_model_index(g, ti[a], mi[c], li[b]);
+                if (id >= g->n_models) return OR_ERR_INDEX;
+                idx[q] = id;
+                double p = (ft[a] * fl[b]) * fm[c];
+                if (w) w[q] = (float)p;
+                if (w64) w64[q] = p;
+            }
+    return OR_OK;
+}
+
+/* interpolate_linear (interpolate.rs:985-1030): every pixel is x / 65535 * factor (no
+ * 128-pixel batch distinction), then the 8-column gemv in column order */
+int or_interpolate_linear_f32(const or_grid *g, const uint16_t *spectra, const float *factors,
+                              int W, double teff, double m, double logg, float *out) {
+    int idx[8];
+    float w[8];
+    int rc = or_local_weights_linear(g, teff, m, logg, idx, w, NULL);
+    if (rc) return rc;
+    float fac[8];
+    for (int j = 0; j < 8; j++) fac[j] = factors ? factors[idx[j]] : (float)(1.0 / 65535.0);
+    for (int x = 0; x < W; x++) {
+        float y = 0.0f;
+        for (int j = 0; j < 8; j++) {
+            float Mv = (float)spectra[(size_t)idx[j] * W + x] / 65535.0f * fac[j];
+            float t = Mv * w[j];
+            y = (j == 0) ? t : t + y;
+        }
+        out[x] = y;
+    }
+    return OR_OK;
+}
+
+int or_interpolate_linear_f64(const or_grid *g, const uint16_t *spectra, const float *factors,
+                              int W, double teff, double m, double logg, double *out) {
+    int idx[8];
+    double w[8];
+    int rc = or_local_weights_linear(g, teff, m, logg, idx, NULL, w);
+    if (rc) return rc;
+    for (int x = 0; x < W; x++) {
+        double y = 0.0;
+        for (int j = 0; j < 8; j++) {
+            double f = factors ? (double)factors[idx[j]] : 1.0 / 65535.0;
+            y += (double)spectra[(size_t)idx[j] * W + x] / 65535.0 * f * w[j];
+        }
+        out[x] = y;
+    }
+    return OR_OK;
+}
+
+/* DVector::mean of nalgebra 0.32.6, restated as a sequential f32 fold and one division
+ * (ULP behaviour unpinned: the crate is absent from the reference tree) */
+static float mean_f32(const float *v, int n) {
+    float s = 0.0f;
+    for (int i = 0; i < n; i++) s = s + v[i];
+    return s / (float)n;
+}
+
+/* produce_binary_components + shift_resample_and_add_binary_components
+ * (interpolate.rs:822-890, convolve_rv.rs:606-690, Logspace synthetic grid) */
+int or_binary_model_norm(const or_model *M, const or_model *CM, const or_disp *d,
+                         const double lab1[5], const double lab2[5], float light_ratio,
+                         float *out) {
+    int W = M->sg.n, rc;
+    if (CM->sg.n != W) return OR_ERR_ARG;
+    float *interp = malloc(sizeof(float) * W);
+    float *c1 = malloc(sizeof(float) * W), *c2 = malloc(sizeof(float) * W);
+    seg32 s1 = {0}, s2 = {0};
+    /* components in the reference's order: model 1, continuum 1, model 2, continuum 2 */
+    rc = or_interpolate_f32(M->g, M->spectra, M->factors, W, lab1[0], lab1[1], lab1[2], interp);
+    if (!rc) rc = rot_conv32(M, interp, lab1[3], &s1);
+    if (!rc) rc = disp_conv32(d, &s1);
+    if (!rc) rc = or_interpolate_linear_f32(CM->g, CM->spectra, CM->factors, W, lab1[0], lab1[1],
+                                            lab1[2], c1);
+    if (!rc) rc = or_interpolate_f32(M->g, M->spectra, M->factors, W, lab2[0], lab2[1], lab2[2],
+                                     interp);
+    if (!rc) rc = rot_conv32(M, interp, lab2[3], &s2);
+    if (!rc) rc = disp_conv32(d, &s2);
+    if (!rc) rc = or_interpolate_linear_f32(CM->g, CM->spectra, CM->factors, W, lab2[0], lab2[1],
+                                            lab2[2], c2);
+    if (rc) goto done;
+    {
+        float lr = light_ratio * mean_f32(c1, W) / mean_f32(c2, W);
+        double sf1 = 1.0 - lab1[4] / C_KMS, sf2 = 1.0 - lab2[4] / C_KMS;
+        double first = pow(10.0, M->sg.first);
+        double last = pow(10.0, M->sg.first + M->sg.step * (double)W);
+        if (d->wl[0] * fmin(sf1, sf2) < first) { rc = OR_ERR_RANGE; goto done; }
+        if (d->wl[d->n_obs - 1] * fmax(sf1, sf2) > last) { rc = OR_ERR_RANGE; goto done; }
+        double sl1 = log10(sf1) / M->sg.step, sl2 = log10(sf2) / M->sg.step;
+        for (int i = 0; i < d->n_obs; i++) {
+            double x1 = d->fidx[i] + sl1, x2 = d->fidx[i] + sl2;
+            double f1 = floor(x1), f2 = floor(x2);
+            long j1 = f1 > 0 ? (long)f1 : 0, j2 = f2 > 0 ? (long)f2 : 0;
+            if (s1.start + j1 + 1 >= s1.alen || s2.start + j2 + 1 >= s2.alen || j1 + 1 >= W ||
+                j2 + 1 >= W) { rc = OR_ERR_INDEX; goto done; }
+            float w1 = (float)x1 - (float)j1, w2 = (float)x2 - (float)j2;
+            float n1 = (1.0f - w1) * s1.a[s1.start + j1] + w1 * s1.a[s1.start + j1 + 1];
+            float k1 = (1.0f - w1) * c1[j1] + w1 * c1[j1 + 1];
+            float n2 = (1.0f - w2) * s2.a[s2.start + j2] + w2 * s2.a[s2.start + j2 + 1];
+            float k2 = (1.0f - w2) * c2[j2] + w2 * c2[j2 + 1];
+            out[i] = (n1 * k1 + lr * n2 * k2) / (k1 + lr * k2);
+        }
+    }
+done:
+    free(interp); free(c1); free(c2); free(s1.a); free(s2.a);
+    return rc;
+}
+
+/* BinaryCostFunction::cost (fitting.rs:581-604): labels (star 1, star 2, light ratio) */
+int or_binary_cost(const or_model *M, const or_model *CM, const or_disp *d, const or_cont *c,
+                   const float *flux, const float *var, const double lab[11], double *chi2,
+                   float *params) {
+    float *synth = malloc(sizeof(float) * d->n_obs);
+    int rc = or_binary_model_norm(M, CM, d, lab, lab + 5, (float)lab[10], synth);
+    if (!rc) rc = or_fit_continuum(c, flux, var, synth, params, chi2);
+    free(synth);
+    return rc;
+}
+
+/* BinaryBounds (bounds.rs:188-276) + ConstrainedPSOBounds (bounds.rs:64-132) */
+static int binb_limits(const or_binbounds *b, const double p[11], int i, double *lo, double *hi) {
+    int rc = OR_OK;
+    switch (i) {
+    case 3: case 8: *lo = b->vsini_lo; *hi = b->vsini_hi; break;
+    case 4: case 9: *lo = b->rv_lo; *hi = b->rv_hi; break;
+    case 10: *lo = b->lr_lo; *hi = b->lr_hi; break;
+    default:
+        rc = i < 5 ? grid_limits_at(b->g, p, i, lo, hi) : grid_limits_at(b->g, p + 5, i - 5, lo, hi);
+    }
+    if (rc) return rc;
+    if (b->ckind[i] == 1) { *lo = b->ca[i]; *hi = b->ca[i]; }
+    else if (b->ckind[i] == 2) { *lo = fmax(*lo, b->ca[i]); *hi = fmin(*hi, b->cb[i]); }
+    return OR_OK;
+}
+
+int or_binbounds_outer(const or_binbounds *b, double lo[11], double hi[11]) {
+    const or_grid *g = b->g;
+    for (int s = 0; s < 2; s++) {
+        lo[5 * s] = g->teff[0]; hi[5 * s] = g->teff[g->n_teff - 1];
+        lo[5 * s + 1] = g->m[0]; hi[5 * s + 1] = g->m[g->n_m - 1];
+        lo[5 * s + 2] = g->logg[0]; hi[5 * s + 2] = g->logg[g->n_logg - 1];
+        lo[5 * s + 3] = b->vsini_lo; hi[5 * s + 3] = b->vsini_hi;
+        lo[5 * s + 4] = b->rv_lo; hi[5 * s + 4] = b->rv_hi;
+    }
+    lo[10] = b->lr_lo; hi[10] = b->lr_hi;
+    for (int d = 0; d < 11; d++) {
+        if (b->ckind[d] == 1) { lo[d] = b->ca[d]; hi[d] = b->ca[d]; }
+        else if (b->ckind[d] == 2) { lo[d] = fmax(lo[d], b->ca[d]); hi[d] = fmin(hi[d], b->cb[d]); }
+    }
+    return OR_OK;
+}
+
+int or_binbounds_within(const or_binbounds *b, const double p[11]) {
+    const or_grid *g = b->g;
+    int ok = 1;
+    for (int s = 0; s < 2 && ok; s++) {
+        const double *q = p + 5 * s;
+        ok = or_is_teff_logg_between_bounds(g, q[0], q[2]) && q[1] >= g->m[0] &&
+             q[1] <= g->m[g->n_m - 1];
+    }
+    ok = ok && p[3] >= b->vsini_lo && p[3] <= b->vsini_hi && p[4] >= b->rv_lo &&
+         p[4] <= b->rv_hi && p[8] >= b->vsini_lo && p[8] <= b->vsini_hi && p[9] >= b->rv_lo &&
+         p[9] <= b->rv_hi && p[10] >= b->lr_lo && p[10] <= b->lr_hi;
+    if (!ok) return 0;
+    for (int d = 0; d < 11; d++) {
+        if (b->ckind[d] == 1 && !(fabs(p[d] - b->ca[d]) <= 2.220446049250313e-16)) return 0;
+        if (b->ckind[d] == 2 && !(p[d] >= b->ca[d] && p[d] <= b->cb[d])) return 0;
+    }
+    return 1;
+}
+
+int or_binbounds_clamp(const or_binbounds *b, const double p[11], int d, double *out) {
+    double lo, hi;
+    int rc = binb_limits(b, p, d, &lo, &hi);
+    if (rc) return OR_ERR_PSO;
+    if (lo > hi) return OR_ERR_PSO;
+    *out = clampd(p[d], lo, hi);
+    return OR_OK;
+}
+
+typedef struct { double pos[11], vel[11], cost, bpos[11], bcost; } particle11_t;
+
+static void sort_particles11(particle11_t *p, int n) {
+    for (int i = 1; i < n; i++) {
+        particle11_t t = p[i];
+        int j = i - 1;
+        while (j >= 0 && p[j].cost > t.cost) { p[j + 1] = p[j]; j--; }
+        p[j + 1] = t;
+    }
+}
+
+/* The PSO of particleswarm.rs over the 11 binary parameters (BinaryFitter::fit,
+ * fitting.rs:635-706); same draws and update order as or_pso_run with D = 11 */
+int or_pso_run_binary(const or_binbounds *b, const or_pso_settings *s, uint64_t seed,
+                      const double *init_pos, or_cost_fn fn, void *ctx, double best[11],
+                      double *best_cost, double *trace_best, long *n_evals) {
+    const int D = 11, P = s->num_particles;
+    double lo[11], hi[11];
+    or_binbounds_outer(b, lo, hi);
+    double *pos = malloc(sizeof(double) * D * P), *vel = malloc(sizeof(double) * D * P);
+    double *cost = malloc(sizeof(double) * P);
+    particle11_t *pt = malloc(sizeof(particle11_t) * P);
+    int rc = OR_OK;
+    uint64_t kp = or_rng_u64(seed, KEY_POS), kv = or_rng_u64(seed, KEY_VEL);
+    uint64_t attempt = 0;
+    for (int got = 0; got < P;) {
+        double p[11];
+        for (int d = 0; d < D; d++)
+            p[d] = rand_range(lo[d], hi[d], or_rng_uniform(kp, attempt * D + d));
+        attempt++;
+        if (attempt > 100000000ULL) { rc = OR_ERR_PSO; goto done; }
+        if (or_binbounds_within(b, p)) { memcpy(pos + D * got, p, sizeof(p)); got++; }
+    }
+    for (int q = 0; q < P; q++)
+        for (int d = 0; d < D; d++) {
+            double delta = hi[d] - lo[d];
+            vel[D * q + d] = rand_range(-delta, delta, or_rng_uniform(kv, (uint64_t)q * D + d));
+        }
+    if (init_pos) memcpy(pos, init_pos, sizeof(double) * D * P);
+    long evals = 0;
+    rc = fn(ctx, P, pos, cost);
+    evals += P;
+    if (rc) goto done;
+    for (int q = 0; q < P; q++) {
+        memcpy(pt[q].pos, pos + D * q, sizeof(double) * D);
+        memcpy(pt[q].vel, vel + D * q, sizeof(double) * D);
+        memcpy(pt[q].bpos, pos + D * q, sizeof(double) * D);
+        pt[q].cost = pt[q].bcost = cost[q];
+    }
+    sort_particles11(pt, P);
+    double g[11], gc = pt[0].cost;
+    memcpy(g, pt[0].pos, sizeof(g));
+    if (trace_best) { trace_best[0] = gc; memcpy(trace_best + 1, g, sizeof(g)); }
+    uint64_t ki = or_rng_u64(seed, KEY_ITER);
+    for (int it = 1; it <= s->max_iters && !rc; it++) {
+        for (int q = 0; q < P && !rc; q++) {
+            for (int d = 0; d < D; d++) {
+                uint64_t c0 = (((uint64_t)it * P + q) * D + d) * 2;
+                double momentum = pt[q].vel[d] * s->inertia;
+                double to_opt = pt[q].bpos[d] - pt[q].pos[d];
+                double pull = rand_range(0.0, to_opt, or_rng_uniform(ki, c0)) * s->cognitive;
+                double to_g = g[d] - pt[q].pos[d];
+                double pullg = rand_range(0.0, to_g, or_rng_uniform(ki, c0 + 1)) * s->social;
+                pt[q].vel[d] = momentum + pull + pullg;
+                double np[11];
+                memcpy(np, pt[q].pos, sizeof(np));
+                np[d] += pt[q].vel[d];
+                double cl;
+                rc = or_binbounds_clamp(b, np, d, &cl);
+                if (rc) break;
+                pt[q].pos[d] = cl;
+                if (!or_binbounds_within(b, pt[q].pos)) { rc = OR_ERR_PSO; break; }
+            }
+        }
+        if (rc) break;
+        for (int q = 0; q < P; q++) memcpy(pos + D * q, pt[q].pos, sizeof(double) * D);
+        rc = fn(ctx, P, pos, cost);
+        evals += P;
+        if (rc) break;
+        for (int q = 0; q < P; q++) {
+            pt[q].cost = cost[q];
+            if (pt[q].cost < pt[q].bcost) {
+                memcpy(pt[q].bpos, pt[q].pos, sizeof(double) * D);
+                pt[q].bcost = pt[q].cost;
+                if (pt[q].cost < gc) {
+                    memcpy(g, pt[q].pos, sizeof(g));
+                    gc = pt[q].cost;
+                }
+            }
+        }
+        if (trace_best) { trace_best[12 * it] = gc; memcpy(trace_best + 12 * it + 1, g, sizeof(g)); }
+    }
+    if (!rc) { memcpy(best, g, sizeof(g)); *best_cost = gc; }
+    if (n_evals) *n_evals = evals;
+done:
+    free(pos); free(vel); free(cost); free(pt);
+    return rc;
+}

@@ -192,6 +192,38 @@ int or_fit(const or_model *M, const or_disp *d, const or_cont *c, const float *f
            const double *init_pos, int n_threads, double best[5], double *best_cost,
            float *params, long *n_evals);
 
+/* ---------------- binary stars (interpolate.rs:796-890,985-1030, convolve_rv.rs:606-703,
+ * fitting.rs:571-728, bounds.rs:188-276) ---------------- */
+/* get_local_grid_linear + calculate_interpolation_coefficients_linear: the 8 neighbours
+ * (teff x logg x [M/H] order) and weights (w f32 and/or w64 f64, either may be NULL) */
+int or_local_weights_linear(const or_grid *g, double teff, double m, double logg, int idx[8],
+                            float w[8], double w64[8]);
+int or_interpolate_linear_f32(const or_grid *g, const uint16_t *spectra, const float *factors,
+                              int W, double teff, double m, double logg, float *out);
+int or_interpolate_linear_f64(const or_grid *g, const uint16_t *spectra, const float *factors,
+                              int W, double teff, double m, double logg, double *out);
+/* produce_binary_model_norm: CM is the continuum (unnormalised) grid; out has n_obs floats */
+int or_binary_model_norm(const or_model *M, const or_model *CM, const or_disp *d,
+                         const double lab1[5], const double lab2[5], float light_ratio,
+                         float *out);
+/* BinaryCostFunction::cost: lab = star 1 (5), star 2 (5), light ratio */
+int or_binary_cost(const or_model *M, const or_model *CM, const or_disp *d, const or_cont *c,
+                   const float *flux, const float *var, const double lab[11], double *chi2,
+                   float *params);
+typedef struct or_binbounds {
+    const or_grid *g;
+    double vsini_lo, vsini_hi, rv_lo, rv_hi, lr_lo, lr_hi;
+    int ckind[11]; /* 0 none, 1 fixed, 2 range */
+    double ca[11], cb[11];
+} or_binbounds;
+int or_binbounds_outer(const or_binbounds *b, double lo[11], double hi[11]);
+int or_binbounds_within(const or_binbounds *b, const double p[11]);
+int or_binbounds_clamp(const or_binbounds *b, const double p[11], int d, double *out);
+/* the PSO over 11 parameters; fn receives positions n x 11; trace_best (max_iters+1) x 12 */
+int or_pso_run_binary(const or_binbounds *b, const or_pso_settings *s, uint64_t seed,
+                      const double *init_pos, or_cost_fn fn, void *ctx, double best[11],
+                      double *best_cost, double *trace_best, long *n_evals);
+
 /* ---------------- uncertainty (fitting.rs:333-372,482-568) ---------------- */
 /* BrentRoot of argmin 0.10.0 (a dependency absent from the reference tree), restated from
  * the Brent / zbrent root finder it implements: brackets [min, max], tolerance tol, returns

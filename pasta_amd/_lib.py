@@ -60,6 +60,37 @@ class FitResult(C.Structure):
     ]
 
 
+class BinarySettings(C.Structure):
+    _fields_ = [
+        ("num_particles", C.c_int),
+        ("max_iters", C.c_int),
+        ("inertia_factor", C.c_double),
+        ("cognitive_factor", C.c_double),
+        ("social_factor", C.c_double),
+        ("delta", C.c_double),
+        ("vsini_range", C.c_double * 2),
+        ("rv_range", C.c_double * 2),
+        ("light_ratio_range", C.c_double * 2),
+        ("constraint_kind", C.c_int * 11),
+        ("constraint_a", C.c_double * 11),
+        ("constraint_b", C.c_double * 11),
+    ]
+
+
+class BinaryResult(C.Structure):
+    _fields_ = [
+        ("label1", C.c_double * 5),
+        ("label2", C.c_double * 5),
+        ("light_ratio", C.c_double),
+        ("chi2", C.c_double),
+        ("iterations", C.c_int),
+        ("status", C.c_int),
+        ("time_s", C.c_double),
+        ("n_params", C.c_int),
+        ("continuum_params", C.c_float * 512),
+    ]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("max_stars", C.c_int),
@@ -128,6 +159,13 @@ SIGNATURES = {
     "pasta_server_timing": [_P, C.c_int, _dp],
     "pasta_server_set_timing": [_P, C.c_int],
     "pasta_grid_read_rows": [_P, C.c_int, C.c_int, _u16p],
+    "pasta_interpolate_linear": [_P, _dp, C.c_int, _fp, _ip],
+    "pasta_binary_create": [_P, _P, _P, _P, _fp, _fp, C.c_int, _PP],
+    "pasta_binary_destroy": [_P],
+    "pasta_binary_model": [_P, _dp, C.c_int, _fp, _ip],
+    "pasta_binary_eval_chi2": [_P, _dp, C.c_int, _dp, _ip, _fp],
+    "pasta_binary_fit": [_P, C.POINTER(BinarySettings), C.c_uint64, _dp, C.POINTER(BinaryResult),
+                         _dp],
 }
 
 

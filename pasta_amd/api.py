@@ -12,6 +12,8 @@ for the model-evaluation path:
         get_neighbor_indices, teffs, ms, loggs, logg_limits, cumulative_grid_size,
         list_gridpoints, is_teff_logg_between_bounds, clamp_1d}
     InMemSingleFitter.{fit, fit_bulk, chi2}
+    InMemInterpolator.{interpolate_linear, produce_binary_model_norm, get_binary_fitter}
+    BinaryFitter.{fit, chi2}, BinaryOptimizationResult
 
 Every computation runs in the HIP library; there is no CPU fallback.  Errors
 raise ValueError like PyO3's anyhow conversion.
@@ -31,7 +33,7 @@ __all__ = [
     "NoConvolutionDispersion", "FixedResolutionDispersion", "VariableResolutionDispersion",
     "ChunkContinuumFitter", "LinearModelContinuumFitter", "FixedContinuum", "ConstantContinuum",
     "FixConstraint", "RangeConstraint", "get_vsini_kernel", "InMemSingleFitter", "FitServer",
-    "LabelUncertainties",
+    "LabelUncertainties", "BinaryFitter", "BinaryOptimizationResult",
 ]
 
 
@@ -108,6 +110,35 @@ class OptimizationResult:
         lab = Label(*r.label[:5])
         return OptimizationResult(lab, r.continuum_params[: r.n_params], r.chi2, r.iterations,
                                   r.time_s)
+
+
+class BinaryOptimizationResult:
+    """Output of a binary fit (src/lib.rs:296-360)."""
+
+    def __init__(self, label1, label2, light_ratio, continuum_params, chi2, iterations, time):
+        self.label1, self.label2 = label1, label2
+        self.light_ratio, self.continuum_params = float(light_ratio), list(continuum_params)
+        self.chi2, self.iterations, self.time = float(chi2), int(iterations), float(time)
+
+    def to_dict(self):
+        return dict(label1=self.label1.to_dict(), label2=self.label2.to_dict(),
+                    light_ratio=self.light_ratio, continuum_params=self.continuum_params,
+                    chi2=self.chi2, iterations=self.iterations, time=self.time)
+
+    def to_json(self):
+        return json.dumps(self.to_dict())
+
+    @staticmethod
+    def from_dict(d):
+        return BinaryOptimizationResult(Label.from_dict(d["label1"]), Label.from_dict(d["label2"]),
+                                        d["light_ratio"], d["continuum_params"], d["chi2"],
+                                        d["iterations"], d["time"])
+
+    @staticmethod
+    def _from_c(r):
+        return BinaryOptimizationResult(Label(*r.label1[:5]), Label(*r.label2[:5]), r.light_ratio,
+                                        r.continuum_params[: r.n_params], r.chi2, r.iterations,
+                                        r.time_s)
 
 
 class LabelUncertainties:
@@ -435,6 +466,41 @@ class InMemInterpolator:
                               ptr(st, C.c_int))
         return out, st
 
+    def interpolate_linear(self, teff, m, logg):
+        """Trilinear interpolation of the 8 neighbours (src/lib.rs:1127-1139)."""
+        out, st = self.interpolate_linear_bulk([[teff, m, logg]])
+        if st[0]:
+            raise ValueError(f"Error while interpolating: {_lib.STATUS_TEXT.get(int(st[0]))}")
+        return out[0]
+
+    def interpolate_linear_bulk(self, labels):
+        lab = f64(labels).reshape(-1, 3)
+        out = np.zeros((len(lab), self.n_pix), np.float32)
+        st = np.zeros(len(lab), np.int32)
+        lib.pasta_interpolate_linear(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_float),
+                                     ptr(st, C.c_int))
+        return out, st
+
+    def produce_binary_model_norm(self, continuum_interpolator, dispersion, labels1, labels2,
+                                  light_ratio):
+        """Normalised model of a binary (src/lib.rs:1013-1035): the two components weighted
+        by the linearly interpolated continua of `continuum_interpolator`."""
+        out, st = self.produce_binary_model_norm_bulk(
+            continuum_interpolator, dispersion,
+            [list(labels1) + list(labels2) + [light_ratio]])
+        if st[0]:
+            raise ValueError(f"produce_binary_model_norm: {_lib.STATUS_TEXT.get(int(st[0]))}")
+        return out[0]
+
+    def produce_binary_model_norm_bulk(self, continuum_interpolator, dispersion, labels11):
+        """Many binary models (labels P x 11) in one batched evaluation; (models, status)."""
+        b = _Binary(self, continuum_interpolator, dispersion, None, None, None)
+        return b.model(labels11)
+
+    def get_binary_fitter(self, dispersion, continuum_fitter, settings, vsini_range, rv_range):
+        """BinaryFitter (src/lib.rs:1177-1193)."""
+        return BinaryFitter(dispersion, continuum_fitter, settings, vsini_range, rv_range)
+
     def interpolate_and_convolve(self, dispersion, teff, m, logg, vsini):
         lab = f64([[teff, m, logg, vsini]])
         out = np.zeros(self.n_pix, np.float32)
@@ -650,6 +716,117 @@ class InMemSingleFitter:
         """chi2 at each label vector (src/lib.rs:1603-1627): NaN on failure unless allow_nan=False."""
         star = self._star(interpolator, observed_flux, observed_var)
         out, st, _ = star.chi2(labels)
+        if allow_nan is False and st.any():
+            raise ValueError(_lib.STATUS_TEXT.get(int(st[st != 0][0])))
+        return out.tolist()
+
+
+class _Binary:
+    """An observed binary resident in HBM (pasta_binary_*)."""
+
+    def __init__(self, interpolator, continuum_interpolator, dispersion, continuum, flux, var):
+        h = C.c_void_p()
+        n = len(dispersion.wl)
+        self.n_params = 0
+        if flux is not None:
+            flux, var = f32(flux), f32(var)
+            if len(flux) != len(var):
+                raise ValueError("Flux and variance vectors must have the same length")
+            if len(flux) != n:
+                raise ValueError("Observed spectrum and target dispersion must have the same length")
+            self.n_params = continuum.n_params(n)
+        rc = lib.pasta_binary_create(interpolator._h, continuum_interpolator._h,
+                                     dispersion.handle(interpolator),
+                                     None if continuum is None else continuum.handle(n),
+                                     None if flux is None else ptr(flux, C.c_float),
+                                     None if var is None else ptr(var, C.c_float), n, C.byref(h))
+        _raise(rc, "binary")
+        self._h, self.n = h, n
+        self._refs = (interpolator, continuum_interpolator, dispersion, continuum)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.pasta_binary_destroy(self._h)
+            self._h = None
+
+    def model(self, labels11):
+        lab = f64(labels11).reshape(-1, 11)
+        out = np.zeros((len(lab), self.n), np.float32)
+        st = np.zeros(len(lab), np.int32)
+        lib.pasta_binary_model(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_float),
+                               ptr(st, C.c_int))
+        return out, st
+
+    def chi2(self, labels11):
+        lab = f64(labels11).reshape(-1, 11)
+        out = np.zeros(len(lab))
+        st = np.zeros(len(lab), np.int32)
+        params = np.zeros((len(lab), max(1, self.n_params)), np.float32)
+        lib.pasta_binary_eval_chi2(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_double),
+                                   ptr(st, C.c_int), ptr(params, C.c_float))
+        return out, st, params
+
+
+def _binary_settings_c(settings, vsini_range, rv_range, constraints):
+    s = _lib.BinarySettings()
+    s.num_particles = settings.num_particles
+    s.max_iters = settings.max_iters
+    s.inertia_factor = settings.inertia_factor
+    s.cognitive_factor = settings.cognitive_factor
+    s.social_factor = settings.social_factor
+    s.delta = settings.delta
+    s.vsini_range[0], s.vsini_range[1] = vsini_range
+    s.rv_range[0], s.rv_range[1] = rv_range
+    s.light_ratio_range[0], s.light_ratio_range[1] = 0.0, 1.0
+    for c in constraints or []:
+        if not 0 <= c.parameter < 11:
+            raise ValueError(f"constraint parameter {c.parameter} outside the 11 binary labels")
+        s.constraint_kind[c.parameter] = c.kind
+        s.constraint_a[c.parameter] = c.a
+        s.constraint_b[c.parameter] = c.b
+    return s
+
+
+class BinaryFitter:
+    """BinaryFitter (src/fitting.rs:606-728, Python src/lib.rs:1631-1700): PSO over the
+    11 labels (star 1, star 2, light ratio) with BinaryBounds."""
+
+    def __init__(self, dispersion, continuum_fitter, settings, vsini_range, rv_range):
+        self.dispersion, self.continuum_fitter, self.settings = dispersion, continuum_fitter, settings
+        self.vsini_range, self.rv_range = tuple(vsini_range), tuple(rv_range)
+
+    def _binary(self, interpolator, continuum_interpolator, flux, var):
+        return _Binary(interpolator, continuum_interpolator, self.dispersion,
+                       self.continuum_fitter, flux, var)
+
+    def fit(self, interpolator, continuum_interpolator, observed_flux, observed_var,
+            trace_directory=None, best_particle_file=None, parallelize=None, constraints=None,
+            seed=0, init_pos=None):
+        """PSO fit of a binary (src/lib.rs:1637-1665); `seed` selects the random stream."""
+        if trace_directory is not None and best_particle_file is not None:
+            raise ValueError("Cannot specify both trace_directory and best_particle_file")
+        b = self._binary(interpolator, continuum_interpolator, observed_flux, observed_var)
+        s = _binary_settings_c(self.settings, self.vsini_range, self.rv_range, constraints)
+        res = _lib.BinaryResult()
+        trace = np.zeros((self.settings.max_iters + 1, 12))
+        keep = None if init_pos is None else f64(init_pos)
+        rc = lib.pasta_binary_fit(b._h, C.byref(s), int(seed),
+                                  None if keep is None else ptr(keep, C.c_double), C.byref(res),
+                                  ptr(trace, C.c_double))
+        _raise(rc, "fit")
+        out = BinaryOptimizationResult._from_c(res)
+        if best_particle_file is not None:
+            with open(best_particle_file, "w") as f:
+                json.dump([{"position": t[1:].tolist(), "cost": t[0]} for t in trace], f)
+        out._trace = trace
+        return out
+
+    def chi2(self, interpolator, continuum_interpolator, observed_flux, observed_var, labels,
+             allow_nan=None):
+        """chi2 at each 11-label vector (src/lib.rs:1667-1699): NaN on failure unless
+        allow_nan=False."""
+        b = self._binary(interpolator, continuum_interpolator, observed_flux, observed_var)
+        out, st, _ = b.chi2(labels)
         if allow_nan is False and st.any():
             raise ValueError(_lib.STATUS_TEXT.get(int(st[st != 0][0])))
         return out.tolist()

@@ -12,158 +12,13 @@
 // src/continuum_fitting.rs:181,223) is done as FP64 normal equations: the Gram
 // matrix AᵀA of the f32 design values is fixed per star, so its Cholesky factor
 // is computed here once and the device only forms Aᵀy per evaluation.
-#include <hip/hip_runtime.h>
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstring>
-#include <memory>
-#include <mutex>
-#include <string>
-#include <vector>
-
-#include "engine.h"
-#include "../../include/pasta_hip.h"
-
-namespace pasta {
-hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
-                       const double* labels, float* rot, int* tbound, int tb_stride,
-                       int* sort_hist, int* perm, int* pair_eq, int2* items, int* n_items,
-                       int n_model,
-                       const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
-                       const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
-                       double* cpart, double* chi2, int* status, int mode,
-                       hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st);
-hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, int n, double* pos,
-                             double* vel, const double* bpos, hipStream_t st);
-hipError_t launch_pso_best(DevPso* pso, int n_stars, const DevEval* evals, const double* pos,
-                           double* bpos, double* bcost, const double* cost, double* trace,
-                           hipStream_t st);
-hipError_t launch_pso_init(DevPso* pso, int n_stars, const DevEval* evals, double* pos,
-                           double* vel, double* bpos, double* bcost, double* cost, double* trace,
-                           hipStream_t st);
-}  // namespace pasta
-
-using namespace pasta;
+#include "host.h"
 
 static thread_local std::string g_err;
-static int fail(int code, const std::string& msg) {
+int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
-#define HIPCHK(x)                                                                         \
-    do {                                                                                  \
-        hipError_t e_ = (x);                                                              \
-        if (e_ != hipSuccess)                                                             \
-            return fail(e_ == hipErrorOutOfMemory ? PASTA_ERR_NOMEM : PASTA_ERR_HIP,      \
-                        std::string(#x) + ": " + hipGetErrorString(e_));                  \
-    } while (0)
-#define CHK(x)                  \
-    do {                        \
-        int r_ = (x);           \
-        if (r_) return r_;      \
-    } while (0)
-
-static const double LN_10 = 2.302585092994045684017991454684364208;
-static const double C_KMS = 299792.0;
-
-// ---------------------------------------------------------------------------
-// device buffer
-// ---------------------------------------------------------------------------
-template <typename T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    int alloc(size_t count) {
-        if (count <= n && p) return PASTA_OK;
-        release();
-        size_t c = count ? count : 1;
-        HIPCHK(hipMalloc((void**)&p, c * sizeof(T)));
-        n = c;
-        return PASTA_OK;
-    }
-    int upload(const T* h, size_t count) {
-        CHK(alloc(count));
-        if (count) HIPCHK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
-        return PASTA_OK;
-    }
-    int upload(const std::vector<T>& v) { return upload(v.data(), v.size()); }
-};
-
-// ---------------------------------------------------------------------------
-// handles
-// ---------------------------------------------------------------------------
-struct pasta_grid {
-    GridGeom geom{};
-    DBuf<GridGeom> d_geom;
-    DBuf<uint16_t> d_spectra;
-    DBuf<float> d_factors;
-    bool has_factors = false;
-    long long pitch = 0;
-    int W = 0;
-    double first = 0, step = 0, first_wl = 0, last_wl = 0;
-    std::mutex mu;
-    DevGrid dev() const {
-        DevGrid g;
-        g.spectra = d_spectra.p;
-        g.factors = has_factors ? d_factors.p : nullptr;
-        g.geom = d_geom.p;
-        g.pitch = pitch;
-        g.W = W;
-        g.Wq = (W / 128) * 128;  // BATCH_SIZE (src/interpolate.rs:917,969)
-        g.sg_first = first;
-        g.sg_step = step;
-        g.synth_first_wl = first_wl;
-        g.synth_last_wl = last_wl;
-        g.n_models_bins = geom.n_models + 1;
-        g.pad0 = 0;
-        return g;
-    }
-};
-
-struct pasta_disp {
-    int kind = DISP_NONE;
-    std::vector<double> wl, fidx;
-    std::vector<float> kern;  // fixed: K; variable: W x K
-    int K = 0;
-    int W = 0;
-    double first = 0, step = 0;
-};
-
-struct pasta_cont {
-    int kind = CONT_CONSTANT;
-    int n = 0;
-    int n_chunks = 0, p1 = 1, ignore = 0, n_params = 1;
-    std::vector<int> startstop;
-    std::vector<double> lohi, chol, wl;
-    std::vector<float> fac, design_q, design_rm, fixedc;
-    std::vector<int4> mblocks, cblocks;
-    std::vector<int> mb_off;
-    double chi2_div = 1.0;
-};
-
-struct Batch;
-
-struct pasta_star {
-    pasta_grid* grid = nullptr;
-    int N = 0;
-    int n_params = 1, p1 = 1, cont_kind = CONT_CONSTANT, n_chunks = 0;
-    DevStar dev{};
-    std::vector<void*> allocs;
-    std::unique_ptr<Batch> cache;
-    int cache_P = -1;
-    std::mutex mu;
-    ~pasta_star();
-};
 
 const char* pasta_last_error(void) { return g_err.c_str(); }
 int pasta_version(void) { return 100; }
@@ -723,7 +578,7 @@ pasta_star::~pasta_star() {
 }
 
 // Star without continuum (produce_model / interpolate paths)
-static int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, const float* flux,
+int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, const float* flux,
                       const float* var, int n, std::unique_ptr<pasta_star>& out) {
     auto s = std::make_unique<pasta_star>();
     s->grid = g;
@@ -819,41 +674,7 @@ int pasta_star_eval_bytes(const pasta_star* s, int64_t* bytes) {
 // ---------------------------------------------------------------------------
 // batch of evaluations
 // ---------------------------------------------------------------------------
-struct Batch {
-    int mode = 0;  // 0: resample to the observed pixels; 1: synthetic-grid output
-    int n_evals = 0, n_stars = 0;
-    std::vector<ContClass> cls;
-    pasta_grid* grid = nullptr;
-    std::vector<pasta_star*> stars;
-    std::vector<int> first_eval, count;
-    std::vector<DevEval> h_evals;
-    int64_t model_len = 0, mpart_len = 0, cpart_len = 0, pf_len = 0;
-    DBuf<DevStar> d_stars;
-    DBuf<DevEval> d_evals;
-    DBuf<double> d_labels, d_mpart, d_cpart, d_chi2;
-    DBuf<float> d_rot, d_model, d_pf;
-    DBuf<int> d_status, d_hist, d_perm, d_tbound, d_pair_eq, d_nitems;
-    DBuf<int2> d_items;
-    int tb_stride = 1;
-    DBuf<int4> d_bmom, d_bchi2;
-    DBuf<int2> d_solve;
-    int n_bm = 0, n_bmom = 0, n_solve = 0, n_bchi2 = 0;
-    // batched PSO state
-    DBuf<DevPso> d_pso;
-    DBuf<double> d_vel, d_bpos, d_bcost, d_trace;
-    hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // model begin/end, iteration begin/end
-    bool eager_events = false;  // record ev[] around eager (non-graph) launches
-    ~Batch() {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
+int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
                        const std::vector<int>& counts, int mode) {
     B.mode = mode;
     B.grid = g;
@@ -958,7 +779,7 @@ static int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& 
     return PASTA_OK;
 }
 
-static int batch_launch(Batch& B) {
+int batch_launch(Batch& B) {
     DevGrid G = B.grid->dev();
     HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p,
                        B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.d_pair_eq.p, B.d_items.p,
@@ -970,7 +791,7 @@ static int batch_launch(Batch& B) {
     return PASTA_OK;
 }
 
-static int batch_eval(Batch& B, const double* labels, double* chi2, int* status) {
+int batch_eval(Batch& B, const double* labels, double* chi2, int* status) {
     HIPCHK(hipMemcpyAsync(B.d_labels.p, labels, sizeof(double) * 5 * B.n_evals,
                           hipMemcpyHostToDevice, B.stream));
     CHK(batch_launch(B));
@@ -984,7 +805,7 @@ static int batch_eval(Batch& B, const double* labels, double* chi2, int* status)
     return PASTA_OK;
 }
 
-static const char* status_text(int st) {
+const char* status_text(int st) {
     switch (st) {
     case ST_BOUNDS: return "labels out of the grid bounds";
     case ST_NEIGHBORS: return "not enough neighbours";
@@ -1145,7 +966,7 @@ int pasta_vsini_kernel(const pasta_grid* g, double vsini, float* out, int cap, i
 // ---------------------------------------------------------------------------
 // PSO set-up shared by the plain and batched fits
 // ---------------------------------------------------------------------------
-static int settings_check(const pasta_fit_settings* st) {
+int settings_check(const pasta_fit_settings* st) {
     if (!st || st->num_particles < 1 || st->max_iters < 0)
         return fail(PASTA_ERR_ARG, "bad PSO settings");
     if (st->social_factor < 0.0) return fail(PASTA_ERR_ARG, "social factor must be >=0.");

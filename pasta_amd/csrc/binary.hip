@@ -1,0 +1,351 @@
+// binary.hip -- binary-star model, chi2 and fit behind the C ABI (include/pasta_hip.h).
+//
+// Restated from the reference (SimonVanSchuylenbergh/PASTA):
+//   interpolate_linear                         src/interpolate.rs:985-1030
+//   produce_binary_components / _model_norm    src/interpolate.rs:822-890
+//   shift_resample_and_add_binary_components   src/convolve_rv.rs:606-690
+//   BinaryCostFunction / BinaryFitter          src/fitting.rs:571-728
+//   BinaryBounds                               src/bounds.rs:188-276
+// One evaluation of P binary labels runs on one stream: the 2P component models on the
+// synthetic grid (k_prep ... k_model in synthetic-grid mode), the 2P linear continua of
+// the continuum grid and their means (k_interp_lin, k_seq_mean), the combination at the
+// observed pixels (k_binary_combine), then the continuum fit and chi2 of the P models
+// (k_moments ... k_final), with one copy back at the end.
+#include "host.h"
+
+namespace pasta {
+hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const int4* bmom,
+                       const int4* bchi2, const ContClass* cls, int n_cls,
+                       const int2* solve_items, int n_solve, const float* model, double* mpart,
+                       float* pf, double* cpart, double* chi2, int* status, hipStream_t st);
+hipError_t launch_binary(const DevGrid& G, const DevGrid& CG, const DevStar* stars,
+                         DevEval* oevals, int n, const double* labels11, const DevEval* cevals,
+                         const double* clabels, const float* comp, int* cstat, float* cont,
+                         float* cmean, int n_obs_max, float* model, hipStream_t st);
+hipError_t launch_interp_lin(const DevGrid& G, const double* labels, int lstride, int n,
+                             float* out, int* status, hipStream_t st);
+}  // namespace pasta
+
+struct pasta_binary {
+    pasta_grid* g = nullptr;
+    pasta_grid* cg = nullptr;
+    std::unique_ptr<pasta_star> obs;    // observed pixels (+ continuum, flux, var for chi2)
+    std::unique_ptr<pasta_star> synth;  // the same dispersion on the synthetic grid
+    bool has_chi2 = false;
+    int N = 0, n_params = 0, cont_kind = -1;
+    int P = -1;  // batch size the buffers below are built for
+    std::unique_ptr<Batch> Bc, Bo;  // components (2P, synthetic grid), observed (P)
+    DBuf<double> d_lab11;
+    DBuf<float> d_cont, d_cmean;
+    DBuf<int> d_cstat;
+    std::vector<double> h_comp;
+    std::mutex mu;
+};
+
+static int binary_prepare(pasta_binary* b, int P) {
+    if (b->P == P) return PASTA_OK;
+    b->P = -1;
+    b->Bc.reset(new Batch());
+    b->Bo.reset(new Batch());
+    CHK(batch_build(*b->Bc, b->g, {b->synth.get()}, {2 * P}, 1));
+    CHK(batch_build(*b->Bo, b->g, {b->obs.get()}, {P}, 0));
+    CHK(b->d_lab11.alloc((size_t)11 * P));
+    CHK(b->d_cont.alloc((size_t)2 * P * b->g->W));
+    CHK(b->d_cmean.alloc((size_t)2 * P));
+    CHK(b->d_cstat.alloc((size_t)2 * P));
+    b->h_comp.assign((size_t)10 * P, 0.0);
+    b->P = P;
+    return PASTA_OK;
+}
+
+// the whole evaluation of P binary labels, enqueued on the observed batch's stream
+static int binary_launch(pasta_binary* b, const double* labels, int P, bool chi2) {
+    CHK(binary_prepare(b, P));
+    Batch& Bc = *b->Bc;
+    Batch& Bo = *b->Bo;
+    hipStream_t st = Bo.stream;
+    for (int q = 0; q < P; q++)
+        for (int s = 0; s < 2; s++) {
+            double* c = &b->h_comp[(size_t)5 * (2 * q + s)];
+            for (int k = 0; k < 4; k++) c[k] = labels[11 * q + 5 * s + k];
+            c[4] = 0.0;  // components stay on the synthetic grid (interpolate_and_convolve)
+        }
+    HIPCHK(hipMemcpyAsync(Bc.d_labels.p, b->h_comp.data(), sizeof(double) * 10 * P,
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_lab11.p, labels, sizeof(double) * 11 * P, hipMemcpyHostToDevice,
+                          st));
+    DevGrid G = b->g->dev(), CG = b->cg->dev();
+    HIPCHK(launch_eval(G, Bc.d_stars.p, Bc.d_evals.p, Bc.n_evals, Bc.d_labels.p, Bc.d_rot.p,
+                       Bc.d_tbound.p, Bc.tb_stride, Bc.d_hist.p, Bc.d_perm.p, Bc.d_pair_eq.p,
+                       Bc.d_items.p, Bc.d_nitems.p, Bc.n_bm, Bc.d_bmom.p, Bc.d_bchi2.p,
+                       Bc.cls.data(), 0, Bc.d_solve.p, 0, Bc.d_model.p, Bc.d_mpart.p, Bc.d_pf.p,
+                       Bc.d_cpart.p, Bc.d_chi2.p, Bc.d_status.p, 1, nullptr, nullptr, st));
+    HIPCHK(launch_binary(G, CG, Bo.d_stars.p, Bo.d_evals.p, P, b->d_lab11.p, Bc.d_evals.p,
+                         Bc.d_labels.p, Bc.d_model.p, b->d_cstat.p, b->d_cont.p, b->d_cmean.p,
+                         b->N, Bo.d_model.p, st));
+    if (chi2)
+        HIPCHK(launch_cont(Bo.d_stars.p, Bo.d_evals.p, Bo.n_evals, Bo.d_bmom.p, Bo.d_bchi2.p,
+                           Bo.cls.data(), (int)Bo.cls.size(), Bo.d_solve.p, Bo.n_solve,
+                           Bo.d_model.p, Bo.d_mpart.p, Bo.d_pf.p, Bo.d_cpart.p, Bo.d_chi2.p,
+                           Bo.d_status.p, st));
+    return PASTA_OK;
+}
+
+// status of each binary evaluation (written by k_binary_combine into the observed records)
+static int binary_status(pasta_binary* b, int P, int* status) {
+    Batch& Bo = *b->Bo;
+    std::vector<DevEval> ev(P);
+    HIPCHK(hipMemcpyAsync(ev.data(), Bo.d_evals.p, sizeof(DevEval) * P, hipMemcpyDeviceToHost,
+                          Bo.stream));
+    HIPCHK(hipStreamSynchronize(Bo.stream));
+    for (int q = 0; q < P; q++) status[q] = ev[q].status;
+    return PASTA_OK;
+}
+
+int pasta_binary_create(pasta_grid* g, pasta_grid* cg, const pasta_disp* d, const pasta_cont* c,
+                        const float* flux, const float* var, int n, pasta_binary** out) {
+    if (!g || !cg || !d || !out) return fail(PASTA_ERR_ARG, "null argument");
+    if ((int)d->wl.size() != n)
+        return fail(PASTA_ERR_ARG, "Observed spectrum and target dispersion must have the same length");
+    if (d->W != g->W || d->step != g->step || d->first != g->first)
+        return fail(PASTA_ERR_ARG, "dispersion was built for another synthetic grid");
+    if (cg->W != g->W || cg->step != g->step || cg->first != g->first)
+        return fail(PASTA_ERR_ARG, "the continuum grid has other synthetic wavelengths");
+    const bool chi2 = c && flux && var;
+    if (chi2 && c->n != n)
+        return fail(PASTA_ERR_ARG, "Length of observed and synthetic spectrum are different");
+    auto b = std::make_unique<pasta_binary>();
+    b->g = g;
+    b->cg = cg;
+    b->N = n;
+    b->has_chi2 = chi2;
+    CHK(star_build(g, d, chi2 ? c : nullptr, chi2 ? flux : nullptr, chi2 ? var : nullptr, n, b->obs));
+    CHK(star_build(g, d, nullptr, nullptr, nullptr, g->W, b->synth));
+    b->synth->dev.n_tiles = (g->W + TILE - 1) / TILE;
+    b->n_params = chi2 ? c->n_params : 0;
+    b->cont_kind = chi2 ? c->kind : -1;
+    *out = b.release();
+    return PASTA_OK;
+}
+
+int pasta_binary_destroy(pasta_binary* b) {
+    delete b;
+    return PASTA_OK;
+}
+
+int pasta_binary_model(pasta_binary* b, const double* labels, int P, float* out, int* status) {
+    if (!b || !labels || P <= 0 || !out) return fail(PASTA_ERR_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(b->mu);
+    std::lock_guard<std::mutex> lg(b->g->mu);
+    CHK(binary_launch(b, labels, P, false));
+    std::vector<int> st(P);
+    CHK(binary_status(b, P, st.data()));
+    const std::vector<DevEval>& he = b->Bo->h_evals;
+    int bad = 0;
+    for (int q = 0; q < P; q++) {
+        if (status) status[q] = st[q];
+        float* o = out + (size_t)q * b->N;
+        if (st[q]) {
+            if (!bad) bad = st[q];
+            for (int i = 0; i < b->N; i++) o[i] = NAN;
+        } else {
+            HIPCHK(hipMemcpy(o, b->Bo->d_model.p + he[q].m_off, sizeof(float) * b->N,
+                             hipMemcpyDeviceToHost));
+        }
+    }
+    if (bad) return fail(PASTA_ERR_MODEL, status_text(bad));
+    return PASTA_OK;
+}
+
+int pasta_binary_eval_chi2(pasta_binary* b, const double* labels, int P, double* chi2, int* status,
+                           float* params) {
+    if (!b || !labels || P <= 0 || !chi2) return fail(PASTA_ERR_ARG, "bad arguments");
+    if (!b->has_chi2) return fail(PASTA_ERR_ARG, "binary built without an observed spectrum");
+    std::lock_guard<std::mutex> lk(b->mu);
+    std::lock_guard<std::mutex> lg(b->g->mu);
+    CHK(binary_launch(b, labels, P, true));
+    Batch& Bo = *b->Bo;
+    std::vector<int> st(P);
+    HIPCHK(hipMemcpyAsync(chi2, Bo.d_chi2.p, sizeof(double) * P, hipMemcpyDeviceToHost, Bo.stream));
+    HIPCHK(hipMemcpyAsync(st.data(), Bo.d_status.p, sizeof(int) * P, hipMemcpyDeviceToHost,
+                          Bo.stream));
+    HIPCHK(hipStreamSynchronize(Bo.stream));
+    int bad = 0;
+    const int np = b->n_params;
+    for (int q = 0; q < P; q++) {
+        if (status) status[q] = st[q];
+        if (st[q]) {
+            chi2[q] = NAN;
+            if (!bad) bad = st[q];
+        }
+    }
+    if (params && np > 0) {
+        const std::vector<DevEval>& he = Bo.h_evals;
+        for (int q = 0; q < P; q++) {
+            float* o = params + (size_t)q * np;
+            if (st[q] || b->cont_kind == CONT_FIXED) {
+                for (int k = 0; k < np; k++) o[k] = st[q] ? NAN : 0.0f;
+            } else {
+                HIPCHK(hipMemcpy(o, Bo.d_pf.p + he[q].pf_off, sizeof(float) * np,
+                                 hipMemcpyDeviceToHost));
+            }
+        }
+    }
+    if (bad) return fail(PASTA_ERR_MODEL, status_text(bad));
+    return PASTA_OK;
+}
+
+int pasta_interpolate_linear(pasta_grid* g, const double* labels, int P, float* out, int* status) {
+    if (!g || !labels || P <= 0 || !out) return fail(PASTA_ERR_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lg(g->mu);
+    DBuf<double> dl;
+    DBuf<float> dout;
+    DBuf<int> dst;
+    CHK(dl.upload(labels, (size_t)3 * P));
+    CHK(dout.alloc((size_t)P * g->W));
+    CHK(dst.alloc(P));
+    HIPCHK(launch_interp_lin(g->dev(), dl.p, 3, P, dout.p, dst.p, nullptr));
+    std::vector<int> st(P);
+    HIPCHK(hipMemcpy(st.data(), dst.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, dout.p, sizeof(float) * P * g->W, hipMemcpyDeviceToHost));
+    int bad = 0;
+    for (int q = 0; q < P; q++) {
+        if (status) status[q] = st[q];
+        if (st[q]) {
+            if (!bad) bad = st[q];
+            for (int i = 0; i < g->W; i++) out[(size_t)q * g->W + i] = NAN;
+        }
+    }
+    if (bad) return fail(PASTA_ERR_MODEL, status_text(bad));
+    return PASTA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// BinaryFitter::fit: the PSO of src/particleswarm.rs over the 11 binary parameters with
+// BinaryBounds, one batched evaluation per iteration
+// ---------------------------------------------------------------------------
+static BoundsN binary_settings_bounds(const pasta_binary_settings* st) {
+    BoundsN bn = binary_bounds(true, st->vsini_range[0], st->vsini_range[1], st->rv_range[0],
+                               st->rv_range[1], st->light_ratio_range[0], st->light_ratio_range[1]);
+    for (int d = 0; d < 11; d++) {
+        bn.ckind[d] = st->constraint_kind[d];
+        bn.ca[d] = st->constraint_a[d];
+        bn.cb[d] = st->constraint_b[d];
+    }
+    return bn;
+}
+
+// initialize_positions_and_velocities with the rejection sampler for D parameters
+// (src/particleswarm.rs:137-150, src/bounds.rs:21-35)
+static int pso_init_positions_n(const GridGeom& g, const BoundsN& b, int P, uint64_t seed,
+                                double* pos, double* vel) {
+    const int D = b.D;
+    double lo[PASTA_MAX_D], hi[PASTA_MAX_D];
+    boundsn_outer(g, b, lo, hi);
+    const uint64_t kp = rng_u64(seed, KEY_POS), kv = rng_u64(seed, KEY_VEL);
+    uint64_t attempt = 0;
+    for (int got = 0; got < P;) {
+        double p[PASTA_MAX_D];
+        for (int d = 0; d < D; d++) p[d] = rand_range(lo[d], hi[d], rng_uniform(kp, attempt * D + d));
+        attempt++;
+        if (attempt > 100000000ULL) return fail(PASTA_ERR_FIT, "cannot draw particles inside the bounds");
+        if (boundsn_within(g, b, p)) {
+            std::memcpy(pos + (size_t)D * got, p, sizeof(double) * D);
+            got++;
+        }
+    }
+    for (int q = 0; q < P; q++)
+        for (int d = 0; d < D; d++) {
+            const double delta = hi[d] - lo[d];
+            vel[(size_t)D * q + d] = rand_range(-delta, delta, rng_uniform(kv, (uint64_t)q * D + d));
+        }
+    return PASTA_OK;
+}
+
+int pasta_binary_fit(pasta_binary* b, const pasta_binary_settings* st, uint64_t seed,
+                     const double* init_pos, pasta_binary_result* out, double* trace) {
+    if (!b || !st || !out) return fail(PASTA_ERR_ARG, "null argument");
+    if (!b->has_chi2) return fail(PASTA_ERR_ARG, "binary built without an observed spectrum");
+    if (st->num_particles < 1 || st->max_iters < 0) return fail(PASTA_ERR_ARG, "bad PSO settings");
+    if (st->social_factor < 0.0) return fail(PASTA_ERR_ARG, "social factor must be >=0.");
+    if (st->delta < 0.0) return fail(PASTA_ERR_ARG, "delta must be >=0.");
+    std::memset(out, 0, sizeof(*out));
+    const auto t0 = std::chrono::steady_clock::now();
+    const GridGeom& g = b->g->geom;
+    const BoundsN bn = binary_settings_bounds(st);
+    const PsoParams ps{st->inertia_factor, st->cognitive_factor, st->social_factor};
+    const int P = st->num_particles, D = 11;
+    std::vector<double> pos((size_t)D * P), vel((size_t)D * P), bpos, cost(P), bcost;
+    std::vector<int> sts(P);
+    CHK(pso_init_positions_n(g, bn, P, seed, pos.data(), vel.data()));
+    if (init_pos) std::memcpy(pos.data(), init_pos, sizeof(double) * D * P);
+    auto evaluate = [&]() -> int {
+        int rc = pasta_binary_eval_chi2(b, pos.data(), P, cost.data(), sts.data(), nullptr);
+        if (rc == PASTA_ERR_MODEL)
+            for (int q = 0; q < P; q++)
+                if (sts[q]) {
+                    out->status = sts[q];
+                    return fail(PASTA_ERR_FIT, std::string("Error computing cost: ") + status_text(sts[q]));
+                }
+        return rc;
+    };
+    CHK(evaluate());
+    std::vector<int> order(P);
+    for (int q = 0; q < P; q++) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] < cost[c]; });
+    {
+        std::vector<double> p2(pos.size()), v2(vel.size()), c2(P);
+        for (int q = 0; q < P; q++) {
+            std::memcpy(&p2[(size_t)D * q], &pos[(size_t)D * order[q]], sizeof(double) * D);
+            std::memcpy(&v2[(size_t)D * q], &vel[(size_t)D * order[q]], sizeof(double) * D);
+            c2[q] = cost[order[q]];
+        }
+        pos.swap(p2); vel.swap(v2); cost.swap(c2);
+    }
+    bpos = pos;
+    bcost = cost;
+    double gb[11], gc = cost[0];
+    std::memcpy(gb, pos.data(), sizeof(gb));
+    if (trace) { trace[0] = gc; std::memcpy(trace + 1, gb, sizeof(gb)); }
+    const uint64_t ki = rng_u64(seed, KEY_ITER);
+    for (int it = 1; it <= st->max_iters; it++) {
+        for (int q = 0; q < P; q++) {
+            const int rc = pso_update_particle_n(g, bn, ps, ki, it, P, q, &pos[(size_t)D * q],
+                                                 &vel[(size_t)D * q], &bpos[(size_t)D * q], gb);
+            if (rc) {
+                out->status = rc;
+                return fail(PASTA_ERR_FIT, "Particle is not within bounds");
+            }
+        }
+        CHK(evaluate());
+        for (int q = 0; q < P; q++)
+            if (cost[q] < bcost[q]) {
+                std::memcpy(&bpos[(size_t)D * q], &pos[(size_t)D * q], sizeof(double) * D);
+                bcost[q] = cost[q];
+                if (cost[q] < gc) {
+                    std::memcpy(gb, &pos[(size_t)D * q], sizeof(gb));
+                    gc = cost[q];
+                }
+            }
+        if (trace) { trace[12 * it] = gc; std::memcpy(trace + 12 * it + 1, gb, sizeof(gb)); }
+    }
+    std::memcpy(out->label1, gb, sizeof(out->label1));
+    std::memcpy(out->label2, gb + 5, sizeof(out->label2));
+    out->light_ratio = gb[10];
+    out->chi2 = gc;
+    out->iterations = st->max_iters;
+    // continuum parameters of the best model (src/fitting.rs:687-696)
+    const int np = std::min(512, std::max(1, b->n_params));
+    std::vector<float> prm((size_t)std::max(1, b->n_params));
+    double c;
+    int s1;
+    int rc = pasta_binary_eval_chi2(b, gb, 1, &c, &s1, prm.data());
+    if (rc) {
+        out->status = s1;
+        return rc == PASTA_ERR_MODEL ? fail(PASTA_ERR_FIT, status_text(s1)) : rc;
+    }
+    out->n_params = b->n_params > 0 ? np : 0;
+    for (int k = 0; k < out->n_params; k++) out->continuum_params[k] = prm[k];
+    out->time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return PASTA_OK;
+}

@@ -12,6 +12,9 @@
 //   InMemFetcher::find_spectrum index            src/model_fetchers.rs:234-236
 //   bounds / clamp_1d / constraints               src/bounds.rs:9-186, src/interpolate.rs:625-669
 //   PSO velocity/position update                  src/particleswarm.rs:153-161,234-298
+//   linear neighbours + trilinear weights         src/interpolate.rs:168-212,536-620,
+//                                                 src/cubic.rs:6-10,246-262
+//   binary bounds (11 and 9 parameters)           src/bounds.rs:188-345
 #pragma once
 #include <stdint.h>
 #include <math.h>
@@ -228,6 +231,61 @@ PHD int local_weights(const GridGeom& g, double teff, double m, double logg, int
     return ST_OK;
 }
 
+// Range::find_neighbors_linear (interpolate.rs:168-212): the two bracketing indices
+PHD int find_neighbors_linear(const double* v, int n, double x, int left, int right, int out[2]) {
+    int r = right_index(v, n, x);
+    if (r >= 0) {
+        if (r < left || r > right) return ST_BOUNDS;
+        if (r == right) { out[0] = r - 1; out[1] = r; }
+        else { out[0] = r; out[1] = r + 1; }
+        return ST_OK;
+    }
+    int i = -r - 1;
+    if (i <= left || i > right) return ST_BOUNDS;
+    out[0] = i - 1;
+    out[1] = i;
+    return ST_OK;
+}
+
+// get_local_grid_linear + calculate_interpolation_coefficients_linear + the neighbour
+// order of interpolate_linear (interpolate.rs:536-620,985-995, cubic.rs:6-10,246-262):
+// q = (a*2 + b)*2 + c for teff neighbour a, logg neighbour b, [M/H] neighbour c; weight
+// f32((ft[a] * fl[b]) * fm[c]).  Every neighbour is fetched with the storage index
+// formula (model_fetchers.rs:234-236) whether or not it lies inside its Teff column's
+// logg range, as the reference does; an index the reference could not fetch (logg below
+// the column's first, or past the last model) is ST_INDEX.
+PHD int local_weights_linear(const GridGeom& g, double teff, double m, double logg, int idx[8],
+                             float w[8]) {
+    if (!teff_logg_between_bounds(g, teff, logg)) return ST_BOUNDS;
+    if (!m_between_bounds(g, m)) return ST_BOUNDS;
+    int tlo, thi, llo, lhi, rc;
+    if ((rc = teff_index_limits_at(g, logg, &tlo, &thi))) return rc;
+    if ((rc = logg_index_limits_at(g, teff, &llo, &lhi))) return rc;
+    int tn[2], mn[2], ln[2];
+    if ((rc = find_neighbors_linear(g.teff, g.n_teff, teff, tlo, thi, tn))) return rc;
+    if ((rc = find_neighbors_linear(g.m, g.n_m, m, 0, g.n_m - 1, mn))) return rc;
+    if ((rc = find_neighbors_linear(g.logg, g.n_logg, logg, llo, lhi, ln))) return rc;
+    double ft[2], fl[2], fm[2];
+    ft[0] = (teff - g.teff[tn[1]]) / (g.teff[tn[0]] - g.teff[tn[1]]);
+    ft[1] = (teff - g.teff[tn[0]]) / (g.teff[tn[1]] - g.teff[tn[0]]);
+    fl[0] = (logg - g.logg[ln[1]]) / (g.logg[ln[0]] - g.logg[ln[1]]);
+    fl[1] = (logg - g.logg[ln[0]]) / (g.logg[ln[1]] - g.logg[ln[0]]);
+    fm[0] = (m - g.m[mn[1]]) / (g.m[mn[0]] - g.m[mn[1]]);
+    fm[1] = (m - g.m[mn[0]]) / (g.m[mn[1]] - g.m[mn[0]]);
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int c = 0; c < 2; c++) {
+                const int q = (a * 2 + b) * 2 + c;
+                const int i = tn[a], k = ln[b];
+                if (k < g.logg_lo[i]) return ST_INDEX;
+                const int id = model_index(g, i, mn[c], k);
+                if (id >= g.n_models) return ST_INDEX;
+                idx[q] = id;
+                w[q] = (float)((ft[a] * fl[b]) * fm[c]);
+            }
+    return ST_OK;
+}
+
 // ---------------- bounds (bounds.rs:9-186) ----------------
 struct Bounds {
     double vsini_lo, vsini_hi, rv_lo, rv_hi;
@@ -327,6 +385,120 @@ PHD int pso_update_particle(const GridGeom& g, const Bounds& b, const PsoParams&
         if (rc) return rc;
         pos[d] = cl;
         if (!bounds_within(g, b, pos)) return ST_PSO;
+    }
+    return ST_OK;
+}
+
+// ---------------- bounds of the binary searches (bounds.rs:188-345) ----------------
+// A D-parameter search over one or two stars: parameter d is a grid axis (teff, [M/H],
+// logg) of the star whose Teff sits at base[d], or a plain range [lo, hi] (vsini, RV,
+// light ratio); constraints as in ConstrainedPSOBounds (bounds.rs:64-132).
+#define PASTA_MAX_D 11
+enum BoundKind : int { BK_TEFF = 0, BK_M = 1, BK_LOGG = 2, BK_RANGE = 3 };
+struct BoundsN {
+    int D;
+    int kind[PASTA_MAX_D];
+    int base[PASTA_MAX_D];
+    double lo[PASTA_MAX_D], hi[PASTA_MAX_D];
+    int ckind[PASTA_MAX_D];  // 0 none, 1 fixed, 2 range
+    double ca[PASTA_MAX_D], cb[PASTA_MAX_D];
+};
+
+// BinaryBounds (11: star 1, star 2, light ratio) or BinaryBoundsWithoutRV (9)
+PHD BoundsN binary_bounds(bool with_rv, double vs_lo, double vs_hi, double rv_lo, double rv_hi,
+                          double lr_lo, double lr_hi) {
+    BoundsN b{};
+    const int per = with_rv ? 5 : 4;
+    b.D = 2 * per + 1;
+    for (int s = 0; s < 2; s++) {
+        const int o = s * per;
+        b.kind[o] = BK_TEFF; b.kind[o + 1] = BK_M; b.kind[o + 2] = BK_LOGG;
+        b.base[o] = b.base[o + 1] = b.base[o + 2] = o;
+        b.kind[o + 3] = BK_RANGE; b.lo[o + 3] = vs_lo; b.hi[o + 3] = vs_hi;
+        if (with_rv) { b.kind[o + 4] = BK_RANGE; b.lo[o + 4] = rv_lo; b.hi[o + 4] = rv_hi; }
+    }
+    b.kind[b.D - 1] = BK_RANGE;
+    b.lo[b.D - 1] = lr_lo;
+    b.hi[b.D - 1] = lr_hi;
+    return b;
+}
+
+PHD void boundsn_outer(const GridGeom& g, const BoundsN& b, double* lo, double* hi) {
+    for (int d = 0; d < b.D; d++) {
+        switch (b.kind[d]) {
+        case BK_TEFF: lo[d] = g.teff[0]; hi[d] = g.teff[g.n_teff - 1]; break;
+        case BK_M: lo[d] = g.m[0]; hi[d] = g.m[g.n_m - 1]; break;
+        case BK_LOGG: lo[d] = g.logg[0]; hi[d] = g.logg[g.n_logg - 1]; break;
+        default: lo[d] = b.lo[d]; hi[d] = b.hi[d];
+        }
+        if (b.ckind[d] == 1) { lo[d] = b.ca[d]; hi[d] = b.ca[d]; }
+        else if (b.ckind[d] == 2) { lo[d] = fmax(lo[d], b.ca[d]); hi[d] = fmin(hi[d], b.cb[d]); }
+    }
+}
+
+PHD bool boundsn_within(const GridGeom& g, const BoundsN& b, const double* p) {
+    for (int d = 0; d < b.D; d++)
+        if (b.kind[d] == BK_TEFF) {
+            const int o = b.base[d];
+            if (!(teff_logg_between_bounds(g, p[o], p[o + 2]) && m_between_bounds(g, p[o + 1])))
+                return false;
+        }
+    for (int d = 0; d < b.D; d++)
+        if (b.kind[d] == BK_RANGE && !(p[d] >= b.lo[d] && p[d] <= b.hi[d])) return false;
+    for (int d = 0; d < b.D; d++) {
+        if (b.ckind[d] == 1 && !(fabs(p[d] - b.ca[d]) <= 2.220446049250313e-16)) return false;
+        if (b.ckind[d] == 2 && !(p[d] >= b.ca[d] && p[d] <= b.cb[d])) return false;
+    }
+    return true;
+}
+
+PHD int boundsn_clamp(const GridGeom& g, const BoundsN& b, const double* p, int d, double* out) {
+    double lo, hi;
+    int a, c;
+    const int o = b.base[d];
+    switch (b.kind[d]) {
+    case BK_TEFF:
+        if (teff_index_limits_at(g, p[o + 2], &a, &c)) return ST_PSO;
+        lo = g.teff[a]; hi = g.teff[c];
+        break;
+    case BK_M: lo = g.m[0]; hi = g.m[g.n_m - 1]; break;
+    case BK_LOGG:
+        if (logg_index_limits_at(g, p[o], &a, &c)) return ST_PSO;
+        lo = g.logg[a]; hi = g.logg[c];
+        break;
+    default: lo = b.lo[d]; hi = b.hi[d];
+    }
+    if (b.ckind[d] == 1) { lo = b.ca[d]; hi = b.ca[d]; }
+    else if (b.ckind[d] == 2) { lo = fmax(lo, b.ca[d]); hi = fmin(hi, b.cb[d]); }
+    if (lo > hi) return ST_PSO;
+    double x = p[d];
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    *out = x;
+    return ST_OK;
+}
+
+// pso_update_particle for D parameters (the same draws for D = 5)
+PHD int pso_update_particle_n(const GridGeom& g, const BoundsN& b, const PsoParams& s,
+                              uint64_t key_iter, int it, int P, int q, double* pos, double* vel,
+                              const double* bpos, const double* gbest) {
+    const int D = b.D;
+    for (int d = 0; d < D; d++) {
+        uint64_t c0 = (((uint64_t)it * (uint64_t)P + (uint64_t)q) * (uint64_t)D + (uint64_t)d) * 2;
+        double momentum = vel[d] * s.inertia;
+        double to_opt = bpos[d] - pos[d];
+        double pull = rand_range(0.0, to_opt, rng_uniform(key_iter, c0)) * s.cognitive;
+        double to_g = gbest[d] - pos[d];
+        double pullg = rand_range(0.0, to_g, rng_uniform(key_iter, c0 + 1)) * s.social;
+        vel[d] = momentum + pull + pullg;
+        double np[PASTA_MAX_D];
+        for (int e = 0; e < D; e++) np[e] = pos[e];
+        np[d] += vel[d];
+        double cl;
+        int rc = boundsn_clamp(g, b, np, d, &cl);
+        if (rc) return rc;
+        pos[d] = cl;
+        if (!boundsn_within(g, b, pos)) return ST_PSO;
     }
     return ST_OK;
 }

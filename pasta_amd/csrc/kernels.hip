@@ -1502,6 +1502,11 @@ __global__ __launch_bounds__(1024) void k_pair_items(const int* __restrict__ per
     if (tid == 0) *n_items = s_carry[2];
 }
 
+hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const int4* bmom,
+                       const int4* bchi2, const ContClass* cls, int n_cls,
+                       const int2* solve_items, int n_solve, const float* model, double* mpart,
+                       float* pf, double* cpart, double* chi2, int* status, hipStream_t st);
+
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
                        int* sort_hist, int* perm, int* pair_eq, int2* items, int* n_items,
@@ -1529,6 +1534,16 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
         hipLaunchKernelGGL(k_model, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot, items,
                            n_items, n_evals, tbound, tb_stride, model, mode);
     if (ev_model1) (void)hipEventRecord(ev_model1, st);
+    return launch_cont(stars, evals, n_evals, bmom, bchi2, cls, n_cls, solve_items, n_solve,
+                       model, mpart, pf, cpart, chi2, status, st);
+}
+
+// continuum fit + chi2 of evaluations whose models are in `model` and whose status is set
+hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const int4* bmom,
+                       const int4* bchi2, const ContClass* cls, int n_cls,
+                       const int2* solve_items, int n_solve, const float* model, double* mpart,
+                       float* pf, double* cpart, double* chi2, int* status, hipStream_t st) {
+    if (n_evals <= 0) return hipSuccess;
     for (int c = 0; c < n_cls; c++) {
         const ContClass& K = cls[c];
         if (K.n_mom <= 0) continue;
@@ -1578,6 +1593,151 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
     }
     hipLaunchKernelGGL(k_final, dim3((n_evals + 127) / 128), dim3(128), 0, st, stars, evals,
                        n_evals, cpart, chi2, status);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// binary stars
+// ---------------------------------------------------------------------------
+
+// interpolate_linear (src/interpolate.rs:985-1030) of evaluation e = blockIdx.y at the
+// labels labels[e * lstride .. +3): every pixel is x / 65535 * factor, then the gemv
+// over the 8 neighbours in column order, rounded exactly as the reference's f32 steps;
+// block x covers pixels [8 NT x, 8 NT (x + 1)).  status[e]: 0 or the weights' failure.
+__global__ __launch_bounds__(NT) void k_interp_lin(DevGrid G, const double* __restrict__ labels,
+                                                   int lstride, float* __restrict__ out,
+                                                   int* __restrict__ status) {
+    __shared__ int s_idx[8];
+    __shared__ float s_w[8], s_f[8];
+    __shared__ int s_st;
+    const int e = blockIdx.y;
+    if (threadIdx.x == 0) {
+        const double* l = labels + (size_t)e * lstride;
+        int idx[8];
+        float w[8];
+        const int st = local_weights_linear(*G.geom, l[0], l[1], l[2], idx, w);
+        s_st = st;
+        for (int j = 0; j < 8; j++) {
+            const int id = st ? 0 : idx[j];
+            s_idx[j] = id;
+            s_w[j] = st ? 0.0f : w[j];
+            s_f[j] = G.factors ? G.factors[id] : (float)(1.0 / 65535.0);
+        }
+        if (blockIdx.x == 0) status[e] = st;
+    }
+    __syncthreads();
+    if (s_st) return;
+    const long long x0 = (long long)blockIdx.x * NT * 8 + threadIdx.x;
+    float* o = out + (size_t)e * G.W;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const long long x = x0 + (long long)k * NT;
+        if (x >= G.W) break;
+        float y = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const float u = (float)G.spectra[(size_t)s_idx[j] * G.pitch + x];
+            const float t = (u / 65535.0f * s_f[j]) * s_w[j];
+            y = (j == 0) ? t : t + y;
+        }
+        o[x] = y;
+    }
+}
+
+// DVector::mean of each row of `in` (W values): a sequential f32 fold in pixel order and
+// one division; one wavefront per row stages 1024 values at a time in LDS and lane 0
+// adds them in order.  Rows with a non-zero status are skipped.
+__global__ __launch_bounds__(64) void k_seq_mean(const float* __restrict__ in, int W,
+                                                 const int* __restrict__ status,
+                                                 float* __restrict__ mean) {
+    __shared__ float buf[1024];
+    const int e = blockIdx.x;
+    if (status[e]) return;
+    const float* r = in + (size_t)e * W;
+    float s = 0.0f;
+    for (int x0 = 0; x0 < W; x0 += 1024) {
+        for (int k = threadIdx.x; k < 1024; k += 64) buf[k] = x0 + k < W ? r[x0 + k] : 0.0f;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = W - x0 < 1024 ? W - x0 : 1024;
+            for (int k = 0; k < m; k++) s = s + buf[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mean[e] = s / (float)W;
+}
+
+// shift_resample_and_add_binary_components (src/convolve_rv.rs:606-690, Logspace grid)
+// for binary evaluation b = blockIdx.y: components 2b (star 1) and 2b+1 (star 2) are the
+// convolved models `comp` (synthetic grid, W per component, status in cevals), their
+// linear continua `cont` (status cstat) and the continua's means.  Writes the normalised
+// model at obs evaluation b's m_off and its status: the first failure in the reference's
+// order (model 1, continuum 1, model 2, continuum 2, the wavelength range of both shifts,
+// then a resample index past the continuum's end, checked at the first and last pixel of
+// the increasing observed wavelengths).
+__global__ __launch_bounds__(NT) void k_binary_combine(DevGrid G, const DevStar* __restrict__ stars,
+                                                       DevEval* __restrict__ oevals,
+                                                       const double* __restrict__ labels11,
+                                                       const DevEval* __restrict__ cevals,
+                                                       const float* __restrict__ comp,
+                                                       const int* __restrict__ cstat,
+                                                       const float* __restrict__ cont,
+                                                       const float* __restrict__ cmean,
+                                                       float* __restrict__ model) {
+    const int b = blockIdx.y;
+    const DevStar& S = stars[oevals[b].star];
+    const double* l = labels11 + (size_t)11 * b;
+    int st = cevals[2 * b].status;
+    if (!st) st = cstat[2 * b];
+    if (!st) st = cevals[2 * b + 1].status;
+    if (!st) st = cstat[2 * b + 1];
+    const double sf1 = 1.0 - l[4] / C_KMS, sf2 = 1.0 - l[9] / C_KMS;
+    if (!st && (S.wl0 * fmin(sf1, sf2) < G.synth_first_wl || S.wlN * fmax(sf1, sf2) > G.synth_last_wl))
+        st = ST_RANGE;
+    const double sl1 = log10(sf1) / G.sg_step, sl2 = log10(sf2) / G.sg_step;
+    if (!st) {
+        const double f0 = S.fidx[0], f1 = S.fidx[S.N - 1];
+        if (jfloor(f0 + sl1) + 1 >= G.W || jfloor(f1 + sl1) + 1 >= G.W ||
+            jfloor(f0 + sl2) + 1 >= G.W || jfloor(f1 + sl2) + 1 >= G.W)
+            st = ST_INDEX;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) oevals[b].status = st;
+    if (st) return;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= S.N) return;
+    const float lr = (float)l[10] * cmean[2 * b] / cmean[2 * b + 1];
+    const float* n1 = comp + (size_t)(2 * b) * G.W;
+    const float* n2 = n1 + G.W;
+    const float* c1 = cont + (size_t)(2 * b) * G.W;
+    const float* c2 = c1 + G.W;
+    const double x1 = S.fidx[i] + sl1, x2 = S.fidx[i] + sl2;
+    const long long j1 = jfloor(x1), j2 = jfloor(x2);
+    const float w1 = (float)x1 - (float)j1, w2 = (float)x2 - (float)j2;
+    const float a1 = (1.0f - w1) * n1[j1] + w1 * n1[j1 + 1];
+    const float k1 = (1.0f - w1) * c1[j1] + w1 * c1[j1 + 1];
+    const float a2 = (1.0f - w2) * n2[j2] + w2 * n2[j2 + 1];
+    const float k2 = (1.0f - w2) * c2[j2] + w2 * c2[j2 + 1];
+    model[oevals[b].m_off + i] = (a1 * k1 + lr * a2 * k2) / (k1 + lr * k2);
+}
+
+hipError_t launch_binary(const DevGrid& G, const DevGrid& CG, const DevStar* stars,
+                         DevEval* oevals, int n, const double* labels11, const DevEval* cevals,
+                         const double* clabels, const float* comp, int* cstat, float* cont,
+                         float* cmean, int n_obs_max, float* model, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_interp_lin, dim3((G.W + 8 * NT - 1) / (8 * NT), 2 * n), dim3(NT), 0, st, CG,
+                       clabels, 5, cont, cstat);
+    hipLaunchKernelGGL(k_seq_mean, dim3(2 * n), dim3(64), 0, st, cont, G.W, cstat, cmean);
+    hipLaunchKernelGGL(k_binary_combine, dim3((n_obs_max + NT - 1) / NT, n), dim3(NT), 0, st, G,
+                       stars, oevals, labels11, cevals, comp, cstat, cont, cmean, model);
+    return hipGetLastError();
+}
+
+hipError_t launch_interp_lin(const DevGrid& G, const double* labels, int lstride, int n,
+                             float* out, int* status, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_interp_lin, dim3((G.W + 8 * NT - 1) / (8 * NT), n), dim3(NT), 0, st, G,
+                       labels, lstride, out, status);
     return hipGetLastError();
 }
 

@@ -104,6 +104,25 @@ def make_grid(spec, labels=None, includes_factor_seed=None):
     return u16, factors.astype(np.float32)
 
 
+def make_continuum_grid(spec, labels=None):
+    """Unnormalised continuum grid for binary models: the Planck curve
+    B(lambda, Teff) ~ lambda^-5 / (exp(hc / (lambda k Teff)) - 1) times 10^(0.2 [M/H]) and
+    (1 + 0.05 (logg - 4)), stored like an includes_factor grid (uint16 flux / max and the
+    f32 factor max / 65535 per model)."""
+    if labels is None:
+        labels = spec.labels()
+    wl = spec.synth_wl()
+    hc_k = 1.4387769e8  # Angstrom K
+    t = labels[:, 0:1]
+    b = wl[None, :] ** -5.0 / np.expm1(hc_k / (wl[None, :] * t))
+    b = b * 10.0 ** (0.2 * labels[:, 1:2]) * (1.0 + 0.05 * (labels[:, 2:3] - 4.0))
+    b = b / b.max() * 1e6
+    bmax = b.max(axis=1)
+    u16 = (b / bmax[:, None] * 65535.0).astype(np.uint16)
+    factors = (bmax.astype(np.float32) / np.float32(65535.0)).astype(np.float32)
+    return u16, factors
+
+
 def observed_wavelengths(spec, n_obs, margin_px):
     """Log-uniform observed sampling kept margin_px synthetic pixels inside each edge."""
     lo = spec.log10_first + spec.log10_step * margin_px

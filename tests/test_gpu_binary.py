@@ -1,0 +1,131 @@
+"""Binary stars on the GPU against the oracle: interpolate_linear bit-exact
+(src/interpolate.rs:985-1030), produce_binary_model_norm (src/interpolate.rs:868-890,
+src/convolve_rv.rs:606-690) and BinaryCostFunction::cost (src/fitting.rs:581-604)
+within 1e-5, failures as NaN + status, and BinaryFitter::fit (src/fitting.rs:635-706):
+the host PSO fed by the GPU chi2 makes the same decisions as the oracle's 11-parameter
+PSO fed by the same chi2 (bit-identical traces)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+L1 = [6123.0, -0.3, 4.21, 10.0, 20.0]
+L2 = [5400.0, 0.1, 4.6, 30.0, -40.0]
+
+
+@pytest.fixture(scope="module")
+def binary(small, gpu_small):
+    from pasta_amd import synth
+    pasta, wlg, interp = gpu_small
+    cs, cf = synth.make_continuum_grid(small.spec)
+    cgrid = O.Grid(small.labels, cs, cf, small.spec.log10_first, small.spec.log10_step)
+    cinterp = pasta.InMemInterpolator.from_arrays(cs, small.labels, wlg, factors=cf)
+    truth = np.array(L1 + L2 + [0.6])
+    m = O.binary_model_norm(small.og, cgrid, small.odisp, truth[:5], truth[5:10], np.float32(0.6))
+    flux, var = synth.observe(m, small.wl, 150.0, 77)
+    d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
+    return dict(pasta=pasta, interp=interp, cinterp=cinterp, cgrid=cgrid, d=d, flux=flux,
+                var=var, truth=truth, cs=cs, cf=cf)
+
+
+def _labels(spec, n, seed):
+    from pasta_amd import synth
+    a = synth.random_labels(spec, n, seed, margin=100.0)
+    b = synth.random_labels(spec, n, seed + 1, margin=100.0)
+    lr = np.random.default_rng(seed).uniform(0.05, 0.95, n)
+    return np.concatenate([a, b, lr[:, None]], axis=1)
+
+
+def test_interpolate_linear_bit_exact(small, binary):
+    labs = np.array([[6123.0, -0.3, 4.21], [5000.0, -1.0, 3.5], [7000.0, 0.5, 5.0],
+                     [5625.0, 0.05, 4.0], [6480.0, -0.77, 3.9]])
+    out, st = binary["cinterp"].interpolate_linear_bulk(labs)
+    assert not st.any()
+    for k, lab in enumerate(labs):
+        ref = binary["cgrid"].interpolate_linear(*lab)
+        assert np.array_equal(out[k], ref), (lab.tolist(), np.abs(out[k] - ref).max())
+    out, st = binary["cinterp"].interpolate_linear_bulk([[4000.0, 0.0, 4.0], [6000.0, 0.0, 4.0]])
+    assert st[0] == 1 and st[1] == 0 and np.isnan(out[0]).all()
+
+
+def test_binary_model_matches_oracle(small, binary):
+    labs = np.concatenate([[binary["truth"]], _labels(small.spec, 12, 21)])
+    out, st = binary["interp"].produce_binary_model_norm_bulk(binary["cinterp"], binary["d"], labs)
+    assert not st.any()
+    worst = 0.0
+    for k, lab in enumerate(labs):
+        ref = O.binary_model_norm(small.og, binary["cgrid"], small.odisp, lab[:5], lab[5:10],
+                                  np.float32(lab[10]))
+        worst = max(worst, float(np.abs(out[k] - ref).max() / np.abs(ref).max()))
+    print(f"binary model: max err / max {worst:.2e} over {len(labs)} labels", flush=True)
+    assert worst < RTOL
+    one = binary["interp"].produce_binary_model_norm(binary["cinterp"], binary["d"], L1, L2, 0.6)
+    assert np.array_equal(one, out[0])
+
+
+def test_binary_chi2_matches_oracle(small, binary):
+    pasta = binary["pasta"]
+    fitter = binary["interp"].get_binary_fitter(binary["d"], small_cont(pasta, small),
+                                                pasta.PSOSettings(8, 2), (0, 300), (-150, 150))
+    labs = np.concatenate([[binary["truth"]], _labels(small.spec, 20, 31)])
+    bad = [L1 + [3000.0, 0.0, 4.0, 10.0, 0.0] + [0.5], L1 + L2[:4] + [9000.0] + [0.5],
+           L1[:3] + [-5.0, 0.0] + L2 + [0.5]]
+    allv = np.concatenate([labs, bad])
+    chi = np.array(fitter.chi2(binary["interp"], binary["cinterp"], binary["flux"], binary["var"], allv))
+    assert np.isnan(chi[len(labs):]).all() and np.isfinite(chi[: len(labs)]).all()
+    worst = 0.0
+    for k, lab in enumerate(labs):
+        ref, _ = O.binary_cost(small.og, binary["cgrid"], small.odisp, small.ocont, binary["flux"],
+                               binary["var"], lab)
+        worst = max(worst, abs(chi[k] - ref) / ref)
+    print(f"binary chi2: max rel {worst:.2e} over {len(labs)} labels", flush=True)
+    assert worst < RTOL
+    with pytest.raises(ValueError):
+        fitter.chi2(binary["interp"], binary["cinterp"], binary["flux"], binary["var"], bad,
+                    allow_nan=False)
+
+
+def small_cont(pasta, small):
+    return pasta.ChunkContinuumFitter(small.wl, 5, 4, 0.2)
+
+
+def test_binary_fit_identity_with_oracle_pso(small, binary):
+    pasta = binary["pasta"]
+    P, iters, seed = 24, 12, 5
+    fitter = binary["interp"].get_binary_fitter(binary["d"], small_cont(pasta, small),
+                                                pasta.PSOSettings(P, iters), (0, 300), (-150, 150))
+    res = fitter.fit(binary["interp"], binary["cinterp"], binary["flux"], binary["var"], seed=seed)
+    b = fitter._binary(binary["interp"], binary["cinterp"], binary["flux"], binary["var"])
+
+    def gpu_cost(pos):
+        c, st, _ = b.chi2(pos)
+        assert not st.any()
+        return c
+
+    ob = O.binary_bounds(small.og, (0.0, 300.0), (-150.0, 150.0))
+    best, bc, trace, ne = O.pso_run_binary(ob, O.pso_settings(P, iters), seed, gpu_cost)
+    print(f"binary fit: chi2 {res.chi2:.6f}, oracle PSO {bc:.6f}", flush=True)
+    assert np.array_equal(res._trace, trace)
+    assert res.chi2 == bc
+    assert res.label1.as_list() + res.label2.as_list() + [res.light_ratio] == best.tolist()
+    assert len(res.continuum_params) == 25
+
+
+def test_binary_fit_recovers(small, binary):
+    pasta = binary["pasta"]
+    fitter = binary["interp"].get_binary_fitter(binary["d"], small_cont(pasta, small),
+                                                pasta.PSOSettings(60, 60), (0, 300), (-150, 150))
+    cons = [pasta.FixConstraint(10, 0.6), pasta.FixConstraint(3, 10.0), pasta.FixConstraint(8, 30.0)]
+    res = fitter.fit(binary["interp"], binary["cinterp"], binary["flux"], binary["var"], seed=2,
+                     constraints=cons)
+    c_truth = fitter.chi2(binary["interp"], binary["cinterp"], binary["flux"], binary["var"],
+                          [binary["truth"]])[0]
+    print(f"binary fit: {res.label1.as_list()} {res.label2.as_list()} chi2 {res.chi2:.5f} "
+          f"(truth {c_truth:.5f})", flush=True)
+    assert res.light_ratio == 0.6
+    assert res.chi2 <= c_truth * (1 + 2e-3)
+    d = res.to_dict()
+    assert pasta.BinaryOptimizationResult.from_dict(d).to_dict() == d

@@ -1,0 +1,184 @@
+"""The oracle's binary-star restatement, pinned without the reference:
+interpolate_linear (src/interpolate.rs:985-1030, src/cubic.rs:246-262) reproduces
+functions trilinear in the labels exactly and is one-hot at nodes;
+produce_binary_model_norm (src/interpolate.rs:868-890, src/convolve_rv.rs:606-690)
+collapses to produce_model for identical stars or a zero light ratio and matches an FP64
+numpy composition (np.interp resampling of the separately pinned components);
+BinaryBounds (src/bounds.rs:188-276) and the 11-parameter PSO keep every particle in
+bounds, never raise the global best and converge on a bowl."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pasta_amd import synth
+
+
+def _trilinear_grid(spec):
+    """uint16 rows whose value is trilinear in the grid indices (exact in f64)."""
+    labels = spec.labels()
+    t = np.searchsorted(spec.teff, labels[:, 0])
+    m = np.searchsorted(spec.m, labels[:, 1])
+    g = np.searchsorted(spec.logg, labels[:, 2])
+    x = np.arange(spec.n_pix) % 7
+    v = (1000 + 300 * t[:, None] + 50 * m[:, None] + 20 * g[:, None] + 4 * (t * m)[:, None]
+         + 3 * (m * g)[:, None] + 2 * (t * g)[:, None] + (t * m * g)[:, None] + x[None, :])
+    return labels, v.astype(np.uint16)
+
+
+def test_linear_weights_node_and_sum(small):
+    g = small.og
+    teff, m, logg = g.axes()[0], g.axes()[1], g.axes()[2]
+    idx, w32, w64 = g.local_weights_linear(teff[3], m[2], logg[1])
+    assert np.isclose(w64.sum(), 1.0, atol=1e-14)
+    assert np.count_nonzero(w64) == 1
+    node = idx[np.argmax(w64)]
+    assert np.allclose(g.list_gridpoints()[node], [teff[3], m[2], logg[1]])
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        lab = [rng.uniform(teff[0], teff[-1]), rng.uniform(m[0], m[-1]), rng.uniform(logg[0], logg[-1])]
+        idx, w32, w64 = g.local_weights_linear(*lab)
+        assert abs(w64.sum() - 1.0) < 1e-12
+        pts = g.list_gridpoints()[idx]
+        # the 8 neighbours are the bracketing corners, teff x logg x [M/H] order
+        assert np.all(pts[:4, 0] <= lab[0]) and np.all(pts[4:, 0] >= lab[0])
+        assert np.all(pts[[0, 1, 4, 5], 2] <= lab[2]) and np.all(pts[[2, 3, 6, 7], 2] >= lab[2])
+        assert np.all(pts[0::2, 1] <= lab[1]) and np.all(pts[1::2, 1] >= lab[1])
+        # the weight interpolates every label itself
+        assert np.allclose(w64 @ pts, lab, rtol=0, atol=1e-9)
+
+
+def test_linear_weights_out_of_bounds(small):
+    with pytest.raises(O.OracleError):
+        small.og.local_weights_linear(4000.0, 0.0, 4.0)
+    with pytest.raises(O.OracleError):
+        small.og.local_weights_linear(6000.0, 2.0, 4.0)
+
+
+def test_interpolate_linear_trilinear_exact(small):
+    labels, v = _trilinear_grid(small.spec)
+    fac = np.full(len(labels), 2.5, np.float32)
+    g = O.Grid(labels, v, fac, small.spec.log10_first, small.spec.log10_step)
+    rng = np.random.default_rng(5)
+    sp = small.spec
+    for _ in range(6):
+        t = rng.uniform(sp.teff[0], sp.teff[-1]); m = rng.uniform(sp.m[0], sp.m[-1])
+        lg = rng.uniform(sp.logg[0], sp.logg[-1])
+        # fractional grid coordinates
+        ft = np.interp(t, sp.teff, np.arange(len(sp.teff)))
+        fm = np.interp(m, sp.m, np.arange(len(sp.m)))
+        fg = np.interp(lg, sp.logg, np.arange(len(sp.logg)))
+        x = np.arange(sp.n_pix) % 7
+        exact = (1000 + 300 * ft + 50 * fm + 20 * fg + 4 * ft * fm + 3 * fm * fg + 2 * ft * fg
+                 + ft * fm * fg + x) / 65535.0 * 2.5
+        y64 = g.interpolate_linear(t, m, lg, f64=True)
+        y32 = g.interpolate_linear(t, m, lg)
+        np.testing.assert_allclose(y64, exact, rtol=1e-12)
+        np.testing.assert_allclose(y32, exact, rtol=1e-6)
+
+
+@pytest.fixture(scope="module")
+def cgrid(small):
+    cs, cf = synth.make_continuum_grid(small.spec)
+    return O.Grid(small.labels, cs, cf, small.spec.log10_first, small.spec.log10_step)
+
+
+L1 = np.array([6123.0, -0.3, 4.21, 10.0, 20.0])
+L2 = np.array([5400.0, 0.1, 4.6, 30.0, -40.0])
+
+
+def test_binary_reduces_to_single(small, cgrid):
+    m1 = O.produce_model(small.og, small.odisp, L1)
+    for lr in (0.0, 0.3, 1.0):
+        mm = O.binary_model_norm(small.og, cgrid, small.odisp, L1, L1, lr)
+        assert np.abs(mm - m1).max() / np.abs(m1).max() < 4e-7, lr
+    m0 = O.binary_model_norm(small.og, cgrid, small.odisp, L1, L2, 0.0)
+    assert np.abs(m0 - m1).max() / np.abs(m1).max() < 2e-7
+
+
+def test_binary_matches_fp64_composition(small, cgrid):
+    """Components from interpolate_and_convolve (the synthetic-grid segment) and the FP64
+    linear continuum, resampled with np.interp at fidx + log10(1 - rv/c)/step."""
+    sp = small.spec
+    n1 = O.interpolate_and_convolve(small.og, small.odisp, L1[:4]).astype(np.float64)
+    n2 = O.interpolate_and_convolve(small.og, small.odisp, L2[:4]).astype(np.float64)
+    c1 = cgrid.interpolate_linear(*L1[:3], f64=True)
+    c2 = cgrid.interpolate_linear(*L2[:3], f64=True)
+    lrin = 0.6
+    lr = lrin * c1.mean() / c2.mean()
+    fidx = (np.log10(small.wl) - sp.log10_first) / sp.log10_step
+    ax = np.arange(sp.n_pix)
+    x1 = fidx + np.log10(1 - L1[4] / 299792.0) / sp.log10_step
+    x2 = fidx + np.log10(1 - L2[4] / 299792.0) / sp.log10_step
+    a1, k1 = np.interp(x1, ax, n1), np.interp(x1, ax, c1)
+    a2, k2 = np.interp(x2, ax, n2), np.interp(x2, ax, c2)
+    exact = (a1 * k1 + lr * a2 * k2) / (k1 + lr * k2)
+    got = O.binary_model_norm(small.og, cgrid, small.odisp, L1, L2, lrin)
+    # the reference quantises each resample weight to f32 (<= 1.5e-3 of a pixel step here)
+    step = np.abs(np.diff(exact)).max()
+    assert np.abs(got - exact).max() < 2e-3 * step + 2e-6
+
+
+def test_binary_range_error(small, cgrid):
+    with pytest.raises(O.OracleError) as e:
+        O.binary_model_norm(small.og, cgrid, small.odisp, L1, [5400.0, 0.1, 4.6, 30.0, 9000.0], 0.5)
+    assert e.value.code == -5
+
+
+def test_binary_cost_composition(small, cgrid):
+    lab = np.concatenate([L1, L2, [0.6]])
+    c, p = O.binary_cost(small.og, cgrid, small.odisp, small.ocont, small.flux, small.var, lab)
+    synth_m = O.binary_model_norm(small.og, cgrid, small.odisp, L1, L2, np.float32(0.6))
+    p2, c2 = small.ocont.fit(small.flux, small.var, synth_m)
+    assert c == c2 and np.array_equal(p, p2)
+    # identical stars: the single-star cost
+    same = np.concatenate([L1, L1, [0.4]])
+    c3, _ = O.binary_cost(small.og, cgrid, small.odisp, small.ocont, small.flux, small.var, same)
+    c4, _ = O.cost(small.og, small.odisp, small.ocont, small.flux, small.var, L1)
+    assert abs(c3 - c4) / c4 < 1e-5
+
+
+def test_binary_bounds(small):
+    b = O.binary_bounds(small.og, (0.0, 300.0), (-150.0, 150.0))
+    lo, hi = O.binary_bounds_outer(b)
+    sp = small.spec
+    assert lo[0] == sp.teff[0] and hi[5] == sp.teff[-1] and lo[10] == 0.0 and hi[10] == 1.0
+    assert hi[3] == 300.0 and lo[9] == -150.0
+    p = np.concatenate([L1, L2, [0.5]])
+    assert O.binary_bounds_within(b, p)
+    q = p.copy(); q[10] = 1.2
+    assert not O.binary_bounds_within(b, q)
+    q = p.copy(); q[5] = 9000.0
+    assert not O.binary_bounds_within(b, q)
+    assert O.binary_bounds_clamp(b, q, 5) == sp.teff[-1]
+    q = p.copy(); q[10] = -3.0
+    assert O.binary_bounds_clamp(b, q, 10) == 0.0
+    bc = O.binary_bounds(small.og, (0.0, 300.0), (-150.0, 150.0),
+                         constraints=[(9, 1, 12.5, 0.0), (6, 2, -0.5, 0.0)])
+    lo, hi = O.binary_bounds_outer(bc)
+    assert lo[9] == hi[9] == 12.5 and lo[6] == -0.5 and hi[6] == 0.0
+    assert O.binary_bounds_clamp(bc, p, 9) == 12.5
+
+
+def test_binary_pso_bowl(small):
+    b = O.binary_bounds(small.og, (0.0, 300.0), (-150.0, 150.0))
+    lo, hi = O.binary_bounds_outer(b)
+    target = np.concatenate([L1, L2, [0.35]])
+    scale = hi - lo
+    seen = []
+
+    def cost(pos):
+        for p in pos:
+            assert O.binary_bounds_within(b, p)
+        seen.append(len(pos))
+        return (((pos - target) / scale) ** 2).sum(axis=1)
+
+    s = O.pso_settings(80, 150)
+    best, bc, trace, ne = O.pso_run_binary(b, s, 17, cost)
+    assert ne == 80 * 151 and sum(seen) == ne
+    assert np.all(np.diff(trace[:, 0]) <= 0)
+    # the swarm can stall against a bound (vsini at 0 here), so the bowl is bounded loosely
+    assert bc < 0.02 * trace[0, 0] and bc < 5e-3
+    assert np.all(np.abs(best - target) / scale < 0.05)
+    # same seed, same trace
+    best2, bc2, trace2, _ = O.pso_run_binary(b, s, 17, cost)
+    assert np.array_equal(trace, trace2)
