@@ -273,6 +273,58 @@ typedef struct pasta_binary_result {
 int pasta_binary_fit(pasta_binary *b, const pasta_binary_settings *st, uint64_t seed,
                      const double *init_pos, pasta_binary_result *out, double *trace);
 
+/* BinaryTimeriesKnownRVFitter (src/fitting.rs:882-1078; Python
+ * get_timeseries_known_rv_fitter, src/lib.rs:1212-1227,1781-1855): S spectra of one binary
+ * on the dispersion's wavelengths (fluxes, vars S x n) with known RVs (rvs S x 2); labels
+ * are 9 values, (Teff, [M/H], logg, vsini) of each star and the light ratio, and the
+ * chi2 of a label is the spectrum-ordered sum (BinaryBoundsWithoutRV, src/bounds.rs:278-345). */
+int pasta_binary_ts_create(pasta_grid *g, pasta_grid *cg, const pasta_disp *d,
+                           const pasta_cont *c, const float *fluxes, const float *vars,
+                           const double *rvs, int S, int n, pasta_binary **out);
+int pasta_binary_ts_eval_chi2(pasta_binary *b, const double *labels, int P, double *chi2,
+                              int *status);
+typedef struct pasta_ts_result {
+    double label1[4];
+    double label2[4];
+    double light_ratio;
+    double chi2;
+    int iterations;
+    int status;
+    double time_s;
+    int n_params;   /* per spectrum */
+    int n_spectra;
+} pasta_ts_result;
+/* continuum_params (nullable) S x n_params and chis (nullable) S of the best labels;
+ * trace optional (max_iters+1) x 10 */
+int pasta_binary_ts_fit(pasta_binary *b, const pasta_binary_settings *st, uint64_t seed,
+                        const double *init_pos, pasta_ts_result *out, float *continuum_params,
+                        double *chis, double *trace);
+
+/* BinaryRVFitter (src/fitting.rs:730-880 with RVCostFunction and BinaryRVBounds; Python
+ * get_binary_rv_fitter, src/lib.rs:1195-1210,1702-1779): the two components and their
+ * continua are fixed synthetic-grid arrays (n_pix each) and only (rv1, rv2) are fitted;
+ * the settings' rv_range and constraints 0-1 apply. */
+typedef struct pasta_binary_rv pasta_binary_rv;
+int pasta_binary_rv_create(pasta_grid *g, const pasta_disp *d, const pasta_cont *c,
+                           const float *flux, const float *var, int n, const float *model1,
+                           const float *model2, const float *continuum1,
+                           const float *continuum2, double light_ratio, pasta_binary_rv **out);
+int pasta_binary_rv_destroy(pasta_binary_rv *r);
+/* RVCostFunction::cost for rvs P x 2 */
+int pasta_binary_rv_eval_chi2(pasta_binary_rv *r, const double *rvs, int P, double *chi2,
+                              int *status);
+typedef struct pasta_rv_result {
+    double rv1;
+    double rv2;
+    double chi2;      /* the swapped pair's chi2 when lower (src/fitting.rs:846-863) */
+    int iterations;
+    int status;
+    double time_s;
+} pasta_rv_result;
+/* trace optional (max_iters+1) x 3 */
+int pasta_binary_rv_fit(pasta_binary_rv *r, const pasta_binary_settings *st, uint64_t seed,
+                        const double *init_pos, pasta_rv_result *out, double *trace);
+
 /* ---------------------------------------------------------------- server --
  * Continuous batching of fit requests over a resident grid: requests are
  * admitted into free slots (and free HBM) between PSO iterations, every

@@ -135,6 +135,12 @@ _sig = {
     "or_binbounds_clamp": (C.c_int, [C.POINTER(_BinBounds), _dp, C.c_int, _dp]),
     "or_pso_run_binary": (C.c_int, [C.POINTER(_BinBounds), C.POINTER(_PsoSettings), C.c_uint64,
                                     _dp, COST_FN, C.c_void_p, _dp, _dp, _dp, C.POINTER(C.c_long)]),
+    "or_pso_run_d": (C.c_int, [C.POINTER(_BinBounds), C.c_int, C.POINTER(_PsoSettings), C.c_uint64,
+                               _dp, COST_FN, C.c_void_p, _dp, _dp, _dp, C.POINTER(C.c_long)]),
+    "or_binary_rv_cost": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _fp, _fp,
+                                    _fp, _fp, C.c_double, C.c_double, C.c_double, _dp]),
+    "or_binary_ts_cost": (C.c_int, [C.POINTER(_Model), C.POINTER(_Model), C.c_void_p, C.c_void_p,
+                                    _fp, _fp, _dp, C.c_int, _dp, _dp]),
     "or_brent_root": (C.c_int, [SCALAR_FN, C.c_void_p, C.c_double, C.c_double, C.c_double, _dp, _ip]),
     "or_compute_uncertainty": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp,
                                          C.c_double, _dp, _dp, _dp, _ip]),
@@ -568,6 +574,55 @@ def binary_bounds_clamp(b, p, d):
     out = C.c_double()
     _chk(_L.or_binbounds_clamp(C.byref(b), _p(_d(p), C.c_double), d, C.byref(out)), "clamp")
     return out.value
+
+
+def binary_rv_cost(grid, disp, cont, flux, var, model1, model2, cont1, cont2, light_ratio,
+                   rv1, rv2):
+    """RVCostFunction::cost (fitting.rs:736-781) on synthetic-grid components."""
+    flux, var = _f(flux), _f(var)
+    arrs = [_f(a) for a in (model1, model2, cont1, cont2)]
+    M = grid.model()
+    c = C.c_double()
+    _chk(_L.or_binary_rv_cost(C.byref(M), disp._h, cont._h, _p(flux, C.c_float), _p(var, C.c_float),
+                              *[_p(a, C.c_float) for a in arrs], float(light_ratio), float(rv1),
+                              float(rv2), C.byref(c)), "binary_rv_cost")
+    return c.value
+
+
+def binary_ts_cost(grid, cgrid, disp, cont, fluxes, vars_, rvs, labels9):
+    """BinaryTimeseriesKnownRVCostFunction::cost (fitting.rs:954-985)."""
+    fl, vr = _f(np.stack(fluxes)), _f(np.stack(vars_))
+    rv = _d(rvs).reshape(-1, 2)
+    M, CM = grid.model(), cgrid.model()
+    c = C.c_double()
+    _chk(_L.or_binary_ts_cost(C.byref(M), C.byref(CM), disp._h, cont._h, _p(fl, C.c_float),
+                              _p(vr, C.c_float), _p(rv, C.c_double), len(rv),
+                              _p(_d(labels9), C.c_double), C.byref(c)), "binary_ts_cost")
+    return c.value
+
+
+def pso_run_d(b, D, s, seed, cost_fn, init_pos=None):
+    """The binary PSO over D = 11, 9 or 2 parameters with a Python cost f(pos[n,D])."""
+    def _cb(ctx, n, pos, out):
+        try:
+            p = np.ctypeslib.as_array(pos, shape=(n, D)).copy()
+            c = np.asarray(cost_fn(p), dtype=np.float64)
+            for i in range(n):
+                out[i] = c[i]
+            return 0
+        except Exception:
+            return -7
+    cb = COST_FN(_cb)
+    best = np.zeros(D)
+    bc = C.c_double()
+    trace = np.zeros((s.max_iters + 1, D + 1))
+    ne = C.c_long()
+    ipa = None if init_pos is None else _d(init_pos)
+    _chk(_L.or_pso_run_d(C.byref(b), D, C.byref(s), seed,
+                         None if ipa is None else _p(ipa, C.c_double), cb, None,
+                         _p(best, C.c_double), C.byref(bc), _p(trace, C.c_double), C.byref(ne)),
+         "pso_run_d")
+    return best, bc.value, trace, ne.value
 
 
 def pso_run_binary(b, s, seed, cost_fn, init_pos=None):

@@ -1998,6 +1998,98 @@ int or_binbounds_clamp(const or_binbounds *b, const double p[11], int d, double 
     return OR_OK;
 }
 
+/* BinaryBoundsWithoutRV (bounds.rs:278-345): (teff, m, logg, vsini) x 2 + light ratio */
+static int binb9_limits(const or_binbounds *b, const double p[9], int i, double *lo, double *hi) {
+    int rc = OR_OK;
+    switch (i) {
+    case 3: case 7: *lo = b->vsini_lo; *hi = b->vsini_hi; break;
+    case 8: *lo = b->lr_lo; *hi = b->lr_hi; break;
+    default:
+        rc = i < 3 ? grid_limits_at(b->g, p, i, lo, hi) : grid_limits_at(b->g, p + 4, i - 4, lo, hi);
+    }
+    if (rc) return rc;
+    if (b->ckind[i] == 1) { *lo = b->ca[i]; *hi = b->ca[i]; }
+    else if (b->ckind[i] == 2) { *lo = fmax(*lo, b->ca[i]); *hi = fmin(*hi, b->cb[i]); }
+    return OR_OK;
+}
+
+static void binb9_outer(const or_binbounds *b, double lo[9], double hi[9]) {
+    const or_grid *g = b->g;
+    for (int s = 0; s < 2; s++) {
+        lo[4 * s] = g->teff[0]; hi[4 * s] = g->teff[g->n_teff - 1];
+        lo[4 * s + 1] = g->m[0]; hi[4 * s + 1] = g->m[g->n_m - 1];
+        lo[4 * s + 2] = g->logg[0]; hi[4 * s + 2] = g->logg[g->n_logg - 1];
+        lo[4 * s + 3] = b->vsini_lo; hi[4 * s + 3] = b->vsini_hi;
+    }
+    lo[8] = b->lr_lo; hi[8] = b->lr_hi;
+    for (int d = 0; d < 9; d++) {
+        if (b->ckind[d] == 1) { lo[d] = b->ca[d]; hi[d] = b->ca[d]; }
+        else if (b->ckind[d] == 2) { lo[d] = fmax(lo[d], b->ca[d]); hi[d] = fmin(hi[d], b->cb[d]); }
+    }
+}
+
+static int binb9_within(const or_binbounds *b, const double p[9]) {
+    const or_grid *g = b->g;
+    int ok = 1;
+    for (int s = 0; s < 2 && ok; s++) {
+        const double *q = p + 4 * s;
+        ok = or_is_teff_logg_between_bounds(g, q[0], q[2]) && q[1] >= g->m[0] &&
+             q[1] <= g->m[g->n_m - 1];
+    }
+    ok = ok && p[3] >= b->vsini_lo && p[3] <= b->vsini_hi && p[7] >= b->vsini_lo &&
+         p[7] <= b->vsini_hi && p[8] >= b->lr_lo && p[8] <= b->lr_hi;
+    if (!ok) return 0;
+    for (int d = 0; d < 9; d++) {
+        if (b->ckind[d] == 1 && !(fabs(p[d] - b->ca[d]) <= 2.220446049250313e-16)) return 0;
+        if (b->ckind[d] == 2 && !(p[d] >= b->ca[d] && p[d] <= b->cb[d])) return 0;
+    }
+    return 1;
+}
+
+/* BinaryRVBounds (bounds.rs:347-372): two RVs in rv_range */
+static void binrv_outer(const or_binbounds *b, double lo[2], double hi[2]) {
+    for (int d = 0; d < 2; d++) {
+        lo[d] = b->rv_lo; hi[d] = b->rv_hi;
+        if (b->ckind[d] == 1) { lo[d] = b->ca[d]; hi[d] = b->ca[d]; }
+        else if (b->ckind[d] == 2) { lo[d] = fmax(lo[d], b->ca[d]); hi[d] = fmin(hi[d], b->cb[d]); }
+    }
+}
+
+static int binrv_within(const or_binbounds *b, const double p[2]) {
+    if (!(p[0] >= b->rv_lo && p[0] <= b->rv_hi && p[1] >= b->rv_lo && p[1] <= b->rv_hi)) return 0;
+    for (int d = 0; d < 2; d++) {
+        if (b->ckind[d] == 1 && !(fabs(p[d] - b->ca[d]) <= 2.220446049250313e-16)) return 0;
+        if (b->ckind[d] == 2 && !(p[d] >= b->ca[d] && p[d] <= b->cb[d])) return 0;
+    }
+    return 1;
+}
+
+/* the three binary searches: D = 11 (BinaryBounds), 9 (without RV), 2 (the RVs) */
+static void bn_outer(const or_binbounds *b, int D, double *lo, double *hi) {
+    if (D == 11) or_binbounds_outer(b, lo, hi);
+    else if (D == 9) binb9_outer(b, lo, hi);
+    else binrv_outer(b, lo, hi);
+}
+
+static int bn_within(const or_binbounds *b, int D, const double *p) {
+    return D == 11 ? or_binbounds_within(b, p) : D == 9 ? binb9_within(b, p) : binrv_within(b, p);
+}
+
+static int bn_clamp(const or_binbounds *b, int D, const double *p, int d, double *out) {
+    double lo, hi;
+    int rc;
+    if (D == 11) rc = binb_limits(b, p, d, &lo, &hi);
+    else if (D == 9) rc = binb9_limits(b, p, d, &lo, &hi);
+    else {
+        lo = b->rv_lo; hi = b->rv_hi; rc = OR_OK;
+        if (b->ckind[d] == 1) { lo = b->ca[d]; hi = b->ca[d]; }
+        else if (b->ckind[d] == 2) { lo = fmax(lo, b->ca[d]); hi = fmin(hi, b->cb[d]); }
+    }
+    if (rc || lo > hi) return OR_ERR_PSO;
+    *out = clampd(p[d], lo, hi);
+    return OR_OK;
+}
+
 typedef struct { double pos[11], vel[11], cost, bpos[11], bcost; } particle11_t;
 
 static void sort_particles11(particle11_t *p, int n) {
@@ -2009,14 +2101,16 @@ static void sort_particles11(particle11_t *p, int n) {
     }
 }
 
-/* The PSO of particleswarm.rs over the 11 binary parameters (BinaryFitter::fit,
- * fitting.rs:635-706); same draws and update order as or_pso_run with D = 11 */
-int or_pso_run_binary(const or_binbounds *b, const or_pso_settings *s, uint64_t seed,
-                      const double *init_pos, or_cost_fn fn, void *ctx, double best[11],
-                      double *best_cost, double *trace_best, long *n_evals) {
-    const int D = 11, P = s->num_particles;
+/* The PSO of particleswarm.rs over D = 11, 9 or 2 binary parameters (BinaryFitter,
+ * BinaryTimeriesKnownRVFitter, BinaryRVFitter); the same draws and update order as
+ * or_pso_run with D parameters */
+int or_pso_run_d(const or_binbounds *b, int D, const or_pso_settings *s, uint64_t seed,
+                 const double *init_pos, or_cost_fn fn, void *ctx, double *best,
+                 double *best_cost, double *trace_best, long *n_evals) {
+    if (D != 11 && D != 9 && D != 2) return OR_ERR_ARG;
+    const int P = s->num_particles;
     double lo[11], hi[11];
-    or_binbounds_outer(b, lo, hi);
+    bn_outer(b, D, lo, hi);
     double *pos = malloc(sizeof(double) * D * P), *vel = malloc(sizeof(double) * D * P);
     double *cost = malloc(sizeof(double) * P);
     particle11_t *pt = malloc(sizeof(particle11_t) * P);
@@ -2029,7 +2123,7 @@ int or_pso_run_binary(const or_binbounds *b, const or_pso_settings *s, uint64_t 
             p[d] = rand_range(lo[d], hi[d], or_rng_uniform(kp, attempt * D + d));
         attempt++;
         if (attempt > 100000000ULL) { rc = OR_ERR_PSO; goto done; }
-        if (or_binbounds_within(b, p)) { memcpy(pos + D * got, p, sizeof(p)); got++; }
+        if (bn_within(b, D, p)) { memcpy(pos + D * got, p, sizeof(double) * D); got++; }
     }
     for (int q = 0; q < P; q++)
         for (int d = 0; d < D; d++) {
@@ -2049,8 +2143,8 @@ int or_pso_run_binary(const or_binbounds *b, const or_pso_settings *s, uint64_t 
     }
     sort_particles11(pt, P);
     double g[11], gc = pt[0].cost;
-    memcpy(g, pt[0].pos, sizeof(g));
-    if (trace_best) { trace_best[0] = gc; memcpy(trace_best + 1, g, sizeof(g)); }
+    memcpy(g, pt[0].pos, sizeof(double) * D);
+    if (trace_best) { trace_best[0] = gc; memcpy(trace_best + 1, g, sizeof(double) * D); }
     uint64_t ki = or_rng_u64(seed, KEY_ITER);
     for (int it = 1; it <= s->max_iters && !rc; it++) {
         for (int q = 0; q < P && !rc; q++) {
@@ -2063,13 +2157,13 @@ int or_pso_run_binary(const or_binbounds *b, const or_pso_settings *s, uint64_t 
                 double pullg = rand_range(0.0, to_g, or_rng_uniform(ki, c0 + 1)) * s->social;
                 pt[q].vel[d] = momentum + pull + pullg;
                 double np[11];
-                memcpy(np, pt[q].pos, sizeof(np));
+                memcpy(np, pt[q].pos, sizeof(double) * D);
                 np[d] += pt[q].vel[d];
                 double cl;
-                rc = or_binbounds_clamp(b, np, d, &cl);
+                rc = bn_clamp(b, D, np, d, &cl);
                 if (rc) break;
                 pt[q].pos[d] = cl;
-                if (!or_binbounds_within(b, pt[q].pos)) { rc = OR_ERR_PSO; break; }
+                if (!bn_within(b, D, pt[q].pos)) { rc = OR_ERR_PSO; break; }
             }
         }
         if (rc) break;
@@ -2083,16 +2177,78 @@ int or_pso_run_binary(const or_binbounds *b, const or_pso_settings *s, uint64_t 
                 memcpy(pt[q].bpos, pt[q].pos, sizeof(double) * D);
                 pt[q].bcost = pt[q].cost;
                 if (pt[q].cost < gc) {
-                    memcpy(g, pt[q].pos, sizeof(g));
+                    memcpy(g, pt[q].pos, sizeof(double) * D);
                     gc = pt[q].cost;
                 }
             }
         }
-        if (trace_best) { trace_best[12 * it] = gc; memcpy(trace_best + 12 * it + 1, g, sizeof(g)); }
+        if (trace_best) {
+            trace_best[(D + 1) * it] = gc;
+            memcpy(trace_best + (D + 1) * it + 1, g, sizeof(double) * D);
+        }
     }
-    if (!rc) { memcpy(best, g, sizeof(g)); *best_cost = gc; }
+    if (!rc) { memcpy(best, g, sizeof(double) * D); *best_cost = gc; }
     if (n_evals) *n_evals = evals;
 done:
     free(pos); free(vel); free(cost); free(pt);
+    return rc;
+}
+
+int or_pso_run_binary(const or_binbounds *b, const or_pso_settings *s, uint64_t seed,
+                      const double *init_pos, or_cost_fn fn, void *ctx, double best[11],
+                      double *best_cost, double *trace_best, long *n_evals) {
+    return or_pso_run_d(b, 11, s, seed, init_pos, fn, ctx, best, best_cost, trace_best, n_evals);
+}
+
+/* RVCostFunction::cost (fitting.rs:736-781): the components are synthetic-grid arrays
+ * (ArraySegments from vectors); each is shifted and resampled on its own, then
+ * lr = light_ratio * mean(c2') / mean(c1'), model = (s1' c1' + s2' c2' lr) / (c1' + c2' lr) */
+int or_binary_rv_cost(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                      const float *var, const float *model1, const float *model2,
+                      const float *cont1, const float *cont2, double light_ratio, double rv1,
+                      double rv2, double *chi2) {
+    int W = M->sg.n, n = d->n_obs, rc;
+    float *s1 = malloc(sizeof(float) * n), *s2 = malloc(sizeof(float) * n);
+    float *k1 = malloc(sizeof(float) * n), *k2 = malloc(sizeof(float) * n);
+    seg32 a1 = {(float *)model1, W, 0, W}, a2 = {(float *)model2, W, 0, W};
+    seg32 b1 = {(float *)cont1, W, 0, W}, b2 = {(float *)cont2, W, 0, W};
+    rc = resample32(M, d, &a1, rv1, s1);
+    if (!rc) rc = resample32(M, d, &a2, rv2, s2);
+    if (!rc) rc = resample32(M, d, &b1, rv1, k1);
+    if (!rc) rc = resample32(M, d, &b2, rv2, k2);
+    if (!rc) {
+        float lr = (float)light_ratio * mean_f32(k2, n) / mean_f32(k1, n);
+        for (int i = 0; i < n; i++) {
+            float synth = s1[i] * k1[i] + s2[i] * k2[i] * lr;
+            float cont = k1[i] + k2[i] * lr;
+            s1[i] = synth / cont;
+        }
+        rc = or_fit_continuum(c, flux, var, s1, NULL, chi2);
+    }
+    free(s1); free(s2); free(k1); free(k2);
+    return rc;
+}
+
+/* BinaryTimeseriesKnownRVCostFunction::cost (fitting.rs:954-985): components once from
+ * lab9 = (teff, m, logg, vsini) x 2 + light ratio, then per observed spectrum k (same
+ * wavelengths, RVs rvs[2k], rvs[2k+1]) the binary model and its continuum-fit chi2,
+ * summed in spectrum order */
+int or_binary_ts_cost(const or_model *M, const or_model *CM, const or_disp *d, const or_cont *c,
+                      const float *fluxes, const float *vars, const double *rvs, int S,
+                      const double lab9[9], double *chi2) {
+    int n = d->n_obs, rc = OR_OK;
+    float *synth = malloc(sizeof(float) * n);
+    double total = 0.0;
+    for (int k = 0; k < S && !rc; k++) {
+        double l1[5] = {lab9[0], lab9[1], lab9[2], lab9[3], rvs[2 * k]};
+        double l2[5] = {lab9[4], lab9[5], lab9[6], lab9[7], rvs[2 * k + 1]};
+        rc = or_binary_model_norm(M, CM, d, l1, l2, (float)lab9[8], synth);
+        double ck = 0.0;
+        if (!rc) rc = or_fit_continuum(c, fluxes + (size_t)k * n, vars + (size_t)k * n, synth,
+                                       NULL, &ck);
+        total += ck;
+    }
+    *chi2 = total;
+    free(synth);
     return rc;
 }

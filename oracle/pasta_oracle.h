@@ -219,6 +219,22 @@ typedef struct or_binbounds {
 int or_binbounds_outer(const or_binbounds *b, double lo[11], double hi[11]);
 int or_binbounds_within(const or_binbounds *b, const double p[11]);
 int or_binbounds_clamp(const or_binbounds *b, const double p[11], int d, double *out);
+/* the PSO over D = 11 (BinaryBounds), 9 (BinaryBoundsWithoutRV: star labels without RV
+ * at 0-3 and 4-7, light ratio at 8) or 2 (BinaryRVBounds) parameters; fn receives
+ * positions n x D; trace_best (max_iters+1) x (D+1) */
+int or_pso_run_d(const or_binbounds *b, int D, const or_pso_settings *s, uint64_t seed,
+                 const double *init_pos, or_cost_fn fn, void *ctx, double *best,
+                 double *best_cost, double *trace_best, long *n_evals);
+/* RVCostFunction::cost: components are synthetic-grid arrays (W each) */
+int or_binary_rv_cost(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                      const float *var, const float *model1, const float *model2,
+                      const float *cont1, const float *cont2, double light_ratio, double rv1,
+                      double rv2, double *chi2);
+/* BinaryTimeseriesKnownRVCostFunction::cost: S spectra (S x n fluxes/vars) on the
+ * dispersion's wavelengths, rvs S x 2, lab9 = (teff, m, logg, vsini) x 2 + light ratio */
+int or_binary_ts_cost(const or_model *M, const or_model *CM, const or_disp *d, const or_cont *c,
+                      const float *fluxes, const float *vars, const double *rvs, int S,
+                      const double lab9[9], double *chi2);
 /* the PSO over 11 parameters; fn receives positions n x 11; trace_best (max_iters+1) x 12 */
 int or_pso_run_binary(const or_binbounds *b, const or_pso_settings *s, uint64_t seed,
                       const double *init_pos, or_cost_fn fn, void *ctx, double best[11],

@@ -91,6 +91,31 @@ class BinaryResult(C.Structure):
     ]
 
 
+class TsResult(C.Structure):
+    _fields_ = [
+        ("label1", C.c_double * 4),
+        ("label2", C.c_double * 4),
+        ("light_ratio", C.c_double),
+        ("chi2", C.c_double),
+        ("iterations", C.c_int),
+        ("status", C.c_int),
+        ("time_s", C.c_double),
+        ("n_params", C.c_int),
+        ("n_spectra", C.c_int),
+    ]
+
+
+class RvResult(C.Structure):
+    _fields_ = [
+        ("rv1", C.c_double),
+        ("rv2", C.c_double),
+        ("chi2", C.c_double),
+        ("iterations", C.c_int),
+        ("status", C.c_int),
+        ("time_s", C.c_double),
+    ]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("max_stars", C.c_int),
@@ -166,6 +191,15 @@ SIGNATURES = {
     "pasta_binary_eval_chi2": [_P, _dp, C.c_int, _dp, _ip, _fp],
     "pasta_binary_fit": [_P, C.POINTER(BinarySettings), C.c_uint64, _dp, C.POINTER(BinaryResult),
                          _dp],
+    "pasta_binary_ts_create": [_P, _P, _P, _P, _fp, _fp, _dp, C.c_int, C.c_int, _PP],
+    "pasta_binary_ts_eval_chi2": [_P, _dp, C.c_int, _dp, _ip],
+    "pasta_binary_ts_fit": [_P, C.POINTER(BinarySettings), C.c_uint64, _dp, C.POINTER(TsResult),
+                            _fp, _dp, _dp],
+    "pasta_binary_rv_create": [_P, _P, _P, _fp, _fp, C.c_int, _fp, _fp, _fp, _fp, C.c_double, _PP],
+    "pasta_binary_rv_destroy": [_P],
+    "pasta_binary_rv_eval_chi2": [_P, _dp, C.c_int, _dp, _ip],
+    "pasta_binary_rv_fit": [_P, C.POINTER(BinarySettings), C.c_uint64, _dp, C.POINTER(RvResult),
+                            _dp],
 }
 
 

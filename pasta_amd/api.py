@@ -12,8 +12,11 @@ for the model-evaluation path:
         get_neighbor_indices, teffs, ms, loggs, logg_limits, cumulative_grid_size,
         list_gridpoints, is_teff_logg_between_bounds, clamp_1d}
     InMemSingleFitter.{fit, fit_bulk, chi2}
-    InMemInterpolator.{interpolate_linear, produce_binary_model_norm, get_binary_fitter}
-    BinaryFitter.{fit, chi2}, BinaryOptimizationResult
+    InMemInterpolator.{interpolate_linear, produce_binary_model_norm, get_binary_fitter,
+        get_binary_rv_fitter, get_timeseries_known_rv_fitter}
+    BinaryFitter, BinaryRVFitter, BinaryTimeseriesKnownRVFitter (fit, chi2),
+    BinaryOptimizationResult, RVOptimizationResult, RvLessLabel,
+    BinaryTimeseriesOptimizationResult
 
 Every computation runs in the HIP library; there is no CPU fallback.  Errors
 raise ValueError like PyO3's anyhow conversion.
@@ -33,7 +36,9 @@ __all__ = [
     "NoConvolutionDispersion", "FixedResolutionDispersion", "VariableResolutionDispersion",
     "ChunkContinuumFitter", "LinearModelContinuumFitter", "FixedContinuum", "ConstantContinuum",
     "FixConstraint", "RangeConstraint", "get_vsini_kernel", "InMemSingleFitter", "FitServer",
-    "LabelUncertainties", "BinaryFitter", "BinaryOptimizationResult",
+    "LabelUncertainties", "BinaryFitter", "BinaryOptimizationResult", "BinaryRVFitter",
+    "RVOptimizationResult", "BinaryTimeseriesKnownRVFitter", "RvLessLabel",
+    "BinaryTimeseriesOptimizationResult",
 ]
 
 
@@ -139,6 +144,61 @@ class BinaryOptimizationResult:
         return BinaryOptimizationResult(Label(*r.label1[:5]), Label(*r.label2[:5]), r.light_ratio,
                                         r.continuum_params[: r.n_params], r.chi2, r.iterations,
                                         r.time_s)
+
+
+class RVOptimizationResult:
+    """Output of a binary RV fit (src/lib.rs:420-447)."""
+
+    def __init__(self, rv1, rv2, chi2, iterations, time):
+        self.rv1, self.rv2, self.chi2 = float(rv1), float(rv2), float(chi2)
+        self.iterations, self.time = int(iterations), float(time)
+
+    def to_dict(self):
+        return dict(rv1=self.rv1, rv2=self.rv2, chi2=self.chi2, iterations=self.iterations,
+                    time=self.time)
+
+    def to_json(self):
+        return json.dumps(self.to_dict())
+
+
+class RvLessLabel:
+    """(Teff, [M/H], logg, vsini) of one star (src/lib.rs:449-520)."""
+
+    def __init__(self, teff, m, logg, vsini):
+        self.teff, self.m, self.logg, self.vsini = map(float, (teff, m, logg, vsini))
+
+    def as_list(self):
+        return [self.teff, self.m, self.logg, self.vsini]
+
+    def to_dict(self):
+        return dict(teff=self.teff, m=self.m, logg=self.logg, vsini=self.vsini)
+
+    @staticmethod
+    def from_dict(d):
+        return RvLessLabel(d["teff"], d["m"], d["logg"], d["vsini"])
+
+
+class BinaryTimeseriesOptimizationResult:
+    """Output of a time-series binary fit (src/lib.rs:522-600)."""
+
+    def __init__(self, label1, label2, light_ratio, continuum_params, chis, iters, time):
+        self.label1, self.label2, self.light_ratio = label1, label2, float(light_ratio)
+        self.continuum_params = [list(map(float, p)) for p in continuum_params]
+        self.chis, self.iters, self.time = list(map(float, chis)), int(iters), float(time)
+
+    def to_dict(self):
+        return dict(label1=self.label1.to_dict(), label2=self.label2.to_dict(),
+                    light_ratio=self.light_ratio, continuum_params=self.continuum_params,
+                    chis=self.chis, iters=self.iters, time=self.time)
+
+    def to_json(self):
+        return json.dumps(self.to_dict())
+
+    @staticmethod
+    def from_dict(d):
+        return BinaryTimeseriesOptimizationResult(
+            RvLessLabel.from_dict(d["label1"]), RvLessLabel.from_dict(d["label2"]),
+            d["light_ratio"], d["continuum_params"], d["chis"], d["iters"], d["time"])
 
 
 class LabelUncertainties:
@@ -501,6 +561,23 @@ class InMemInterpolator:
         """BinaryFitter (src/lib.rs:1177-1193)."""
         return BinaryFitter(dispersion, continuum_fitter, settings, vsini_range, rv_range)
 
+    def _check_wl(self, synth_wl):
+        w = self.wavelength
+        if (synth_wl.log != w.log or synth_wl.len != w.len or synth_wl.min != w.min
+                or synth_wl.step != w.step):
+            raise ValueError("synth_wl differs from the interpolator's wavelength grid")
+
+    def get_binary_rv_fitter(self, dispersion, synth_wl, continuum_fitter, settings, rv_range):
+        """BinaryRVFitter (src/lib.rs:1195-1210): fits (rv1, rv2) of fixed components."""
+        self._check_wl(synth_wl)
+        return BinaryRVFitter(self, dispersion, continuum_fitter, settings, rv_range)
+
+    def get_timeseries_known_rv_fitter(self, dispersion, synth_wl, continuum_fitter, settings,
+                                       vsini_range):
+        """BinaryTimeriesKnownRVFitter (src/lib.rs:1212-1227)."""
+        self._check_wl(synth_wl)
+        return BinaryTimeseriesKnownRVFitter(dispersion, continuum_fitter, settings, vsini_range)
+
     def interpolate_and_convolve(self, dispersion, teff, m, logg, vsini):
         lab = f64([[teff, m, logg, vsini]])
         out = np.zeros(self.n_pix, np.float32)
@@ -767,7 +844,84 @@ class _Binary:
         return out, st, params
 
 
-def _binary_settings_c(settings, vsini_range, rv_range, constraints):
+class _BinaryTS:
+    """S spectra of one binary with known RVs (pasta_binary_ts_*)."""
+
+    def __init__(self, interpolator, continuum_interpolator, dispersion, continuum, fluxes, vars_,
+                 rvs):
+        n = len(dispersion.wl)
+        fl = [f32(f) for f in fluxes]
+        vr = [f32(v) for v in vars_]
+        if len(fl) != len(vr) or len(fl) != len(rvs) or not fl:
+            raise ValueError("one flux, variance and RV pair per observed spectrum")
+        for f, v in zip(fl, vr):
+            if len(f) != len(v):
+                raise ValueError("Flux and variance vectors must have the same length")
+            if len(f) != n:
+                raise ValueError("Observed spectrum and target dispersion must have the same length")
+        F, V = np.ascontiguousarray(np.stack(fl)), np.ascontiguousarray(np.stack(vr))
+        R = f64(rvs).reshape(-1, 2)
+        h = C.c_void_p()
+        _raise(lib.pasta_binary_ts_create(interpolator._h, continuum_interpolator._h,
+                                          dispersion.handle(interpolator), continuum.handle(n),
+                                          ptr(F, C.c_float), ptr(V, C.c_float), ptr(R, C.c_double),
+                                          len(R), n, C.byref(h)), "binary time series")
+        self._h, self.S, self.n_params = h, len(R), continuum.n_params(n)
+        self._refs = (interpolator, continuum_interpolator, dispersion, continuum)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.pasta_binary_destroy(self._h)
+            self._h = None
+
+    def chi2(self, labels9):
+        lab = f64(labels9).reshape(-1, 9)
+        out = np.zeros(len(lab))
+        st = np.zeros(len(lab), np.int32)
+        lib.pasta_binary_ts_eval_chi2(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_double),
+                                      ptr(st, C.c_int))
+        return out, st
+
+
+class _BinaryRV:
+    """Fixed binary components, only the RVs free (pasta_binary_rv_*)."""
+
+    def __init__(self, interpolator, dispersion, continuum, model1, model2, continuum1, continuum2,
+                 light_ratio, flux, var):
+        flux, var = f32(flux), f32(var)
+        n = len(dispersion.wl)
+        if len(flux) != len(var):
+            raise ValueError("Flux and variance vectors must have the same length")
+        if len(flux) != n:
+            raise ValueError("Observed spectrum and target dispersion must have the same length")
+        comps = [f32(a) for a in (model1, model2, continuum1, continuum2)]
+        for a in comps:
+            if len(a) != interpolator.n_pix:
+                raise ValueError("Length of input_array and wavelength grid don't match")
+        h = C.c_void_p()
+        _raise(lib.pasta_binary_rv_create(interpolator._h, dispersion.handle(interpolator),
+                                          continuum.handle(n), ptr(flux, C.c_float),
+                                          ptr(var, C.c_float), n,
+                                          *[ptr(a, C.c_float) for a in comps], float(light_ratio),
+                                          C.byref(h)), "binary RV")
+        self._h = h
+        self._refs = (interpolator, dispersion, continuum)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.pasta_binary_rv_destroy(self._h)
+            self._h = None
+
+    def chi2(self, rvs):
+        lab = f64(rvs).reshape(-1, 2)
+        out = np.zeros(len(lab))
+        st = np.zeros(len(lab), np.int32)
+        lib.pasta_binary_rv_eval_chi2(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_double),
+                                      ptr(st, C.c_int))
+        return out, st
+
+
+def _binary_settings_c(settings, vsini_range, rv_range, constraints, n_labels=11):
     s = _lib.BinarySettings()
     s.num_particles = settings.num_particles
     s.max_iters = settings.max_iters
@@ -779,8 +933,8 @@ def _binary_settings_c(settings, vsini_range, rv_range, constraints):
     s.rv_range[0], s.rv_range[1] = rv_range
     s.light_ratio_range[0], s.light_ratio_range[1] = 0.0, 1.0
     for c in constraints or []:
-        if not 0 <= c.parameter < 11:
-            raise ValueError(f"constraint parameter {c.parameter} outside the 11 binary labels")
+        if not 0 <= c.parameter < n_labels:
+            raise ValueError(f"constraint parameter {c.parameter} outside the {n_labels} labels")
         s.constraint_kind[c.parameter] = c.kind
         s.constraint_a[c.parameter] = c.a
         s.constraint_b[c.parameter] = c.b
@@ -827,6 +981,95 @@ class BinaryFitter:
         allow_nan=False."""
         b = self._binary(interpolator, continuum_interpolator, observed_flux, observed_var)
         out, st, _ = b.chi2(labels)
+        if allow_nan is False and st.any():
+            raise ValueError(_lib.STATUS_TEXT.get(int(st[st != 0][0])))
+        return out.tolist()
+
+
+class BinaryRVFitter:
+    """BinaryRVFitter (src/fitting.rs:785-880, Python src/lib.rs:1702-1779): PSO over
+    (rv1, rv2) of fixed components (synthetic-grid models and continua)."""
+
+    def __init__(self, interpolator, dispersion, continuum_fitter, settings, rv_range):
+        self.interpolator, self.dispersion = interpolator, dispersion
+        self.continuum_fitter, self.settings, self.rv_range = continuum_fitter, settings, tuple(rv_range)
+
+    def _rv(self, model1, model2, continuum1, continuum2, light_ratio, flux, var):
+        return _BinaryRV(self.interpolator, self.dispersion, self.continuum_fitter, model1, model2,
+                         continuum1, continuum2, light_ratio, flux, var)
+
+    def fit(self, model1, model2, continuum1, continuum2, light_ratio, observed_flux, observed_var,
+            trace_directory=None, best_particle_file=None, constraints=None, seed=0, init_pos=None):
+        if trace_directory is not None and best_particle_file is not None:
+            raise ValueError("Cannot specify both trace_directory and best_particle_file")
+        r = self._rv(model1, model2, continuum1, continuum2, light_ratio, observed_flux, observed_var)
+        s = _binary_settings_c(self.settings, (0.0, 0.0), self.rv_range, constraints, 2)
+        res = _lib.RvResult()
+        trace = np.zeros((self.settings.max_iters + 1, 3))
+        keep = None if init_pos is None else f64(init_pos)
+        _raise(lib.pasta_binary_rv_fit(r._h, C.byref(s), int(seed),
+                                       None if keep is None else ptr(keep, C.c_double),
+                                       C.byref(res), ptr(trace, C.c_double)), "fit")
+        out = RVOptimizationResult(res.rv1, res.rv2, res.chi2, res.iterations, res.time_s)
+        if best_particle_file is not None:
+            with open(best_particle_file, "w") as f:
+                json.dump([{"position": t[1:].tolist(), "cost": t[0]} for t in trace], f)
+        out._trace = trace
+        return out
+
+    def chi2(self, model1, model2, continuum1, continuum2, light_ratio, observed_flux, observed_var,
+             labels, allow_nan=None):
+        r = self._rv(model1, model2, continuum1, continuum2, light_ratio, observed_flux, observed_var)
+        out, st = r.chi2(labels)
+        if allow_nan is False and st.any():
+            raise ValueError(_lib.STATUS_TEXT.get(int(st[st != 0][0])))
+        return out.tolist()
+
+
+class BinaryTimeseriesKnownRVFitter:
+    """BinaryTimeriesKnownRVFitter (src/fitting.rs:987-1078, Python src/lib.rs:1781-1855):
+    PSO over (Teff, [M/H], logg, vsini) of both stars and the light ratio, the chi2 summed
+    over spectra with known RVs."""
+
+    def __init__(self, dispersion, continuum_fitter, settings, vsini_range):
+        self.dispersion, self.continuum_fitter = dispersion, continuum_fitter
+        self.settings, self.vsini_range = settings, tuple(vsini_range)
+
+    def _ts(self, interpolator, continuum_interpolator, fluxes, vars_, rvs):
+        return _BinaryTS(interpolator, continuum_interpolator, self.dispersion,
+                         self.continuum_fitter, fluxes, vars_, rvs)
+
+    def fit(self, interpolator, continuum_interpolator, observed_fluxes, observed_vars, rvs,
+            trace_directory=None, best_particle_file=None, parallelize=None, constraints=None,
+            seed=0, init_pos=None):
+        if trace_directory is not None and best_particle_file is not None:
+            raise ValueError("Cannot specify both trace_directory and best_particle_file")
+        t = self._ts(interpolator, continuum_interpolator, observed_fluxes, observed_vars, rvs)
+        s = _binary_settings_c(self.settings, self.vsini_range, (0.0, 0.0), constraints, 9)
+        res = _lib.TsResult()
+        np_ = max(1, t.n_params)
+        prm = np.zeros((t.S, np_), np.float32)
+        chis = np.zeros(t.S)
+        trace = np.zeros((self.settings.max_iters + 1, 10))
+        keep = None if init_pos is None else f64(init_pos)
+        _raise(lib.pasta_binary_ts_fit(t._h, C.byref(s), int(seed),
+                                       None if keep is None else ptr(keep, C.c_double),
+                                       C.byref(res), ptr(prm, C.c_float), ptr(chis, C.c_double),
+                                       ptr(trace, C.c_double)), "fit")
+        out = BinaryTimeseriesOptimizationResult(
+            RvLessLabel(*res.label1[:4]), RvLessLabel(*res.label2[:4]), res.light_ratio,
+            [p[: t.n_params] for p in prm], chis, res.iterations, res.time_s)
+        out.chi2 = res.chi2
+        if best_particle_file is not None:
+            with open(best_particle_file, "w") as f:
+                json.dump([{"position": tr[1:].tolist(), "cost": tr[0]} for tr in trace], f)
+        out._trace = trace
+        return out
+
+    def chi2(self, interpolator, continuum_interpolator, observed_fluxes, observed_vars, rvs, labels,
+             allow_nan=None):
+        t = self._ts(interpolator, continuum_interpolator, observed_fluxes, observed_vars, rvs)
+        out, st = t.chi2(labels)
         if allow_nan is False and st.any():
             raise ValueError(_lib.STATUS_TEXT.get(int(st[st != 0][0])))
         return out.tolist()

@@ -1668,7 +1668,8 @@ __global__ __launch_bounds__(64) void k_seq_mean(const float* __restrict__ in, i
 }
 
 // shift_resample_and_add_binary_components (src/convolve_rv.rs:606-690, Logspace grid)
-// for binary evaluation b = blockIdx.y: components 2b (star 1) and 2b+1 (star 2) are the
+// for binary evaluation b = blockIdx.y with the components of c = b % ncomp (the
+// evaluations of several spectra share one set): components 2c (star 1) and 2c+1 (star 2) are the
 // convolved models `comp` (synthetic grid, W per component, status in cevals), their
 // linear continua `cont` (status cstat) and the continua's means.  Writes the normalised
 // model at obs evaluation b's m_off and its status: the first failure in the reference's
@@ -1683,14 +1684,14 @@ __global__ __launch_bounds__(NT) void k_binary_combine(DevGrid G, const DevStar*
                                                        const int* __restrict__ cstat,
                                                        const float* __restrict__ cont,
                                                        const float* __restrict__ cmean,
-                                                       float* __restrict__ model) {
-    const int b = blockIdx.y;
+                                                       int ncomp, float* __restrict__ model) {
+    const int b = blockIdx.y, cb = b % ncomp;
     const DevStar& S = stars[oevals[b].star];
     const double* l = labels11 + (size_t)11 * b;
-    int st = cevals[2 * b].status;
-    if (!st) st = cstat[2 * b];
-    if (!st) st = cevals[2 * b + 1].status;
-    if (!st) st = cstat[2 * b + 1];
+    int st = cevals[2 * cb].status;
+    if (!st) st = cstat[2 * cb];
+    if (!st) st = cevals[2 * cb + 1].status;
+    if (!st) st = cstat[2 * cb + 1];
     const double sf1 = 1.0 - l[4] / C_KMS, sf2 = 1.0 - l[9] / C_KMS;
     if (!st && (S.wl0 * fmin(sf1, sf2) < G.synth_first_wl || S.wlN * fmax(sf1, sf2) > G.synth_last_wl))
         st = ST_RANGE;
@@ -1705,10 +1706,10 @@ __global__ __launch_bounds__(NT) void k_binary_combine(DevGrid G, const DevStar*
     if (st) return;
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= S.N) return;
-    const float lr = (float)l[10] * cmean[2 * b] / cmean[2 * b + 1];
-    const float* n1 = comp + (size_t)(2 * b) * G.W;
+    const float lr = (float)l[10] * cmean[2 * cb] / cmean[2 * cb + 1];
+    const float* n1 = comp + (size_t)(2 * cb) * G.W;
     const float* n2 = n1 + G.W;
-    const float* c1 = cont + (size_t)(2 * b) * G.W;
+    const float* c1 = cont + (size_t)(2 * cb) * G.W;
     const float* c2 = c1 + G.W;
     const double x1 = S.fidx[i] + sl1, x2 = S.fidx[i] + sl2;
     const long long j1 = jfloor(x1), j2 = jfloor(x2);
@@ -1720,16 +1721,106 @@ __global__ __launch_bounds__(NT) void k_binary_combine(DevGrid G, const DevStar*
     model[oevals[b].m_off + i] = (a1 * k1 + lr * a2 * k2) / (k1 + lr * k2);
 }
 
+// binary models of n_obs_evals evaluations (observed records oevals, labels11) from the
+// components of ncomp label pairs (records cevals, labels clabels with stride 5)
 hipError_t launch_binary(const DevGrid& G, const DevGrid& CG, const DevStar* stars,
-                         DevEval* oevals, int n, const double* labels11, const DevEval* cevals,
-                         const double* clabels, const float* comp, int* cstat, float* cont,
-                         float* cmean, int n_obs_max, float* model, hipStream_t st) {
+                         DevEval* oevals, int n_obs_evals, const double* labels11,
+                         const DevEval* cevals, const double* clabels, int ncomp,
+                         const float* comp, int* cstat, float* cont, float* cmean, int n_obs_max,
+                         float* model, hipStream_t st) {
+    if (n_obs_evals <= 0 || ncomp <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_interp_lin, dim3((G.W + 8 * NT - 1) / (8 * NT), 2 * ncomp), dim3(NT), 0,
+                       st, CG, clabels, 5, cont, cstat);
+    hipLaunchKernelGGL(k_seq_mean, dim3(2 * ncomp), dim3(64), 0, st, cont, G.W, cstat, cmean);
+    hipLaunchKernelGGL(k_binary_combine, dim3((n_obs_max + NT - 1) / NT, n_obs_evals), dim3(NT), 0,
+                       st, G, stars, oevals, labels11, cevals, comp, cstat, cont, cmean, ncomp,
+                       model);
+    return hipGetLastError();
+}
+
+// RVCostFunction (src/fitting.rs:736-781), first pass: the continua of evaluation b =
+// blockIdx.y shifted by its two RVs (rv[2b], rv[2b+1]) into kc rows 2b, 2b+1 (N each) and
+// its status: the first failure in the reference's order (model 1 and continuum 1 at
+// rv1: range, then resample index; model 2 and continuum 2 at rv2), written to the
+// observed record and to both rows' status.  Components: W-value synthetic-grid arrays.
+__global__ __launch_bounds__(NT) void k_rv_shift(DevGrid G, const DevStar* __restrict__ stars,
+                                                 DevEval* __restrict__ oevals,
+                                                 const double* __restrict__ rv,
+                                                 const float* __restrict__ cont1,
+                                                 const float* __restrict__ cont2,
+                                                 float* __restrict__ kc, int* __restrict__ kst) {
+    const int b = blockIdx.y;
+    const DevStar& S = stars[oevals[b].star];
+    int st = ST_OK;
+    double sl[2];
+    for (int c = 0; c < 2 && !st; c++) {
+        const double sf = 1.0 - rv[2 * b + c] / C_KMS;
+        if (S.wl0 * sf < G.synth_first_wl || S.wlN * sf > G.synth_last_wl) st = ST_RANGE;
+        sl[c] = log10(sf) / G.sg_step;
+        if (!st && (jfloor(S.fidx[0] + sl[c]) + 1 >= G.W || jfloor(S.fidx[S.N - 1] + sl[c]) + 1 >= G.W))
+            st = ST_INDEX;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        oevals[b].status = st;
+        kst[2 * b] = st;
+        kst[2 * b + 1] = st;
+    }
+    if (st) return;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= S.N) return;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const float* a = c ? cont2 : cont1;
+        const double x = S.fidx[i] + sl[c];
+        const long long j = jfloor(x);
+        const float w = (float)x - (float)j;
+        kc[(size_t)(2 * b + c) * S.N + i] = (1.0f - w) * a[j] + w * a[j + 1];
+    }
+}
+
+// second pass: lr = light_ratio * mean(c2') / mean(c1'), model = (s1' c1' + s2' c2' lr) /
+// (c1' + c2' lr) at the observed pixels, s' the shifted models
+__global__ __launch_bounds__(NT) void k_rv_combine(DevGrid G, const DevStar* __restrict__ stars,
+                                                   const DevEval* __restrict__ oevals,
+                                                   const double* __restrict__ rv,
+                                                   const float* __restrict__ model1,
+                                                   const float* __restrict__ model2,
+                                                   const float* __restrict__ kc,
+                                                   const float* __restrict__ kmean,
+                                                   float light_ratio, float* __restrict__ model) {
+    const int b = blockIdx.y;
+    const DevEval& ev = oevals[b];
+    if (ev.status) return;
+    const DevStar& S = stars[ev.star];
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= S.N) return;
+    float s[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const float* a = c ? model2 : model1;
+        const double x = S.fidx[i] + log10(1.0 - rv[2 * b + c] / C_KMS) / G.sg_step;
+        const long long j = jfloor(x);
+        const float w = (float)x - (float)j;
+        s[c] = (1.0f - w) * a[j] + w * a[j + 1];
+    }
+    const float lr = light_ratio * kmean[2 * b + 1] / kmean[2 * b];
+    const float k1 = kc[(size_t)(2 * b) * S.N + i], k2 = kc[(size_t)(2 * b + 1) * S.N + i];
+    const float synth = s[0] * k1 + s[1] * k2 * lr;
+    const float cont = k1 + k2 * lr;
+    model[ev.m_off + i] = synth / cont;
+}
+
+hipError_t launch_binary_rv(const DevGrid& G, const DevStar* stars, DevEval* oevals, int n,
+                            const double* rv, const float* model1, const float* model2,
+                            const float* cont1, const float* cont2, float light_ratio, int n_obs,
+                            float* kc, int* kst, float* kmean, float* model, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_interp_lin, dim3((G.W + 8 * NT - 1) / (8 * NT), 2 * n), dim3(NT), 0, st, CG,
-                       clabels, 5, cont, cstat);
-    hipLaunchKernelGGL(k_seq_mean, dim3(2 * n), dim3(64), 0, st, cont, G.W, cstat, cmean);
-    hipLaunchKernelGGL(k_binary_combine, dim3((n_obs_max + NT - 1) / NT, n), dim3(NT), 0, st, G,
-                       stars, oevals, labels11, cevals, comp, cstat, cont, cmean, model);
+    const dim3 grid((n_obs + NT - 1) / NT, n);
+    hipLaunchKernelGGL(k_rv_shift, grid, dim3(NT), 0, st, G, stars, oevals, rv, cont1, cont2, kc,
+                       kst);
+    hipLaunchKernelGGL(k_seq_mean, dim3(2 * n), dim3(64), 0, st, kc, n_obs, kst, kmean);
+    hipLaunchKernelGGL(k_rv_combine, grid, dim3(NT), 0, st, G, stars, oevals, rv, model1, model2,
+                       kc, kmean, light_ratio, model);
     return hipGetLastError();
 }
 

@@ -129,3 +129,113 @@ def test_binary_fit_recovers(small, binary):
     assert res.chi2 <= c_truth * (1 + 2e-3)
     d = res.to_dict()
     assert pasta.BinaryOptimizationResult.from_dict(d).to_dict() == d
+
+
+@pytest.fixture(scope="module")
+def rvcase(small, binary):
+    from pasta_amd import synth
+    m1 = O.interpolate_and_convolve(small.og, small.odisp, L1[:4])
+    m2 = O.interpolate_and_convolve(small.og, small.odisp, L2[:4])
+    c1 = binary["cgrid"].interpolate_linear(*L1[:3])
+    c2 = binary["cgrid"].interpolate_linear(*L2[:3])
+    flux, var = synth.observe(O.produce_model(small.og, small.odisp, L1), small.wl, 120.0, 91)
+    return dict(comps=(m1, m2, c1, c2), flux=flux, var=var)
+
+
+def test_binary_rv_chi2_matches_oracle(small, binary, rvcase):
+    pasta = binary["pasta"]
+    wlg = binary["interp"].wavelength
+    fitter = binary["interp"].get_binary_rv_fitter(binary["d"], wlg, small_cont(pasta, small),
+                                                   pasta.PSOSettings(8, 2), (-150, 150))
+    rng = np.random.default_rng(4)
+    rvs = np.concatenate([[[20.0, -40.0]], rng.uniform(-140, 140, (12, 2))])
+    allv = np.concatenate([rvs, [[9000.0, 0.0], [0.0, -9000.0]]])
+    got = np.array(fitter.chi2(*rvcase["comps"], 0.6, rvcase["flux"], rvcase["var"], allv))
+    assert np.isnan(got[-2:]).all()
+    worst = 0.0
+    for k, (a, b) in enumerate(rvs):
+        ref = O.binary_rv_cost(small.og, small.odisp, small.ocont, rvcase["flux"], rvcase["var"],
+                               *rvcase["comps"], 0.6, a, b)
+        worst = max(worst, abs(got[k] - ref) / ref)
+    print(f"binary RV chi2: max rel {worst:.2e} over {len(rvs)} pairs", flush=True)
+    assert worst < RTOL
+
+
+def test_binary_rv_fit_identity_with_oracle_pso(small, binary, rvcase):
+    pasta = binary["pasta"]
+    wlg = binary["interp"].wavelength
+    P, iters, seed = 16, 15, 3
+    fitter = binary["interp"].get_binary_rv_fitter(binary["d"], wlg, small_cont(pasta, small),
+                                                   pasta.PSOSettings(P, iters), (-150, 150))
+    res = fitter.fit(*rvcase["comps"], 0.6, rvcase["flux"], rvcase["var"], seed=seed)
+    r = fitter._rv(*rvcase["comps"], 0.6, rvcase["flux"], rvcase["var"])
+
+    def gpu_cost(pos):
+        c, st = r.chi2(pos)
+        assert not st.any()
+        return c
+
+    ob = O.binary_bounds(small.og, (0.0, 0.0), (-150.0, 150.0))
+    best, bc, trace, _ = O.pso_run_d(ob, 2, O.pso_settings(P, iters), seed, gpu_cost)
+    assert np.array_equal(res._trace, trace)
+    assert [res.rv1, res.rv2] == best.tolist()
+    swapped = gpu_cost(np.array([[best[1], best[0]]]))[0]
+    assert res.chi2 == min(bc, swapped)
+    print(f"binary RV fit: rv {res.rv1:.3f} {res.rv2:.3f} chi2 {res.chi2:.5f}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def tscase(small, binary):
+    from pasta_amd import synth
+    rvs = np.array([[20.0, -40.0], [-35.0, 55.0], [60.0, -10.0]])
+    fl, vr = [], []
+    for k in range(3):
+        m = O.binary_model_norm(small.og, binary["cgrid"], small.odisp, L1[:4] + [rvs[k, 0]],
+                                L2[:4] + [rvs[k, 1]], np.float32(0.4))
+        f, v = synth.observe(m, small.wl, 120.0, 600 + k)
+        fl.append(f); vr.append(v)
+    return dict(rvs=rvs, fl=fl, vr=vr)
+
+
+def test_binary_ts_chi2_matches_oracle(small, binary, tscase):
+    pasta = binary["pasta"]
+    wlg = binary["interp"].wavelength
+    fitter = binary["interp"].get_timeseries_known_rv_fitter(binary["d"], wlg,
+                                                             small_cont(pasta, small),
+                                                             pasta.PSOSettings(8, 2), (0, 300))
+    labs = np.concatenate([[L1[:4] + L2[:4] + [0.4]], _labels(small.spec, 10, 51)[:, [0, 1, 2, 3, 5, 6, 7, 8, 10]]])
+    got = np.array(fitter.chi2(binary["interp"], binary["cinterp"], tscase["fl"], tscase["vr"],
+                               tscase["rvs"], labs))
+    worst = 0.0
+    for k, lab in enumerate(labs):
+        ref = O.binary_ts_cost(small.og, binary["cgrid"], small.odisp, small.ocont, tscase["fl"],
+                               tscase["vr"], tscase["rvs"], lab)
+        worst = max(worst, abs(got[k] - ref) / ref)
+    print(f"binary time-series chi2: max rel {worst:.2e} over {len(labs)} labels", flush=True)
+    assert worst < RTOL
+
+
+def test_binary_ts_fit_identity_with_oracle_pso(small, binary, tscase):
+    pasta = binary["pasta"]
+    wlg = binary["interp"].wavelength
+    P, iters, seed = 16, 8, 11
+    fitter = binary["interp"].get_timeseries_known_rv_fitter(binary["d"], wlg,
+                                                             small_cont(pasta, small),
+                                                             pasta.PSOSettings(P, iters), (0, 300))
+    res = fitter.fit(binary["interp"], binary["cinterp"], tscase["fl"], tscase["vr"], tscase["rvs"],
+                     seed=seed)
+    t = fitter._ts(binary["interp"], binary["cinterp"], tscase["fl"], tscase["vr"], tscase["rvs"])
+
+    def gpu_cost(pos):
+        c, st = t.chi2(pos)
+        assert not st.any()
+        return c
+
+    ob = O.binary_bounds(small.og, (0.0, 300.0), (0.0, 0.0))
+    best, bc, trace, _ = O.pso_run_d(ob, 9, O.pso_settings(P, iters), seed, gpu_cost)
+    assert np.array_equal(res._trace, trace)
+    assert res.label1.as_list() + res.label2.as_list() + [res.light_ratio] == best.tolist()
+    assert len(res.chis) == 3 and len(res.continuum_params) == 3
+    assert abs(sum(res.chis) - bc) <= 1e-12 * bc
+    d = res.to_dict()
+    assert pasta.BinaryTimeseriesOptimizationResult.from_dict(d).to_dict() == d

@@ -5,7 +5,10 @@ produce_binary_model_norm (src/interpolate.rs:868-890, src/convolve_rv.rs:606-69
 collapses to produce_model for identical stars or a zero light ratio and matches an FP64
 numpy composition (np.interp resampling of the separately pinned components);
 BinaryBounds (src/bounds.rs:188-276) and the 11-parameter PSO keep every particle in
-bounds, never raise the global best and converge on a bowl."""
+bounds, never raise the global best and converge on a bowl.  The RV cost (src/fitting.rs:736-781) matches an FP64 numpy
+composition and collapses to the single-star cost for identical components; the
+time-series cost (src/fitting.rs:954-985) is the sum of the binary costs at each
+spectrum's RVs; the 9- and 2-parameter searches keep their bounds."""
 import numpy as np
 import pytest
 
@@ -182,3 +185,70 @@ def test_binary_pso_bowl(small):
     # same seed, same trace
     best2, bc2, trace2, _ = O.pso_run_binary(b, s, 17, cost)
     assert np.array_equal(trace, trace2)
+
+
+def test_rv_cost_matches_fp64_and_single(small, cgrid):
+    sp = small.spec
+    m1 = O.interpolate_and_convolve(small.og, small.odisp, L1[:4])
+    m2 = O.interpolate_and_convolve(small.og, small.odisp, L2[:4])
+    c1 = cgrid.interpolate_linear(*L1[:3])
+    c2 = cgrid.interpolate_linear(*L2[:3])
+    got = O.binary_rv_cost(small.og, small.odisp, small.ocont, small.flux, small.var, m1, m2, c1,
+                           c2, 0.6, L1[4], L2[4])
+    fidx = (np.log10(small.wl) - sp.log10_first) / sp.log10_step
+    ax = np.arange(sp.n_pix)
+    x1 = fidx + np.log10(1 - L1[4] / 299792.0) / sp.log10_step
+    x2 = fidx + np.log10(1 - L2[4] / 299792.0) / sp.log10_step
+    s1, k1 = np.interp(x1, ax, m1.astype(np.float64)), np.interp(x1, ax, c1.astype(np.float64))
+    s2, k2 = np.interp(x2, ax, m2.astype(np.float64)), np.interp(x2, ax, c2.astype(np.float64))
+    lr = 0.6 * k2.mean() / k1.mean()
+    norm = (s1 * k1 + s2 * k2 * lr) / (k1 + k2 * lr)
+    _, ref = small.ocont.fit(small.flux, small.var, norm)
+    assert abs(got - ref) / ref < 1e-4
+    # identical components at one RV: the single-star cost
+    same = O.binary_rv_cost(small.og, small.odisp, small.ocont, small.flux, small.var, m1, m1, c1,
+                            c1, 0.3, L1[4], L1[4])
+    single, _ = O.cost(small.og, small.odisp, small.ocont, small.flux, small.var, L1)
+    assert abs(same - single) / single < 1e-5
+    with pytest.raises(O.OracleError):
+        O.binary_rv_cost(small.og, small.odisp, small.ocont, small.flux, small.var, m1, m2, c1,
+                         c2, 0.6, 9000.0, 0.0)
+
+
+def test_ts_cost_is_sum_of_binary_costs(small, cgrid):
+    rvs = np.array([[20.0, -40.0], [-35.0, 55.0], [5.0, 5.0]])
+    fl, vr = [], []
+    for k in range(3):
+        m = O.binary_model_norm(small.og, cgrid, small.odisp, L1[:4].tolist() + [rvs[k, 0]],
+                                L2[:4].tolist() + [rvs[k, 1]], np.float32(0.4))
+        f, v = synth.observe(m, small.wl, 120.0, 600 + k)
+        fl.append(f); vr.append(v)
+    lab9 = np.concatenate([L1[:4], L2[:4], [0.4]])
+    got = O.binary_ts_cost(small.og, cgrid, small.odisp, small.ocont, fl, vr, rvs, lab9)
+    ref = 0.0
+    for k in range(3):
+        lab = np.concatenate([L1[:4], [rvs[k, 0]], L2[:4], [rvs[k, 1]], [0.4]])
+        ref += O.binary_cost(small.og, cgrid, small.odisp, small.ocont, fl[k], vr[k], lab)[0]
+    assert got == ref
+
+
+def test_pso_9_and_2_parameters(small):
+    b = O.binary_bounds(small.og, (0.0, 300.0), (-150.0, 150.0))
+    t9 = np.concatenate([L1[:4], L2[:4], [0.4]])
+    lo = np.array([5000, -1, 3.5, 0, 5000, -1, 3.5, 0, 0.0])
+    hi = np.array([7000, 0.5, 5.0, 300, 7000, 0.5, 5.0, 300, 1.0])
+
+    def c9(pos):
+        assert np.all(pos >= lo) and np.all(pos <= hi)
+        return (((pos - t9) / (hi - lo)) ** 2).sum(axis=1)
+
+    best, bc, trace, ne = O.pso_run_d(b, 9, O.pso_settings(60, 120), 4, c9)
+    assert ne == 60 * 121 and np.all(np.diff(trace[:, 0]) <= 0) and bc < 0.05 * trace[0, 0]
+    t2 = np.array([37.0, -81.0])
+
+    def c2(pos):
+        assert np.all(np.abs(pos) <= 150.0)
+        return ((pos - t2) ** 2).sum(axis=1)
+
+    best, bc, trace, ne = O.pso_run_d(b, 2, O.pso_settings(20, 60), 8, c2)
+    assert np.all(np.abs(best - t2) < 1e-2) and trace.shape == (61, 3)
