@@ -37,6 +37,18 @@
  *       entropy (particleswarm.rs:65), so the random stream here is this
  *       build's own counter-based SplitMix64, shared bit-for-bit with the GPU
  *       path.
+ *   Brent root / uncertainty ................... pinned by closed-form roots
+ *       (test_oracle_uncertainty.py).
+ *   linear interpolation (interpolate_linear) .. pinned: trilinear-in-label
+ *       grid reproduced exactly, one-hot at nodes, weights interpolate the
+ *       labels themselves (test_oracle_binary.py).
+ *   binary model / costs ........................ pinned: identical stars or a
+ *       zero light ratio give produce_model / the single-star cost; FP64 numpy
+ *       composition with np.interp resampling; the time-series cost is the sum
+ *       of binary costs.  nalgebra's DVector::mean is restated as a sequential
+ *       f32 fold and one division: unpinned at ULP.
+ *   binary searches (11 / 9 / 2 labels) ......... pinned by the PSO invariants
+ *       above on the binary bounds.
  */
 #include "pasta_oracle.h"
 
