@@ -138,6 +138,47 @@ def run_plain(pasta, synth, interp, wlg, spec, iters=200, P=100, seed=2000):
                 chi2=res.chi2, wl=wl, flux=flux, var=var)
 
 
+def run_binary(pasta, synth, interp, wlg, spec, P=128, reps=6, fit_iters=20, seed=7000):
+    """Binary stars on the full grid: a full-size continuum grid (Planck continua, 8400 x
+    200000 u16 with factors), one observed binary (N = 150,000, R = 85,000, 10 chunks of
+    degree 8); chi2 of P binary label vectors per batch (BinaryCostFunction) timed over
+    `reps` batches of new labels, then one PSO fit of P particles x fit_iters iterations."""
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    cs, cf = synth.make_continuum_grid(spec)
+    cinterp = pasta.InMemInterpolator.from_arrays(cs, spec.labels(), wlg, factors=cf)
+    del cs
+    prep_s = time.perf_counter() - t0
+    wl = synth.observed_wavelengths(spec, 150_000, 600)
+    disp = pasta.FixedResolutionDispersion(wl, R_INSTR, wlg)
+    l1, l2 = [6100.0, -0.2, 4.3, 12.0, 25.0], [5200.0, 0.1, 4.6, 40.0, -45.0]
+    m = interp.produce_binary_model_norm(cinterp, disp, l1, l2, 0.6)
+    flux, var = synth.observe(m, wl, 120.0, seed + 1)
+    cont = pasta.ChunkContinuumFitter(wl, 10, 8, 0.2)
+    fitter = interp.get_binary_fitter(disp, cont, pasta.PSOSettings(P, fit_iters), (0.0, 300.0),
+                                      (-150.0, 150.0))
+    b = fitter._binary(interp, cinterp, flux, var)
+
+    def labels(k):
+        a = synth.random_labels(spec, P, seed + 10 * k)
+        c = synth.random_labels(spec, P, seed + 10 * k + 1)
+        return np.concatenate([a, c, rng.uniform(0.05, 0.95, (P, 1))], axis=1)
+
+    c, st, _ = b.chi2(labels(0))  # warm-up (allocations, code objects)
+    bad = int(np.count_nonzero(st))
+    t0 = time.perf_counter()
+    for k in range(reps):
+        c, st, _ = b.chi2(labels(k + 1))
+        bad += int(np.count_nonzero(st))
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    res = fitter.fit(interp, cinterp, flux, var, seed=1)
+    fit_s = time.perf_counter() - t1
+    return dict(evals_per_s=P * reps / dt, ms_per_batch=dt / reps * 1e3, P=P, reps=reps,
+                failed=bad, fit_latency_s=fit_s, fit_evals=P * (fit_iters + 1) + 1,
+                fit_chi2=res.chi2, continuum_grid_prep_s=prep_s)
+
+
 def serving_requests(pasta, synth, interp, wlg, spec, n, seed=5000):
     """Serving-config requests: lengths lognormal (median 80,000, clipped 20,000-200,000),
     labels uniform in the grid, vsini U(0,300), RV U(-150,150), SNR U(30,300), continuum
@@ -285,6 +326,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-binary", action="store_true")
     ap.add_argument("--serve-rates", default="50,200,500")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="arrival window per rate (requests = rate x seconds, at most 1000)")
@@ -332,6 +374,13 @@ def main():
         plain = run_plain(pasta, synth, interp, wlg, spec, iters=args.plain_iters)
         log(f"[bench] plain: {plain['evals_per_s']:,.0f} evals/s, fit {plain['fit_latency_s']:.2f} s, "
             f"label {np.round(plain['label'], 3).tolist()} true {np.round(plain['true'], 3).tolist()}")
+
+    binary = None
+    if not args.no_binary:
+        binary = run_binary(pasta, synth, interp, wlg, spec)
+        log(f"[bench] binary: {binary['evals_per_s']:,.0f} binary chi2 evals/s "
+            f"({binary['ms_per_batch']:.1f} ms per batch of {binary['P']}), fit of "
+            f"{binary['fit_evals']} evaluations {binary['fit_latency_s']:.2f} s")
 
     serving = []
     if not args.no_serving:
@@ -419,6 +468,14 @@ def main():
             "one_core_sample": f"{cpu['n1']} evaluations on 1 thread",
         }
         out["parity_full_size"] = {"max_rel_chi2": cpu["parity_max_rel"], "n": cpu["n_parity"]}
+    if binary is not None:
+        out["binary"] = {
+            "workload": f"binary stars (BinaryCostFunction): full grid + full continuum grid "
+                        f"(8400 x 200000 u16 each), N=150000, R=85000, 10 chunks deg 8, "
+                        f"{binary['P']} label vectors per batch x {binary['reps']} batches; fit "
+                        f"{binary['P']} particles x 20 iterations",
+            **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in binary.items()},
+        }
     if serving:
         out["serving"] = {
             "workload": f"Poisson arrivals, lognormal N (median 80k, 20k-200k), continuum modes "

@@ -834,14 +834,14 @@ class _Binary:
                                ptr(st, C.c_int))
         return out, st
 
-    def chi2(self, labels11):
+    def chi2(self, labels11, params=False):
         lab = f64(labels11).reshape(-1, 11)
         out = np.zeros(len(lab))
         st = np.zeros(len(lab), np.int32)
-        params = np.zeros((len(lab), max(1, self.n_params)), np.float32)
+        prm = np.zeros((len(lab), max(1, self.n_params)), np.float32) if params else None
         lib.pasta_binary_eval_chi2(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_double),
-                                   ptr(st, C.c_int), ptr(params, C.c_float))
-        return out, st, params
+                                   ptr(st, C.c_int), None if prm is None else ptr(prm, C.c_float))
+        return out, st, prm
 
 
 class _BinaryTS:

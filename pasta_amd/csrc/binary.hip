@@ -10,11 +10,11 @@
 //     BinaryTimeriesKnownRVFitter              src/fitting.rs:882-1078
 //   BinaryBounds, BinaryBoundsWithoutRV,
 //     BinaryRVBounds                           src/bounds.rs:188-372
-// One evaluation of P binary labels runs on one stream: the 2P component models on the
-// synthetic grid (k_prep ... k_model in synthetic-grid mode), the 2P linear continua of
-// the continuum grid and their means (k_interp_lin, k_seq_mean), the combination at the
-// observed pixels (k_binary_combine), then the continuum fit and chi2 of the P models
-// (k_moments ... k_final), with one copy back at the end.
+// One evaluation of P binary labels: the 2P component models on the synthetic grid
+// (k_prep ... k_model in synthetic-grid mode) and, on a second stream beside them, the 2P
+// linear continua of the continuum grid and their means (k_interp_lin, k_seq_mean); then
+// the combination at the observed pixels (k_binary_combine) and the continuum fit and chi2
+// of the P models (k_moments ... k_final), with one copy back at the end.
 #include "host.h"
 
 namespace pasta {
@@ -22,11 +22,12 @@ hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const 
                        const int4* bchi2, const ContClass* cls, int n_cls,
                        const int2* solve_items, int n_solve, const float* model, double* mpart,
                        float* pf, double* cpart, double* chi2, int* status, hipStream_t st);
-hipError_t launch_binary(const DevGrid& G, const DevGrid& CG, const DevStar* stars,
-                         DevEval* oevals, int n_obs_evals, const double* labels11,
-                         const DevEval* cevals, const double* clabels, int ncomp,
-                         const float* comp, int* cstat, float* cont, float* cmean, int n_obs_max,
-                         float* model, hipStream_t st);
+hipError_t launch_binary_cont(const DevGrid& CG, const double* clabels, int ncomp, int* cstat,
+                              float* cont, float* cmean, hipStream_t st);
+hipError_t launch_binary(const DevGrid& G, const DevStar* stars, DevEval* oevals,
+                         int n_obs_evals, const double* labels11, const DevEval* cevals, int ncomp,
+                         const float* comp, const int* cstat, const float* cont,
+                         const float* cmean, int n_obs_max, float* model, hipStream_t st);
 hipError_t launch_binary_rv(const DevGrid& G, const DevStar* stars, DevEval* oevals, int n,
                             const double* rv, const float* model1, const float* model2,
                             const float* cont1, const float* cont2, float light_ratio, int n_obs,
@@ -52,7 +53,16 @@ struct pasta_binary {
     DBuf<float> d_cont, d_cmean;
     DBuf<int> d_cstat;
     std::vector<double> h_comp, h_lab11;
+    // the continua and their (sequential) means run on a second stream beside the component
+    // models: fork after the label upload, join before the combination
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::mutex mu;
+    ~pasta_binary() {
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
+    }
 };
 
 static int binary_prepare(pasta_binary* b, int P) {
@@ -70,6 +80,9 @@ static int binary_prepare(pasta_binary* b, int P) {
     CHK(b->d_cstat.alloc((size_t)2 * P));
     b->h_comp.assign((size_t)10 * P, 0.0);
     b->h_lab11.assign((size_t)11 * P * b->S, 0.0);
+    if (!b->side) HIPCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    if (!b->fork) HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    if (!b->join) HIPCHK(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
     b->P = P;
     return PASTA_OK;
 }
@@ -107,14 +120,20 @@ static int binary_launch(pasta_binary* b, const double* labels, int P, bool chi2
     HIPCHK(hipMemcpyAsync(b->d_lab11.p, b->h_lab11.data(), sizeof(double) * 11 * P * b->S,
                           hipMemcpyHostToDevice, st));
     DevGrid G = b->g->dev(), CG = b->cg->dev();
+    HIPCHK(hipEventRecord(b->fork, st));
+    HIPCHK(hipStreamWaitEvent(b->side, b->fork, 0));
+    HIPCHK(launch_binary_cont(CG, Bc.d_labels.p, P, b->d_cstat.p, b->d_cont.p, b->d_cmean.p,
+                              b->side));
+    HIPCHK(hipEventRecord(b->join, b->side));
     HIPCHK(launch_eval(G, Bc.d_stars.p, Bc.d_evals.p, Bc.n_evals, Bc.d_labels.p, Bc.d_rot.p,
                        Bc.d_tbound.p, Bc.tb_stride, Bc.d_hist.p, Bc.d_perm.p, Bc.d_pair_eq.p,
                        Bc.d_items.p, Bc.d_nitems.p, Bc.n_bm, Bc.d_bmom.p, Bc.d_bchi2.p,
                        Bc.cls.data(), 0, Bc.d_solve.p, 0, Bc.d_model.p, Bc.d_mpart.p, Bc.d_pf.p,
                        Bc.d_cpart.p, Bc.d_chi2.p, Bc.d_status.p, 1, nullptr, nullptr, st));
-    HIPCHK(launch_binary(G, CG, Bo.d_stars.p, Bo.d_evals.p, P * b->S, b->d_lab11.p,
-                         Bc.d_evals.p, Bc.d_labels.p, P, Bc.d_model.p, b->d_cstat.p, b->d_cont.p,
-                         b->d_cmean.p, b->N, Bo.d_model.p, st));
+    HIPCHK(hipStreamWaitEvent(st, b->join, 0));
+    HIPCHK(launch_binary(G, Bo.d_stars.p, Bo.d_evals.p, P * b->S, b->d_lab11.p, Bc.d_evals.p, P,
+                         Bc.d_model.p, b->d_cstat.p, b->d_cont.p, b->d_cmean.p, b->N,
+                         Bo.d_model.p, st));
     if (chi2)
         HIPCHK(launch_cont(Bo.d_stars.p, Bo.d_evals.p, Bo.n_evals, Bo.d_bmom.p, Bo.d_bchi2.p,
                            Bo.cls.data(), (int)Bo.cls.size(), Bo.d_solve.p, Bo.n_solve,
@@ -255,16 +274,13 @@ static int binary_chi2(pasta_binary* b, const double* labels, int P, double* chi
         for (int e = 0; e < n; e++) per_spectrum[e] = st[e] ? NAN : c[e];
     const int np = b->n_params;
     if (params && np > 0) {
-        const std::vector<DevEval>& he = Bo.h_evals;
-        for (int e = 0; e < n; e++) {
-            float* o = params + (size_t)e * np;
-            if (st[e] || b->cont_kind == CONT_FIXED) {
-                for (int k = 0; k < np; k++) o[k] = st[e] ? NAN : 0.0f;
-            } else {
-                HIPCHK(hipMemcpy(o, Bo.d_pf.p + he[e].pf_off, sizeof(float) * np,
-                                 hipMemcpyDeviceToHost));
-            }
-        }
+        // every record has the same parameter count, so the parameters are one block
+        if (b->cont_kind != CONT_FIXED)
+            HIPCHK(hipMemcpy(params, Bo.d_pf.p, sizeof(float) * (size_t)n * np,
+                             hipMemcpyDeviceToHost));
+        for (int e = 0; e < n; e++)
+            if (st[e] || b->cont_kind == CONT_FIXED)
+                for (int k = 0; k < np; k++) params[(size_t)e * np + k] = st[e] ? NAN : 0.0f;
     }
     if (bad) return fail(PASTA_ERR_MODEL, status_text(bad));
     return PASTA_OK;

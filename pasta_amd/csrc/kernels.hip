@@ -1645,22 +1645,51 @@ __global__ __launch_bounds__(NT) void k_interp_lin(DevGrid G, const double* __re
 }
 
 // DVector::mean of each row of `in` (W values): a sequential f32 fold in pixel order and
-// one division; one wavefront per row stages 1024 values at a time in LDS and lane 0
-// adds them in order.  Rows with a non-zero status are skipped.
-__global__ __launch_bounds__(64) void k_seq_mean(const float* __restrict__ in, int W,
-                                                 const int* __restrict__ status,
-                                                 float* __restrict__ mean) {
-    __shared__ float buf[1024];
+// one division.  Two wavefronts per row: wavefront 1 stages chunks of 4,096 values into a
+// double-buffered LDS ring while lane 0 of wavefront 0 adds the previous chunk in order,
+// 64 values per step after one batch of LDS reads.  Rows with a non-zero status are
+// skipped.
+constexpr int SEQ_CHUNK = 4096;
+__global__ __launch_bounds__(128) void k_seq_mean(const float* __restrict__ in, int W,
+                                                  const int* __restrict__ status,
+                                                  float* __restrict__ mean) {
+    __shared__ float4 ring[2][SEQ_CHUNK / 4];
     const int e = blockIdx.x;
     if (status[e]) return;
     const float* r = in + (size_t)e * W;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nch = (W + SEQ_CHUNK - 1) / SEQ_CHUNK;
     float s = 0.0f;
-    for (int x0 = 0; x0 < W; x0 += 1024) {
-        for (int k = threadIdx.x; k < 1024; k += 64) buf[k] = x0 + k < W ? r[x0 + k] : 0.0f;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int m = W - x0 < 1024 ? W - x0 : 1024;
-            for (int k = 0; k < m; k++) s = s + buf[k];
+    // chunk c is staged in round c and summed in round c + 1
+    for (int c = 0; c <= nch; c++) {
+        if (wv == 1 && c < nch) {  // all 64 loads of a lane in flight before the stores
+            float* dst = reinterpret_cast<float*>(ring[c & 1]);
+            const int x0 = c * SEQ_CHUNK + lane;
+            float v[SEQ_CHUNK / 64];
+#pragma unroll
+            for (int k = 0; k < SEQ_CHUNK / 64; k++) v[k] = x0 + 64 * k < W ? r[x0 + 64 * k] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < SEQ_CHUNK / 64; k++) dst[lane + 64 * k] = v[k];
+        }
+        if (wv == 0 && lane == 0 && c > 0) {
+            const float4* q = ring[(c - 1) & 1];
+            const int m = W - (c - 1) * SEQ_CHUNK < SEQ_CHUNK ? W - (c - 1) * SEQ_CHUNK : SEQ_CHUNK;
+            // 64 values per step: 16 LDS reads issued together, then 64 adds in order
+            const int m64 = m >> 6;
+            for (int k = 0; k < m64; k++) {
+                float4 a[16];
+#pragma unroll
+                for (int t = 0; t < 16; t++) a[t] = q[16 * k + t];
+#pragma unroll
+                for (int t = 0; t < 16; t++) {
+                    s = s + a[t].x;
+                    s = s + a[t].y;
+                    s = s + a[t].z;
+                    s = s + a[t].w;
+                }
+            }
+            const float* qf = reinterpret_cast<const float*>(q);
+            for (int k = m64 << 6; k < m; k++) s = s + qf[k];
         }
         __syncthreads();
     }
@@ -1721,17 +1750,23 @@ __global__ __launch_bounds__(NT) void k_binary_combine(DevGrid G, const DevStar*
     model[oevals[b].m_off + i] = (a1 * k1 + lr * a2 * k2) / (k1 + lr * k2);
 }
 
-// binary models of n_obs_evals evaluations (observed records oevals, labels11) from the
-// components of ncomp label pairs (records cevals, labels clabels with stride 5)
-hipError_t launch_binary(const DevGrid& G, const DevGrid& CG, const DevStar* stars,
-                         DevEval* oevals, int n_obs_evals, const double* labels11,
-                         const DevEval* cevals, const double* clabels, int ncomp,
-                         const float* comp, int* cstat, float* cont, float* cmean, int n_obs_max,
-                         float* model, hipStream_t st) {
-    if (n_obs_evals <= 0 || ncomp <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_interp_lin, dim3((G.W + 8 * NT - 1) / (8 * NT), 2 * ncomp), dim3(NT), 0,
+// the linear continua of ncomp label pairs (labels clabels with stride 5) and their means
+hipError_t launch_binary_cont(const DevGrid& CG, const double* clabels, int ncomp, int* cstat,
+                              float* cont, float* cmean, hipStream_t st) {
+    if (ncomp <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_interp_lin, dim3((CG.W + 8 * NT - 1) / (8 * NT), 2 * ncomp), dim3(NT), 0,
                        st, CG, clabels, 5, cont, cstat);
-    hipLaunchKernelGGL(k_seq_mean, dim3(2 * ncomp), dim3(64), 0, st, cont, G.W, cstat, cmean);
+    hipLaunchKernelGGL(k_seq_mean, dim3(2 * ncomp), dim3(128), 0, st, cont, CG.W, cstat, cmean);
+    return hipGetLastError();
+}
+
+// binary models of n_obs_evals evaluations (observed records oevals, labels11) from the
+// components of ncomp label pairs (records cevals) and their continua
+hipError_t launch_binary(const DevGrid& G, const DevStar* stars, DevEval* oevals,
+                         int n_obs_evals, const double* labels11, const DevEval* cevals, int ncomp,
+                         const float* comp, const int* cstat, const float* cont,
+                         const float* cmean, int n_obs_max, float* model, hipStream_t st) {
+    if (n_obs_evals <= 0 || ncomp <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_binary_combine, dim3((n_obs_max + NT - 1) / NT, n_obs_evals), dim3(NT), 0,
                        st, G, stars, oevals, labels11, cevals, comp, cstat, cont, cmean, ncomp,
                        model);
@@ -1818,7 +1853,7 @@ hipError_t launch_binary_rv(const DevGrid& G, const DevStar* stars, DevEval* oev
     const dim3 grid((n_obs + NT - 1) / NT, n);
     hipLaunchKernelGGL(k_rv_shift, grid, dim3(NT), 0, st, G, stars, oevals, rv, cont1, cont2, kc,
                        kst);
-    hipLaunchKernelGGL(k_seq_mean, dim3(2 * n), dim3(64), 0, st, kc, n_obs, kst, kmean);
+    hipLaunchKernelGGL(k_seq_mean, dim3(2 * n), dim3(128), 0, st, kc, n_obs, kst, kmean);
     hipLaunchKernelGGL(k_rv_combine, grid, dim3(NT), 0, st, G, stars, oevals, rv, model1, model2,
                        kc, kmean, light_ratio, model);
     return hipGetLastError();

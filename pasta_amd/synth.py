@@ -113,14 +113,18 @@ def make_continuum_grid(spec, labels=None):
         labels = spec.labels()
     wl = spec.synth_wl()
     hc_k = 1.4387769e8  # Angstrom K
-    t = labels[:, 0:1]
-    b = wl[None, :] ** -5.0 / np.expm1(hc_k / (wl[None, :] * t))
-    b = b * 10.0 ** (0.2 * labels[:, 1:2]) * (1.0 + 0.05 * (labels[:, 2:3] - 4.0))
-    b = b / b.max() * 1e6
-    bmax = b.max(axis=1)
-    u16 = (b / bmax[:, None] * 65535.0).astype(np.uint16)
+    # the stored rows depend on Teff only (flux / max); [M/H] and logg scale the factors
+    teffs, ti = np.unique(labels[:, 0], return_inverse=True)
+    rows = np.empty((len(teffs), len(wl)), np.uint16)
+    pmax = np.empty(len(teffs))
+    for k, t in enumerate(teffs):
+        b = wl ** -5.0 / np.expm1(hc_k / (wl * t))
+        pmax[k] = b.max()
+        rows[k] = (b / pmax[k] * 65535.0).astype(np.uint16)
+    scale = pmax[ti] * 10.0 ** (0.2 * labels[:, 1]) * (1.0 + 0.05 * (labels[:, 2] - 4.0))
+    bmax = scale / scale.max() * 1e6
     factors = (bmax.astype(np.float32) / np.float32(65535.0)).astype(np.float32)
-    return u16, factors
+    return rows[ti], factors
 
 
 def observed_wavelengths(spec, n_obs, margin_px):
