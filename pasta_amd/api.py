@@ -22,9 +22,9 @@ Every computation runs in the HIP library; there is no CPU fallback.  Errors
 raise ValueError like PyO3's anyhow conversion.
 """
 import ctypes as C
+import io
 import json
 import os
-import re
 
 import numpy as np
 
@@ -364,14 +364,71 @@ def ConstantContinuum():
 
 
 # ---------------------------------------------------------------- interpolator
-_FNAME = re.compile(r"^l([mp])(\d{4})_(\d{5})_(\d{4})\.npy$")
+def label_from_filename(filename):
+    """(Teff, [M/H], logg) of a grid file name such as lp0020_08000_0430.npy
+    (label_from_filename, src/model_fetchers.rs:23-38): three '_'-separated fields of 6, 5
+    and 4 characters before the first '.'; [M/H] is negative for an 'lm' prefix."""
+    parts = filename.split(".")[0].split("_")
+    if len(parts) != 3 or len(parts[0]) != 6 or len(parts[1]) != 5 or len(parts[2]) != 4:
+        raise ValueError(f"Invalid filename: {filename}")
+    sign = -1.0 if parts[0].startswith("lm") else 1.0
+    try:
+        return (float(int(parts[1])), sign * float(int(parts[0][2:])) / 100.0,
+                float(int(parts[2])) / 100.0)
+    except ValueError:
+        raise ValueError(f"Invalid filename: {filename}") from None
 
 
-def _read_npy_u16(path):
-    a = np.load(path, allow_pickle=False)
+def _npy_u16(buf, what):
+    a = np.load(io.BytesIO(buf) if isinstance(buf, (bytes, bytearray)) else buf, allow_pickle=False)
     if a.dtype != np.uint16:
-        raise ValueError(f"{path}: expected uint16")
+        raise ValueError(f"{what}: expected uint16")
     return a.ravel()
+
+
+def read_spectrum(a, includes_factor):
+    """read_spectrum (src/model_fetchers.rs:48-67): with includes_factor the first two
+    uint16 values are the little-endian f32 factor (max), returned divided by 65535."""
+    if not includes_factor:
+        return a, None
+    fac = np.frombuffer(a[:2].astype("<u2").tobytes(), dtype="<f4")[0]
+    if fac < 0:
+        raise ValueError(f"Negative factor: {fac}")
+    return a[2:], np.float32(fac) / np.float32(65535.0)
+
+
+def _read_grid_dir(dir, includes_factor):
+    labels, spectra, factors = [], [], []
+    for fn in sorted(os.listdir(dir)):
+        labels.append(label_from_filename(fn))
+        a, f = read_spectrum(_npy_u16(os.path.join(dir, fn), fn), includes_factor)
+        spectra.append(a)
+        factors.append(f)
+    return labels, spectra, factors
+
+
+def _read_grid_tar_zstd(file_path, includes_factor):
+    """InMemFetcher::from_tar_zstd (src/model_fetchers.rs:168-225): a zstd-compressed tar
+    stream of grid .npy files; members whose file name does not start with 'l' are
+    skipped.  Decompressed as a stream (pyarrow's zstd codec), one member at a time."""
+    import tarfile
+
+    import pyarrow as pa
+
+    labels, spectra, factors = [], [], []
+    with pa.input_stream(file_path, compression="zstd") as raw:
+        with tarfile.open(fileobj=raw, mode="r|") as tar:
+            for m in tar:
+                name = os.path.basename(m.name)
+                if not m.isfile() or not name.startswith("l"):
+                    continue
+                labels.append(label_from_filename(name))
+                a, f = read_spectrum(_npy_u16(tar.extractfile(m).read(), name), includes_factor)
+                spectra.append(a)
+                factors.append(f)
+    if not labels:
+        raise ValueError(f"no grid spectra in {file_path}")
+    return labels, spectra, factors
 
 
 class InMemInterpolator:
@@ -384,22 +441,7 @@ class InMemInterpolator:
 
     def __init__(self, dir=None, includes_factor=False, wavelength=None, *, _arrays=None):
         if _arrays is None:
-            labels, spectra, factors = [], [], []
-            for fn in sorted(os.listdir(dir)):
-                mt = _FNAME.match(fn)
-                if not mt:
-                    raise ValueError(f"Invalid filename: {fn}")
-                sign = -1.0 if mt.group(1) == "m" else 1.0
-                labels.append((float(mt.group(3)), sign * float(mt.group(2)) / 100.0,
-                               float(mt.group(4)) / 100.0))
-                a = _read_npy_u16(os.path.join(dir, fn))
-                if includes_factor:
-                    fac = np.frombuffer(a[:2].tobytes(), dtype="<f4")[0]
-                    if fac < 0:
-                        raise ValueError(f"Negative factor: {fac}")
-                    factors.append(np.float32(fac) / np.float32(65535.0))
-                    a = a[2:]
-                spectra.append(a)
+            labels, spectra, factors = _read_grid_dir(dir, includes_factor)
             spectra = np.stack(spectra)
             labels = np.array(labels)
             factors = np.array(factors, np.float32) if includes_factor else None
@@ -423,6 +465,17 @@ class InMemInterpolator:
         shape = (C.c_int * 5)()
         lib.pasta_grid_shape(self._h, shape)
         self.n_teff, self.n_m, self.n_logg, self.n_models, self.n_pix = list(shape)
+
+    @staticmethod
+    def from_tar_zstd(file_path, includes_factor, wavelength):
+        """The grid from a .tar.zst archive of .npy files (src/lib.rs:1896-1907)."""
+        labels, spectra, factors = _read_grid_tar_zstd(file_path, includes_factor)
+        n = {len(a) for a in spectra}
+        if len(n) != 1:
+            raise ValueError("grid spectra of different lengths in the archive")
+        return InMemInterpolator(wavelength=wavelength, _arrays=(
+            np.stack(spectra), np.array(labels),
+            np.array(factors, np.float32) if includes_factor else None))
 
     @staticmethod
     def from_arrays(spectra, labels, wavelength, factors=None):
