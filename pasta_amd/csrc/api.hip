@@ -393,9 +393,15 @@ int pasta_cont_create_chunk(const double* wl, int n, int n_chunks, int p_order, 
         if (e - s < p1) return fail(PASTA_ERR_ARG, "chunk shorter than the polynomial order");
         c->lohi[2 * ch] = wl[s];
         c->lohi[2 * ch + 1] = wl[e - 1];
+        // design rows for the device: p1 rows from pixel s & ~3, stride rounded up to 4
+        const int b0 = s & ~3, stride = (e - b0 + 3) & ~3;
+        const size_t off = c->cdesign.size();
+        c->cdoff.push_back((int)off);
+        c->cdesign.resize(off + (size_t)p1 * stride, 0.0f);
         std::vector<double> G((size_t)p1 * p1, 0.0);
         for (int i = s; i < e; i++) {
             host_design_row(chunk_xhat(wl[i], wl[s], wl[e - 1]), p1, a.data());
+            for (int k = 0; k < p1; k++) c->cdesign[off + (size_t)k * stride + (i - b0)] = a[k];
             for (int k = 0; k < p1; k++)
                 for (int l = 0; l <= k; l++) G[k * p1 + l] += (double)a[k] * (double)a[l];
         }
@@ -618,6 +624,8 @@ int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, const fl
         CHK(dev_copy(s.get(), c->chol.data(), c->chol.size(), &D.chol));
         CHK(dev_copy(s.get(), c->fac.data(), c->fac.size(), &D.fac));
         CHK(dev_copy(s.get(), c->design_q.data(), c->design_q.size(), &D.design));
+        CHK(dev_copy(s.get(), c->cdesign.data(), c->cdesign.size(), &D.cdesign));
+        CHK(dev_copy(s.get(), c->cdoff.data(), c->cdoff.size(), &D.cdoff));
         CHK(dev_copy(s.get(), c->fixedc.data(), c->fixedc.size(), &D.fixedc));
         CHK(dev_copy(s.get(), c->mblocks.data(), c->mblocks.size(), &D.mblocks));
         CHK(dev_copy(s.get(), c->mb_off.data(), c->mb_off.size(), &D.mb_off));
@@ -702,7 +710,8 @@ int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
             DevEval& ev = B.h_evals[e];
             ev.star = s;
             ev.active = 1;
-            if (D.cont_kind == CONT_LINEAR) mo = (mo + 3) & ~(int64_t)3;  // float4 reads
+            if (D.cont_kind == CONT_LINEAR || D.cont_kind == CONT_CHUNK)
+                mo = (mo + 3) & ~(int64_t)3;  // float4 reads of the matrix-core moments
             ev.m_off = mo;
             mo += outlen;
             ev.mp_off = mp;

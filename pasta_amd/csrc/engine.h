@@ -32,14 +32,10 @@ struct KVariant {
     int pm, e, f;
 };
 __host__ __device__ constexpr KVariant mom_variant(int kind, int p1) {
-    return kind == CONT_LINEAR   ? KVariant{p1 <= 16 ? 16 : 32, 64, 0}  // matrix-core kernel
-           : kind == CONT_CONSTANT ? KVariant{1, 8, 0}
-           : p1 == 5             ? KVariant{10, 4, 5}
-           : p1 == 9             ? KVariant{10, 4, 9}
-           : p1 == 11            ? KVariant{16, 2, 11}
-           : p1 <= 10            ? KVariant{10, 4, 0}
-           : p1 <= 16            ? KVariant{16, 2, 0}
-                                 : KVariant{32, 1, 0};
+    // chunk and linear: the matrix-core kernel, 64 evaluations per workgroup; constant: the
+    // vector kernel, 8 evaluations per workgroup
+    return (kind == CONT_LINEAR || kind == CONT_CHUNK) ? KVariant{p1 <= 16 ? 16 : 32, 64, 0}
+                                                       : KVariant{1, 8, 0};
 }
 __host__ __device__ constexpr KVariant chi2_variant(int kind, int p1) {
     return (kind == CONT_CONSTANT || kind == CONT_FIXED) ? KVariant{1, 8, 0}
@@ -83,6 +79,9 @@ struct DevStar {
     const float* design;      // linear: PT = 16 (p1 <= 16) or 32 columns, column k at
                               // k * N4 (N4 = N rounded up to 4), zero padded
     const float* fixedc;      // fixed continuum
+    const float* cdesign;     // chunk: per chunk c, p1 rows of f32(powi(x^, k)) from pixel
+                              // startstop[2c] & ~3, row stride rounded up to 4, zero padded
+    const int* cdoff;         // chunk c's design at cdesign + cdoff[c]
     const int4* mblocks;      // moment blocks (chunk, i0, i1, 0)
     const int* mb_off;        // chunk c owns mblocks [mb_off[c], mb_off[c+1])
     const int4* cblocks;      // chi2 blocks (kind, chunk, i0, i1); kind 1 = overlap

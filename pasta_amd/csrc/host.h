@@ -133,6 +133,8 @@ struct pasta_cont {
     std::vector<int> startstop;
     std::vector<double> lohi, chol, wl;
     std::vector<float> fac, design_q, design_rm, fixedc;
+    std::vector<float> cdesign;  // chunk design rows for the matrix-core moments
+    std::vector<int> cdoff;
     std::vector<int4> mblocks, cblocks;
     std::vector<int> mb_off;
     double chi2_div = 1.0;

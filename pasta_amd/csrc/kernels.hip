@@ -912,11 +912,8 @@ __device__ __forceinline__ float dot_na(const float (&a)[PM], const float* b, in
     return r;
 }
 
-// Continuum moments of E evaluations of one star over one block of observed pixels:
-// block descriptor (first evaluation, count, moment block, 0).  The design row of a
-// pixel is formed once and used for every evaluation of the group.  P1F > 0 fixes the
-// parameter count per chunk at compile time (the launch covers one class of stars with
-// P1F parameters), so the per-parameter guards fold away.
+// Constant-continuum moment (the sum of f/m) of E evaluations of one star over one block
+// of observed pixels: block descriptor (first evaluation, count, moment block, 0).
 template <int PM, int E, int P1F>
 __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ stars,
                                                 const DevEval* __restrict__ evals,
@@ -941,42 +938,7 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
 #pragma unroll
     for (int k = 0; k < E * PM; k++) acc[k] = 0.0;
     int np = 1;
-    if (S.cont_kind == CONT_CHUNK) {
-        np = P1F > 0 ? P1F : S.p1;
-        const double lo = S.chunk_lohi[2 * mb.x], hi = S.chunk_lohi[2 * mb.x + 1];
-        // U pixels per thread and round: every load of a round is issued before the
-        // first is used
-        constexpr int U = 4;
-        for (int i0 = mb.y + threadIdx.x; i0 < mb.z; i0 += U * NT) {
-            double wl[U];
-            float fl[U], mv[U][E];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int i = i0 + u * NT;
-                const bool in = i < mb.z;
-                wl[u] = in ? S.wl[i] : lo;
-                fl[u] = in ? S.flux[i] : 0.0f;
-#pragma unroll
-                for (int e = 0; e < E; e++) mv[u][e] = (in && ok[e]) ? m[e][i] : 1.0f;
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (i0 + u * NT >= mb.z) break;
-                float a[PM];
-                design_row<PM>(chunk_xhat(wl[u], lo, hi), np, a);
-                double ad[PM];
-#pragma unroll
-                for (int k = 0; k < PM; k++) ad[k] = (double)a[k];
-#pragma unroll
-                for (int e = 0; e < E; e++) {  // straight-line: an absent evaluation's sums are dropped
-                    const double y = (double)(fl[u] * __builtin_amdgcn_rcpf(mv[u][e]));
-#pragma unroll
-                    for (int k = 0; k < PM; k++)
-                        if (k < np) acc[e * PM + k] = fma(ad[k], y, acc[e * PM + k]);
-                }
-            }
-        }
-    } else {  // constant: sum of f/m
+    {  // constant: sum of f/m
         for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
             const float f = S.flux[i];
 #pragma unroll
@@ -995,21 +957,25 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
     }
 }
 
-// Linear-continuum moments Aᵀ(f/m) on f64 matrix cores (v_mfma_f64_16x16x4f64): the
-// product of the N x p1 design with the N x E ratios y = f/m of E evaluations of one star.
-// A workgroup covers one block of observed pixels and up to 64 evaluations, 16 per
-// wavefront; a wavefront takes 16 pixels per round as 4 k-steps of 4.  Lane (j, g) =
-// (lane % 16, lane / 16) holds in k-step q the design value of parameter 16t + j and the
-// ratio of evaluation j, both at pixel i0 + 4g + q.  The design holds PT = 16 * NTL
-// columns of N4 = N rounded up to 4 values (zero where k >= p1 or i >= N), so a lane reads
-// its 4 pixels of a column as one float4.  Products of two f32 values are exact in f64 and
-// accumulate in f64, so the sums agree with a sequential f64 sum to f64 rounding.
+// Continuum moments Aᵀ(f/m) of the chunk and linear continua on f64 matrix cores
+// (v_mfma_f64_16x16x4f64): the product of a block's design rows (n x p1, f32) with the
+// n x E ratios y = f/m of E <= 64 evaluations of one star.  A workgroup covers one moment
+// block (a chunk's pixels for the chunk continuum) and 16 evaluations per wavefront; a
+// wavefront takes 16 pixels per round as 4 k-steps of 4.  Lane (j, g) = (lane % 16,
+// lane / 16) holds in k-step q the design value of parameter 16t + j and the ratio of
+// evaluation j, both at pixel a0 + 4g + q, with a0 a multiple of 4 so that the model, flux
+// and design quads are float4 reads; pixels outside the block contribute zero.  The
+// design sits in rows of `stride` values starting at pixel b0 (a multiple of 4): the
+// chunk continuum stores the chunk's f32(powi(x^, k)) rows (b0 = the chunk start rounded
+// down to 4), the linear continuum PT = 16 * NTL zero-padded columns (b0 = 0).  Products
+// of two f32 values are exact in f64 and accumulate in f64, so the sums agree with a
+// sequential f64 sum to f64 rounding.
 template <int NTL>
-__global__ __launch_bounds__(NT) void k_moments_lin(const DevStar* __restrict__ stars,
-                                                    const DevEval* __restrict__ evals,
-                                                    const int4* __restrict__ blocks,
-                                                    const float* __restrict__ model,
-                                                    double* __restrict__ mpart) {
+__global__ __launch_bounds__(NT) void k_moments_mfma(const DevStar* __restrict__ stars,
+                                                     const DevEval* __restrict__ evals,
+                                                     const int4* __restrict__ blocks,
+                                                     const float* __restrict__ model,
+                                                     double* __restrict__ mpart) {
     typedef double d4 __attribute__((ext_vector_type(4)));
     const int4 bd = blocks[blockIdx.x];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1020,53 +986,89 @@ __global__ __launch_bounds__(NT) void k_moments_lin(const DevStar* __restrict__ 
     const int j = lane & 15, g = lane >> 4, e = e0 + j;
     const bool ok = e < bd.y && evals[bd.x + e].status == ST_OK;
     const float* m = model + (ok ? evals[bd.x + e].m_off : 0);
-    const size_t n4 = (size_t)(S.N + 3) & ~(size_t)3;
-    d4 acc[2][NTL];
+    const int np = S.p1;
+    const float* dbase;
+    int stride, b0;
+    if (S.cont_kind == CONT_CHUNK) {
+        b0 = S.startstop[2 * mb.x] & ~3;
+        stride = (S.startstop[2 * mb.x + 1] - b0 + 3) & ~3;
+        dbase = S.cdesign + S.cdoff[mb.x];
+    } else {
+        b0 = 0;
+        stride = (S.N + 3) & ~3;
+        dbase = S.design;
+    }
+    d4 acc[4][NTL];  // one accumulator per k-step: independent matrix-core chains
 #pragma unroll
-    for (int h = 0; h < 2; h++)
+    for (int h = 0; h < 4; h++)
 #pragma unroll
         for (int t = 0; t < NTL; t++) acc[h][t] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int i0 = mb.y; i0 < mb.z; i0 += 16) {
-        const int i = i0 + 4 * g;
-        float y[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        float4 a[NTL];
-#pragma unroll
-        for (int t = 0; t < NTL; t++) a[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (i < mb.z) {
-#pragma unroll
-            for (int t = 0; t < NTL; t++)
-                a[t] = *reinterpret_cast<const float4*>(S.design + (size_t)(16 * t + j) * n4 + i);
-            if (i + 4 <= mb.z) {
-                const float4 f = *reinterpret_cast<const float4*>(S.flux + i);
-                if (ok) {
-                    const float4 mv = *reinterpret_cast<const float4*>(m + i);
-                    y[0] = f.x * __builtin_amdgcn_rcpf(mv.x);
-                    y[1] = f.y * __builtin_amdgcn_rcpf(mv.y);
-                    y[2] = f.z * __builtin_amdgcn_rcpf(mv.z);
-                    y[3] = f.w * __builtin_amdgcn_rcpf(mv.w);
-                }
-            } else if (ok) {
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    if (i + q < mb.z) y[q] = S.flux[i + q] * __builtin_amdgcn_rcpf(m[i + q]);
-            }
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) const v4 gf4;  // global-memory quads
+    // design rows beyond p1 only feed output rows that are never stored, and an absent
+    // evaluation only its own output column: such lanes read any valid row
+    const float* drow = dbase + (size_t)min(j, np - 1) * stride - b0;
+    const float* drow2 = dbase + (size_t)min(16 + j, np - 1) * stride - b0;
+    auto round = [&](int a, const float (&y)[4]) {
+        const v4 d0 = *(gf4*)(drow + a);
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.x, (double)y[0], acc[0][0], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.y, (double)y[1], acc[1][0], 0, 0, 0);
+        acc[2][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.z, (double)y[2], acc[2][0], 0, 0, 0);
+        acc[3][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.w, (double)y[3], acc[3][0], 0, 0, 0);
+        if (NTL > 1) {
+            const v4 d1 = *(gf4*)(drow2 + a);
+            acc[0][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.x, (double)y[0], acc[0][NTL - 1], 0, 0, 0);
+            acc[1][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.y, (double)y[1], acc[1][NTL - 1], 0, 0, 0);
+            acc[2][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.z, (double)y[2], acc[2][NTL - 1], 0, 0, 0);
+            acc[3][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.w, (double)y[3], acc[3][NTL - 1], 0, 0, 0);
         }
-#pragma unroll
-        for (int t = 0; t < NTL; t++) {
-            acc[0][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[t].x, (double)y[0], acc[0][t], 0, 0, 0);
-            acc[1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[t].y, (double)y[1], acc[1][t], 0, 0, 0);
-            acc[0][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[t].z, (double)y[2], acc[0][t], 0, 0, 0);
-            acc[1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[t].w, (double)y[3], acc[1][t], 0, 0, 0);
+    };
+    // interior rounds: all 16 pixels inside the block, no per-pixel condition; the next
+    // round's flux and model quads are read before this round's products
+    const int s_in = (mb.y + 3) & ~3;
+    int s0 = s_in;
+    if (s0 + 16 <= mb.z) {
+        v4 f = *(gf4*)(S.flux + s0 + 4 * g);
+        v4 mv = *(gf4*)(m + s0 + 4 * g);
+        for (; s0 + 16 <= mb.z; s0 += 16) {
+            const int a = s0 + 4 * g;
+            const bool more = s0 + 32 <= mb.z;
+            const v4 fn = more ? *(gf4*)(S.flux + a + 16) : f;
+            const v4 mn = more ? *(gf4*)(m + a + 16) : mv;
+            const float y[4] = {f.x * __builtin_amdgcn_rcpf(mv.x), f.y * __builtin_amdgcn_rcpf(mv.y),
+                                f.z * __builtin_amdgcn_rcpf(mv.z), f.w * __builtin_amdgcn_rcpf(mv.w)};
+            round(a, y);
+            f = fn;
+            mv = mn;
         }
     }
+    // the pixels before s_in (fewer than 4) and after the last whole round, masked
+    for (int r0 = -16; r0 < 16 * ((mb.z - s0 + 15) / 16); r0 += 16) {
+        const int base = r0 < 0 ? s_in - 16 : s0 + r0;
+        const int a = base + 4 * g;
+        float y[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = a + q;
+            const bool in = r0 < 0 ? (i >= mb.y && i < s_in) : (i >= s0 && i < mb.z);
+            if (in) y[q] = S.flux[i] * __builtin_amdgcn_rcpf(m[i]);
+            any |= in;
+        }
+        // a quad with no pixel inside reads no design (its y are zero); a partial quad's
+        // design row is in the stored range because the row starts at the chunk start
+        // rounded down to 4 and its length is rounded up to 4
+        if (!__any(any)) continue;
+        const int ac = any ? a : (mb.y & ~3);
+        round(ac, y);
+    }
     if (!ok) return;
-    const int np = S.p1;
     double* out = mpart + evals[bd.x + e].mp_off + (size_t)bd.z * np;
     // output lane layout of the f64 16x16 product: lane (j, g) register r holds row 4r + g
     // (parameter 16t + 4r + g) of column j (evaluation j)
 #pragma unroll
     for (int t = 0; t < NTL; t++) {
-        const d4 s = acc[0][t] + acc[1][t];
+        const d4 s = (acc[0][t] + acc[1][t]) + (acc[2][t] + acc[3][t]);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int k = 16 * t + 4 * r + g;
@@ -1548,12 +1550,12 @@ hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const 
         const ContClass& K = cls[c];
         if (K.n_mom <= 0) continue;
         const int4* blk = bmom + K.mom_off;
-        if (K.kind == CONT_LINEAR) {
+        if (K.kind == CONT_LINEAR || K.kind == CONT_CHUNK) {
             if (K.p1 <= 16)
-                hipLaunchKernelGGL((k_moments_lin<1>), dim3(K.n_mom), dim3(NT), 0, st, stars,
+                hipLaunchKernelGGL((k_moments_mfma<1>), dim3(K.n_mom), dim3(NT), 0, st, stars,
                                    evals, blk, model, mpart);
             else
-                hipLaunchKernelGGL((k_moments_lin<2>), dim3(K.n_mom), dim3(NT), 0, st, stars,
+                hipLaunchKernelGGL((k_moments_mfma<2>), dim3(K.n_mom), dim3(NT), 0, st, stars,
                                    evals, blk, model, mpart);
             continue;
         }
@@ -1563,12 +1565,6 @@ hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const 
         hipLaunchKernelGGL((k_moments<PM, E, F>), dim3(K.n_mom), dim3(NT), 0, st, stars, evals, \
                            blk, model, mpart);
         PASTA_MOM(1, 8, 0)
-        PASTA_MOM(10, 4, 5)
-        PASTA_MOM(10, 4, 9)
-        PASTA_MOM(16, 2, 11)
-        PASTA_MOM(10, 4, 0)
-        PASTA_MOM(16, 2, 0)
-        PASTA_MOM(32, 1, 0)
 #undef PASTA_MOM
     }
     if (n_solve > 0)
