@@ -45,15 +45,33 @@ def measured_traffic(stars, particles):
     return None
 
 
+# matrix-core / VALU utilisation of the same launches (rocprofv3 PMC), written there as
+# "<key>: <value>" lines
+UTIL_FILE = os.path.join(ROOT, "profiles", "k_model_k_moments_batched_512x64_mfma_valu.md")
+
+
+def measured_utilisation(stars, particles):
+    if (stars, particles) != (512, 64) or not os.path.exists(UTIL_FILE):
+        return None
+    keys = ("mfma_util_k_model", "mfma_tflops_k_model_bf16", "mfma_util_k_moments_mfma",
+            "mfma_tflops_k_moments_mfma_f64", "valu_util_k_model")
+    out = {}
+    for line in open(UTIL_FILE):
+        k, _, v = line.partition(":")
+        if k in keys:
+            out[k] = float(v)
+    return out or None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_stars(pasta, synth, interp, wlg, spec, n_stars, seed=4000):
+def make_stars(pasta, synth, interp, wlg, spec, n_stars, seed=4000, n_range=(40_000, 200_000)):
     rng = np.random.default_rng(seed)
     stars, meta = [], []
     for s in range(n_stars):
-        n = int(rng.integers(40_000, 200_001))
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
         wl = synth.observed_wavelengths(spec, n, 600)
         lab = [rng.uniform(4000, 10000), rng.uniform(-1.0, 0.5), rng.uniform(3.0, 5.0),
                rng.uniform(0, 300), rng.uniform(-150, 150)]
@@ -321,6 +339,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--stars", type=int, default=512)
     ap.add_argument("--particles", type=int, default=64)
+    ap.add_argument("--n-range", type=int, nargs=2, default=(40_000, 200_000),
+                    metavar=("LO", "HI"), help="observed pixels per batched star, U{LO..HI}")
     ap.add_argument("--plain-iters", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
@@ -354,7 +374,8 @@ def main():
         synth.copy_probe(int(args.copy_probe_gib * (1 << 30)))
 
     t0 = time.perf_counter()
-    stars, meta = make_stars(pasta, synth, interp, wlg, spec, args.stars)
+    stars, meta = make_stars(pasta, synth, interp, wlg, spec, args.stars,
+                             n_range=tuple(args.n_range))
     log(f"[bench] {len(stars)} stars set up in {time.perf_counter() - t0:.1f} s "
         f"(N {min(m['n'] for m in meta)}..{max(m['n'] for m in meta)})")
 
@@ -423,7 +444,8 @@ def main():
             "workload": f"batched PSO: {len(stars)} stars x {args.particles} particles, full grid "
                         f"8400 x 200000 u16, timed iterations {args.warmup + 2}-"
                         f"{args.warmup + 1 + args.steps} of a {fit_iters}-iteration fit, "
-                        "N U(40k,200k), R=85000, 10 chunks deg 8",
+                        f"N U({args.n_range[0] // 1000}k,{args.n_range[1] // 1000}k), R=85000, "
+                        "10 chunks deg 8",
             "stars": len(stars),
             "particles": args.particles,
             "evals_per_step": n_evals_step,
@@ -446,6 +468,9 @@ def main():
             "measured_hbm_frac": round(traffic / (model_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
             if traffic else None,
             "iteration_ms": round(iter_ms, 3),
+            # PMC: matrix-core busy fraction and rate of the rotation (bf16 x 3) in k_model
+            # and of the f64 continuum moments, VALU busy fraction of k_model
+            "mfma": measured_utilisation(len(stars), args.particles),
         },
     }
     out["batched_fit_latency_s"] = round(fit_s, 3)  # all stars' fits, submit to last result

@@ -30,15 +30,18 @@ FLAGS = [
 ]
 
 
-def build_lib(verbose=True):
+def build_lib(verbose=True, defines=None, out=LIB):
+    """defines: {name: value} compile-time shape parameters (PASTA_TILE, PASTA_CONT_BLOCK,
+    PASTA_CHI2_GROUP; csrc/engine.h) for a library built to `out` instead of the package's."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs
+    tmp = out + ".tmp"
+    dflags = [f"-D{k}={v}" for k, v in sorted((defines or {}).items())]
+    cmd = [HIPCC] + FLAGS + dflags + ["-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 SYNTH_LIB = os.path.join(HERE, "libpasta_synth.so")

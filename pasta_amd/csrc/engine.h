@@ -6,8 +6,21 @@
 
 namespace pasta {
 
-constexpr int TILE = 1776;        // synthetic pixels per model-kernel workgroup: with both
+// Tile and block shapes are build-time parameters (pasta_amd/tune.py rebuilds the library
+// with other values and times the batched iteration with each).
+#ifndef PASTA_TILE
+#define PASTA_TILE 1776
+#endif
+#ifndef PASTA_CONT_BLOCK
+#define PASTA_CONT_BLOCK 4096
+#endif
+#ifndef PASTA_CHI2_GROUP
+#define PASTA_CHI2_GROUP 8
+#endif
+constexpr int TILE = PASTA_TILE;  // synthetic pixels per model-kernel workgroup: with both
                                   // halos at most 7 x 256 rotation outputs (whole MFMA tiles)
+static_assert(TILE % 256 == 240 && TILE >= 752 && TILE <= 1776,
+              "tile = 256 k - 16 pixels, 3 <= k <= 7 (the LDS tiles are sized for k = 7)");
 constexpr int NT = 256;           // threads per workgroup (4 wavefronts)
 constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
 constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)
@@ -19,7 +32,11 @@ constexpr int TW_MAX = (32 * ((KB_MAX + 15 + 31) / 32) + 32) / 2;  // 352
 constexpr int ROT_STRIDE = KROT_MAX + 6 * TW_MAX;
 constexpr int CH_MAX = 32;        // max chunks
 constexpr int P1_MAX = 32;        // max continuum parameters per chunk / linear columns
-constexpr int BLK = 4096;         // observed pixels per reduction block
+constexpr int BLK = PASTA_CONT_BLOCK;  // observed pixels per reduction block
+static_assert(BLK % 1024 == 0 && BLK >= 1024, "reduction blocks of whole 1024-pixel units");
+// evaluations of one star per chi2 workgroup (polynomial continua of up to 16 parameters)
+constexpr int CHI2_GROUP = PASTA_CHI2_GROUP;
+static_assert(CHI2_GROUP == 4 || CHI2_GROUP == 8 || CHI2_GROUP == 16, "chi2 group of 4, 8 or 16");
 
 enum DispKind : int { DISP_NONE = 0, DISP_FIXED = 1, DISP_VARIABLE = 2 };
 enum ContKind : int { CONT_CHUNK = 0, CONT_LINEAR = 1, CONT_FIXED = 2, CONT_CONSTANT = 3 };
@@ -39,8 +56,8 @@ __host__ __device__ constexpr KVariant mom_variant(int kind, int p1) {
 }
 __host__ __device__ constexpr KVariant chi2_variant(int kind, int p1) {
     return (kind == CONT_CONSTANT || kind == CONT_FIXED) ? KVariant{1, 8, 0}
-           : (p1 == 5 || p1 == 9 || p1 == 11)           ? KVariant{16, 8, p1}
-           : p1 <= 16                                    ? KVariant{16, 8, 0}
+           : (p1 == 5 || p1 == 9 || p1 == 11)           ? KVariant{16, CHI2_GROUP, p1}
+           : p1 <= 16                                    ? KVariant{16, CHI2_GROUP, 0}
                                                          : KVariant{32, 4, 0};
 }
 

@@ -891,17 +891,20 @@ __device__ __forceinline__ void design_row(double xh, int p1, float (&a)[PM]) {
     }
 }
 
-// nalgebra 0.32 dot order: 8 partial accumulators, pairwise merge, scalar tail.
+// nalgebra 0.32 dot order: 8 partial accumulators, pairwise merge, scalar tail.  The
+// accumulators start at -0.0 rather than +0.0: -0 + x == x for every x, so the first add of
+// each is exact and the compiler drops it; the result can differ from a +0.0 start only in
+// the sign of a zero (a +0.0 start turns -0 into +0), which no use of it can tell apart.
 template <int PM>
 __device__ __forceinline__ float dot_na(const float (&a)[PM], const float* b, int n) {
     float c[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) c[t] = 0.0f;
+    for (int t = 0; t < 8; t++) c[t] = -0.0f;
     const int nb = (n >> 3) << 3;
 #pragma unroll
     for (int k = 0; k < PM; k++)
         if (k < nb) c[k & 7] = c[k & 7] + a[k] * b[k];
-    float r = 0.0f;
+    float r = -0.0f;
     r = r + (c[0] + c[4]);
     r = r + (c[1] + c[5]);
     r = r + (c[2] + c[6]);
@@ -969,13 +972,15 @@ __global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ star
 // chunk continuum stores the chunk's f32(powi(x^, k)) rows (b0 = the chunk start rounded
 // down to 4), the linear continuum PT = 16 * NTL zero-padded columns (b0 = 0).  Products
 // of two f32 values are exact in f64 and accumulate in f64, so the sums agree with a
-// sequential f64 sum to f64 rounding.
+// sequential f64 sum to f64 rounding.  The kernel streams the models: the occupancy hint
+// (4 workgroups per CU) keeps it at <= 128 registers, i.e. >= 4 wavefronts per SIMD to
+// keep enough reads in flight.
 template <int NTL>
-__global__ __launch_bounds__(NT) void k_moments_mfma(const DevStar* __restrict__ stars,
-                                                     const DevEval* __restrict__ evals,
-                                                     const int4* __restrict__ blocks,
-                                                     const float* __restrict__ model,
-                                                     double* __restrict__ mpart) {
+__global__ __launch_bounds__(NT, 4) void k_moments_mfma(const DevStar* __restrict__ stars,
+                                                        const DevEval* __restrict__ evals,
+                                                        const int4* __restrict__ blocks,
+                                                        const float* __restrict__ model,
+                                                        double* __restrict__ mpart) {
     typedef double d4 __attribute__((ext_vector_type(4)));
     const int4 bd = blocks[blockIdx.x];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1580,10 +1585,10 @@ hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const 
         hipLaunchKernelGGL((k_chi2<PM, E, F>), dim3(K.n_chi2), dim3(NT), 0, st, stars, evals,   \
                            blk, model, pf, cpart);
         PASTA_CHI2(1, 8, 0)
-        PASTA_CHI2(16, 8, 5)
-        PASTA_CHI2(16, 8, 9)
-        PASTA_CHI2(16, 8, 11)
-        PASTA_CHI2(16, 8, 0)
+        PASTA_CHI2(16, CHI2_GROUP, 5)
+        PASTA_CHI2(16, CHI2_GROUP, 9)
+        PASTA_CHI2(16, CHI2_GROUP, 11)
+        PASTA_CHI2(16, CHI2_GROUP, 0)
         PASTA_CHI2(32, 4, 0)
 #undef PASTA_CHI2
     }
