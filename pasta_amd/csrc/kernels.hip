@@ -276,35 +276,36 @@ __device__ __forceinline__ int islot(int k) { return k + ((k >> 4) << 2); }
 
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
-// out[q] = sum_u tr[u] * in[q + u] for q in [0, n); 8 outputs per thread with a
-// sliding register window.  tr is zero-padded to a multiple of 8 and `in` has
-// 16 finite values beyond the last one read for a valid output.
+// out[q] = sum_u tr[u] * in[q + u] for q in [0, n), u over the taps zero-padded to a
+// multiple of 8 in order (sequential fmaf).  One output per thread and round: the lanes of
+// a wavefront read consecutive LDS words, which no two lanes share a bank for (a sliding
+// window of 8 outputs per thread puts the lanes 32 bytes apart: 16 lanes per bank).  Up
+// to 8 taps stay in registers; longer kernels read them as LDS broadcasts.  `in` is
+// finite up to index n - 1 + (padded tap count) - 1.
 __device__ __forceinline__ void conv_tile(const float* in, const float* tr, int K, int n,
                                           float* out, bool mask, long long x0, int W) {
-    const int q0 = threadIdx.x * 8;
-    if (q0 >= n) return;
-    float acc[8], win[16];
+    const int Kp = (K + 7) & ~7;
+    if (Kp == 8) {
+        float h[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) { acc[t] = 0.0f; win[t] = in[q0 + t]; }
-    for (int u0 = 0; u0 < K; u0 += 8) {
+        for (int u = 0; u < 8; u++) h[u] = tr[u];
+        for (int q = threadIdx.x; q < n; q += NT) {
+            float a = 0.0f;
 #pragma unroll
-        for (int t = 0; t < 8; t++) win[8 + t] = in[q0 + u0 + 8 + t];
-#pragma unroll
-        for (int uu = 0; uu < 8; uu++) {
-            const float h = tr[u0 + uu];
-#pragma unroll
-            for (int t = 0; t < 8; t++) acc[t] = fmaf(h, win[t + uu], acc[t]);
+            for (int u = 0; u < 8; u++) a = fmaf(h[u], in[q + u], a);
+            const long long x = x0 + q;
+            out[q] = (mask && (x < 0 || x >= W)) ? 0.0f : a;
         }
-#pragma unroll
-        for (int t = 0; t < 8; t++) win[t] = win[8 + t];
+        return;
     }
+    for (int q = threadIdx.x; q < n; q += NT) {
+        float a = 0.0f;
+        for (int u0 = 0; u0 < Kp; u0 += 8) {
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-        int q = q0 + t;
-        if (q < n) {
-            long long x = x0 + q;
-            out[q] = (mask && (x < 0 || x >= W)) ? 0.0f : acc[t];
+            for (int u = 0; u < 8; u++) a = fmaf(tr[u0 + u], in[q + u0 + u], a);
         }
+        const long long x = x0 + q;
+        out[q] = (mask && (x < 0 || x >= W)) ? 0.0f : a;
     }
 }
 
