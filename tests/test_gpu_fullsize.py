@@ -1,0 +1,104 @@
+"""GPU parity at full size: the 25 x 16 x 21 grid of 200,000 pixels over 3800-9000 A
+(3.36 GB, filled in HBM with the seeded 5,000-line recipe) against the C oracle on the
+same grid (read back from HBM), for stars of 150,000 and 200,000 observed pixels with
+R = 85,000 and a 10-chunk degree-8 continuum.
+
+Tolerance: model spectra and chi2 within 1e-5 relative in FP32, as for the small grid.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-5
+R_INSTR = 85_000.0
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)))
+
+
+@pytest.fixture(scope="module")
+def full():
+    import pasta_amd as pasta
+    from pasta_amd import synth
+
+    spec = synth.full_grid_spec()
+    wlg = pasta.WlGrid(spec.log10_first, spec.log10_step, spec.n_pix, log=True)
+    interp = pasta.InMemInterpolator.empty(spec.labels(), wlg)
+    synth.fill_grid_gpu(interp, spec)
+    rows = synth.read_grid(interp)
+    og = O.Grid(spec.labels(), rows, None, spec.log10_first, spec.log10_step)
+    print(f"full grid {rows.shape} in HBM and on the host", flush=True)
+    return pasta, synth, spec, wlg, interp, og
+
+
+def _star(full, n, seed):
+    pasta, synth, spec, wlg, interp, og = full
+    wl = synth.observed_wavelengths(spec, n, 600)
+    od = O.Dispersion(og, O.DISP_FIXED, wl, resolution=R_INSTR)
+    true = np.array([6789.0, -0.2, 4.13, 120.0, -35.0])
+    flux, var = synth.observe(O.produce_model(og, od, true), wl, 150.0, seed)
+    return wl, od, flux, var
+
+
+def test_fullsize_models_match_oracle(full):
+    """Rotation kernels from pass-through to the f32 matrix-core path (K = 1,017 at vsini
+    500 on this grid), RV at both ends of the fit range."""
+    pasta, synth, spec, wlg, interp, og = full
+    wl, od, _, _ = _star(full, 150_000, 1)
+    d = pasta.FixedResolutionDispersion(wl, R_INSTR, wlg)
+    labs = np.array([[4321.0, -0.93, 3.07, 0.0, 0.0],
+                     [6789.0, -0.2, 4.13, 10.0, 150.0],
+                     [8500.0, 0.3, 4.5, 150.0, -150.0],
+                     [9876.0, 0.45, 4.95, 300.0, 75.0],
+                     [5500.0, -0.5, 3.5, 500.0, -20.0]])
+    dvelo = np.log(10.0) * spec.log10_step
+    ks = [len(O.rotation_kernel(v, dvelo)) if v > 0 else 0 for v in labs[:, 3]]
+    assert ks[3] <= 640 < ks[4], ks
+    print(f"full-size models: taps {ks}", flush=True)
+    out = interp.produce_model_bulk(d, labs)
+    for k, lab in enumerate(labs):
+        ref = O.produce_model(og, od, lab)
+        assert _rel(out[k], ref) < RTOL, (lab.tolist(), _rel(out[k], ref))
+
+
+@pytest.mark.parametrize("n", [150_000, 200_000])
+def test_fullsize_chi2_matches_oracle(full, n):
+    pasta, synth, spec, wlg, interp, og = full
+    wl, od, flux, var = _star(full, n, 2 + n)
+    oc = O.Continuum("chunk", n, wl=wl, n_chunks=10, p_order=8, overlap=0.2)
+    d = pasta.FixedResolutionDispersion(wl, R_INSTR, wlg)
+    star = pasta.Star(interp, d, pasta.ChunkContinuumFitter(wl, 10, 8, 0.2), flux, var)
+    labs = synth.random_labels(spec, 32, 40 + n // 1000, margin=1.0)
+    gpu, st, _ = star.chi2(labs)
+    assert (st == 0).all(), st
+    ref = O.cost_batch(og, od, oc, flux, var, labs, 16)
+    rel = np.abs(gpu - ref) / np.abs(ref)
+    print(f"N = {n}: max rel chi2 difference {rel.max():.2e} over {len(labs)} evaluations",
+          flush=True)
+    assert rel.max() < RTOL, (labs[int(np.argmax(rel))].tolist(), rel.max())
+
+
+def test_fullsize_bulk_equals_single(full):
+    """chi2_bulk (one launch over several stars of different lengths, the reference's
+    NoConvolution + ChunkFitter(5, 8, 0.2)) gives each star's chi2 as that star alone."""
+    pasta, synth, spec, wlg, interp, og = full
+    ns = (200_000, 150_000, 61_234, 40_000)
+    wls, fluxes, vars_ = [], [], []
+    for k, n in enumerate(ns):
+        wl, _, flux, var = _star(full, n, 7 + k)
+        wls.append(wl)
+        fluxes.append(flux)
+        vars_.append(var)
+    labs = synth.random_labels(spec, len(ns), 91, margin=1.0)
+    bulk = interp.chi2_bulk(wls, fluxes, vars_, labs)
+    for k in range(len(ns)):
+        d = pasta.NoConvolutionDispersion(wls[k], wlg)
+        star = pasta.Star(interp, d, pasta.ChunkContinuumFitter(wls[k], 5, 8, 0.2), fluxes[k],
+                          vars_[k])
+        alone, st, _ = star.chi2(labs[k:k + 1])
+        assert st[0] == 0 and alone[0] == bulk[k], (ns[k], alone[0], bulk[k])

@@ -108,6 +108,40 @@ def test_rotation_paths_match_oracle(small, gpu_small):
         assert _rel(out[k], ref) < RTOL, (lab.tolist(), _rel(out[k], ref))
 
 
+def _vsini_for_taps(dvelo, target):
+    """Smallest vsini whose rotation kernel has at least `target` taps (bisection)."""
+    lo, hi = 1.0, 1500.0  # 2,757 taps at 1,500 km/s on this grid
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if len(O.rotation_kernel(mid, dvelo)) >= target:
+            hi = mid
+        else:
+            lo = mid
+    return hi
+
+
+def test_rotation_kernel_limit(small, gpu_small):
+    """The largest kernel the reference convolves (2,047 taps: FFTSIZE 2,048 minus one,
+    made odd) matches the oracle; one of 2,049 taps is the reference's "larger than
+    FFTSIZE" error (src/convolve_rv.rs:512-519), on the GPU as in the oracle."""
+    pasta, wlg, interp = gpu_small
+    d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
+    dvelo = np.log(10.0) * small.spec.log10_step
+    v_max = _vsini_for_taps(dvelo, 2047)
+    v_over = _vsini_for_taps(dvelo, 2049)
+    assert len(O.rotation_kernel(v_max, dvelo)) == 2047
+    assert len(O.rotation_kernel(v_over, dvelo)) == 2049
+    print(f"rotation limit: vsini {v_max:.3f} (2047 taps), {v_over:.3f} (2049 taps)", flush=True)
+    lab = np.array([[6123.0, -0.3, 4.21, v_max, -15.0]])
+    out = interp.produce_model_bulk(d, lab)
+    ref = O.produce_model(small.og, small.odisp, lab[0])
+    assert _rel(out[0], ref) < RTOL, _rel(out[0], ref)
+    _, st = interp._produce(d, np.array([[6123.0, -0.3, 4.21, v_over, -15.0]]))
+    assert st[0] == 4, st  # rotation kernel larger than FFTSIZE
+    with pytest.raises(O.OracleError):
+        O.produce_model(small.og, small.odisp, [6123.0, -0.3, 4.21, v_over, -15.0])
+
+
 def _cont_pair(small, pasta, kind):
     n = len(small.wl)
     if kind == "chunk":
