@@ -601,6 +601,17 @@ __device__ __forceinline__ void gather_rows2(const DevGrid& G, const DevEval& ea
 
 // taps of a job into LDS (and the zero padding the MFMA steps read past its tile); issued
 // before the interpolation so that their latency overlaps it
+// 16-byte quads of each of the six split-tap arrays that the bf16 MFMA groups of a K-tap
+// kernel read: words [0, 16 * ceil((K + 15) / 32) + 8), from the highest B operand index
+// 20 + 16 (C - 1) + 3 of conv_rot_bf16
+__device__ __forceinline__ int tap_quads(int K) { return (16 * ((K + 46) >> 5) + 8) >> 2; }
+
+// the used quads of a K-tap kernel's split taps, as (array, quad) -> index in the arrays
+__device__ __forceinline__ int tap_quad_index(int i, int nq4) {
+    const int a = i / nq4;
+    return a * (TW_MAX / 4) + (i - a * nq4);
+}
+
 __device__ __forceinline__ void load_taps(const TileJob& J, const float* rot, float* Ib,
                                           float* hrev, unsigned int* tw, float* grev,
                                           bool with_tw = true) {
@@ -609,7 +620,11 @@ __device__ __forceinline__ void load_taps(const TileJob& J, const float* rot, fl
     if (J.kind == 1) {  // split taps prepared per evaluation by k_prep
         if (with_tw) {
             const u32x4* src = (const u32x4*)(rot + (size_t)J.e * ROT_STRIDE + KROT_MAX);
-            for (int i = tid; i < 6 * TW_MAX / 4; i += NT) ((u32x4*)tw)[i] = src[i];
+            const int nq4 = tap_quads(J.K);
+            for (int i = tid; i < 6 * nq4; i += NT) {
+                const int q = tap_quad_index(i, nq4);
+                ((u32x4*)tw)[q] = src[q];
+            }
         }
         unsigned short* Ib3 = (unsigned short*)Ib;
         if (tid < 64) {  // the MFMA groups read past the tile
@@ -739,13 +754,14 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
         // shared interpolation: A's groups go to LDS at once, B's wait in registers, and
         // so do B's split taps (read now, stored after A's rotation)
         const bool btw = bev->krot > 0 && bev->krot <= KB_MAX;
+        const int bq4 = btw ? tap_quads(bev->krot) : 0;
         u32x4 tkeep[3];
         {
             const u32x4* src = (const u32x4*)(rot + (size_t)it.y * ROT_STRIDE + KROT_MAX);
 #pragma unroll
             for (int r = 0; r < 3; r++) {
                 const int i = threadIdx.x + r * NT;
-                tkeep[r] = (btw && i < 6 * TW_MAX / 4) ? src[i] : u32x4{0u, 0u, 0u, 0u};
+                tkeep[r] = i < 6 * bq4 ? src[tap_quad_index(i, bq4)] : u32x4{0u, 0u, 0u, 0u};
             }
         }
         float keep[2][8];
@@ -780,7 +796,7 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
 #pragma unroll
             for (int r = 0; r < 3; r++) {
                 const int i = threadIdx.x + r * NT;
-                if (i < 6 * TW_MAX / 4) ((u32x4*)tw)[i] = tkeep[r];
+                if (i < 6 * bq4) ((u32x4*)tw)[tap_quad_index(i, bq4)] = tkeep[r];
             }
         }
         load_taps(B, rot, Ib, hrev, tw, grev, false);
