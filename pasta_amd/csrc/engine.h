@@ -9,7 +9,7 @@ namespace pasta {
 // Tile and block shapes are build-time parameters (pasta_amd/tune.py rebuilds the library
 // with other values and times the batched iteration with each).
 #ifndef PASTA_TILE
-#define PASTA_TILE 1776
+#define PASTA_TILE 2032
 #endif
 #ifndef PASTA_CONT_BLOCK
 #define PASTA_CONT_BLOCK 4096
@@ -17,10 +17,11 @@ namespace pasta {
 #ifndef PASTA_CHI2_GROUP
 #define PASTA_CHI2_GROUP 8
 #endif
-constexpr int TILE = PASTA_TILE;  // synthetic pixels per model-kernel workgroup: with both
-                                  // halos at most 7 x 256 rotation outputs (whole MFMA tiles)
-static_assert(TILE % 256 == 240 && TILE >= 752 && TILE <= 1776,
-              "tile = 256 k - 16 pixels, 3 <= k <= 7 (the LDS tiles are sized for k = 7)");
+constexpr int TILE = PASTA_TILE;  // synthetic pixels per model-kernel workgroup: the tile
+                                  // and a 7-tap instrumental halo are whole 256-output MFMA
+                                  // tiles of the rotation
+static_assert(TILE % 256 == 240 && TILE >= 752 && TILE <= 2032,
+              "tile = 256 k - 16 pixels, 3 <= k <= 8 (k_model's LDS fits 4 workgroups per CU)");
 constexpr int NT = 256;           // threads per workgroup (4 wavefronts)
 constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
 constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)

@@ -26,12 +26,15 @@ namespace pasta {
 // Interpolated tile for the rotation stage: logical index k lives in slot k + 4*(k/16)
 // (4 pad floats after every 16), so that the 16-float-strided MFMA operand rows fall
 // in distinct LDS banks.  Logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KROT_MAX + 55.
-constexpr int ILOG = 4160;
-constexpr int ILDS = ILOG + ILOG / 4;                               // 5200 slots
+constexpr int ILOG = (7 + TILE + 1 + 2 * HD_MAX + KROT_MAX + 55 + 31) & ~31;  // 4416 at 2032
+constexpr int ILDS = ILOG + ILOG / 4;
 // For the bf16 rotation the tile is kept as three bf16 arrays of ILOGB elements
 // (logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KB_MAX + 71).
-constexpr int ILOGB = 2752;
-constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 8 + 16 + 8;           // 2080
+constexpr int ILOGB = (7 + TILE + 1 + 2 * HD_MAX + KB_MAX + 71 + 7) & ~7;     // 3008 at 2032
+constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 8 + 16 + 8;
+// k_model's LDS: 4 workgroups per CU share its 160 KB
+static_assert((ILDS + KROT_MAX + 48 + RLDS + 2 * HD_MAX + 1 + 8) * 4 <= 40960,
+              "k_model LDS beyond a quarter of the CU's");
 constexpr double C_KMS = 299792.0;   // src/convolve_rv.rs:440,547
 constexpr double LN10 = 2.302585092994045684017991454684364208;
 
