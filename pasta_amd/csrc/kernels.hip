@@ -106,21 +106,27 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     const double* lab = labels + 5 * (size_t)e;
     const double vsini = lab[3];
     const double dvelo = LN10 * G.sg_step;
+    __shared__ int s_st;
+    const bool active = ev.active;
+    if (lane == 0) s_st = active ? local_weights(*G.geom, lab[0], lab[1], lab[2], ev.idx, ev.w) : -1;
+    __syncthreads();
+    // neighbour j = lane.  fw = factor * weight: the reference forms f32(u) * factor, then
+    // the gemv multiplies by the weight (interpolate.rs:962,966); one product per neighbour
+    // differs from that by an f32 rounding per term
+    const int ij = ev.idx[lane];
+    if (active) {
+        const float f = (ij >= 0 && G.factors) ? G.factors[ij] : 1.0f / 65535.0f;
+        ev.fw[lane] = ij >= 0 ? f * ev.w[lane] : 0.0f;
+        ev.rowp[lane] = G.spectra + (ij >= 0 ? (long long)ij * G.pitch : 0LL);
+    }
+    unsigned int hsh = 2166136261u;  // FNV-1a over the neighbour rows: fast reject in pairing
+    for (int j = 0; j < 64; j++) hsh = (hsh ^ (unsigned int)__builtin_amdgcn_readlane(ij, j)) * 16777619u;
+    const unsigned long long present = __ballot(ij >= 0);
+    const int first = present ? __builtin_amdgcn_readlane(ij, __builtin_ctzll(present)) : 0;
     if (lane == 0) {
-        int st = ST_OK, krot = 0;
-        if (!ev.active) {
-            st = -1;
-        } else {
+        int st = s_st, krot = 0;
+        if (active) {
             const DevStar& S = stars[ev.star];
-            st = local_weights(*G.geom, lab[0], lab[1], lab[2], ev.idx, ev.w);
-            // fw = factor * weight: the reference forms f32(u) * factor, then the gemv
-            // multiplies by the weight (interpolate.rs:962,966); one product per neighbour
-            // differs from that by an f32 rounding per term
-            for (int j = 0; j < 64; j++) {
-                const float f = (ev.idx[j] >= 0 && G.factors) ? G.factors[ev.idx[j]] : 1.0f / 65535.0f;
-                ev.fw[j] = ev.idx[j] >= 0 ? f * ev.w[j] : 0.0f;
-                ev.rowp[j] = G.spectra + (ev.idx[j] >= 0 ? (long long)ev.idx[j] * G.pitch : 0LL);
-            }
             if (!st && vsini < 0.0) st = ST_VSINI;
             if (!st) {
                 int K = rotation_len(vsini, dvelo);
@@ -152,14 +158,9 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
         }
         ev.status = st;
         ev.krot = krot;
-        int key = G.geom->n_models;
-        unsigned int hsh = 2166136261u;  // FNV-1a over the neighbour rows: fast reject in pairing
-        for (int j = 0; j < 64; j++) hsh = (hsh ^ (unsigned int)ev.idx[j]) * 16777619u;
         ev.rows_hash = (int)hsh;
-        if (st == ST_OK)
-            for (int j = 0; j < 64; j++)
-                if (ev.idx[j] >= 0) { key = ev.idx[j]; break; }
-        ev.key = key;
+        // storage row of the first present neighbour
+        ev.key = (st == ST_OK && present) ? first : G.geom->n_models;
         s_K = (st == ST_OK) ? krot : 0;
     }
     __syncthreads();
