@@ -737,7 +737,9 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     const int tile = b / n_evals;
     const int k = b - tile * n_evals;
     if (k >= __builtin_amdgcn_readfirstlane(*n_items)) return;
-    const int2 it = items[k];
+    // the work item's evaluations are uniform: keep them (and so their weight loads) scalar
+    const int2 it = make_int2(__builtin_amdgcn_readfirstlane(items[k].x),
+                              __builtin_amdgcn_readfirstlane(items[k].y));
     TileJob A;
     const bool hasA = tile_job(G, stars, evals, it.x, tile, tbound, tb_stride, mode, A);
     bool hasB = false;
@@ -749,7 +751,7 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
         hasB = tile_job(G, stars, evals, it.y, tile, tbound, tb_stride, mode, T);
         bg0 = T.g0;
         bng = T.ng;
-        bev = T.ev;
+        bev = evals + it.y;
     }
     const long long pg0 = min(A.g0, bg0);
     const int png = (int)((max(A.g0 + 8LL * A.ng, bg0 + 8LL * bng) - pg0) >> 3);
@@ -779,8 +781,8 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
                 const long long x0 = pg0 + 8LL * gg;
                 const bool inside = x0 >= 0 && x0 < G.W;
                 const int voff = inside ? (int)(2 * x0) : OOB_OFFSET;
-                if (__all(x0 + 7 < G.Wq)) gather_rows2<true>(G, *A.ev, *bev, voff, x0, aa, ab);
-                else gather_rows2<false>(G, *A.ev, *bev, voff, x0, aa, ab);
+                if (__all(x0 + 7 < G.Wq)) gather_rows2<true>(G, evals[it.x], evals[it.y], voff, x0, aa, ab);
+                else gather_rows2<false>(G, evals[it.x], evals[it.y], voff, x0, aa, ab);
                 if (x0 + 8 > G.W) {
 #pragma unroll
                     for (int t = 0; t < 8; t++)
