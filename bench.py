@@ -136,7 +136,8 @@ def run_batched(pasta, _lib, interp, stars, meta, P, warmup, steps, timing_every
 
 
 def run_plain(pasta, synth, interp, wlg, spec, iters=200, P=100, seed=2000):
-    """one star, 150,000 pixels, 10 chunks of degree 8, 100 particles."""
+    """The plain path on the full grid: one star, 150,000 pixels, 10 chunks of degree 8,
+    100 particles."""
     rng = np.random.default_rng(seed)
     wl = synth.observed_wavelengths(spec, 150_000, 600)
     lab = [rng.uniform(4500, 9500), rng.uniform(-0.9, 0.4), rng.uniform(3.2, 4.8),

@@ -50,7 +50,7 @@ def _raise(rc, what):
 
 class WlGrid:
     """WlGrid(min, step, len, log=False) (src/lib.rs:630-653).  Only log-spaced
-    synthetic grids are supported by the HIP path (the north-star configs are
+    synthetic grids are supported by the HIP path (every grid this path is built for is
     log-uniform); Linspace grids raise on use."""
 
     def __init__(self, min, step, len, log=None):

@@ -1,7 +1,7 @@
 """GPU parity: the HIP path against the C oracle on identical inputs (small config).
 
-Tolerance: model spectra and chi2 within 1e-5 relative in FP32 (the bar the
-north star sets).  The oracle restates the reference's f32 arithmetic; the HIP
+Tolerance: model spectra and chi2 within 1e-5 relative in FP32 (this path's parity
+bar).  The oracle restates the reference's f32 arithmetic; the HIP
 path differs in summation order (direct convolution vs the reference's FFT,
 fused multiply-adds, f64 reductions), which the tolerance absorbs.
 """
