@@ -341,9 +341,51 @@ static int cholesky(std::vector<double>& A, int n) {  // in place, lower
 static void push_blocks(std::vector<int4>& out, int kind, int chunk, int i0, int i1) {
     for (int s = i0; s < i1; s += BLK) out.push_back(make_int4(kind, chunk, s, std::min(s + BLK, i1)));
 }
-// moment blocks: (chunk, i0, i1, 0)
-static void push_mblocks(std::vector<int4>& out, int chunk, int i0, int i1) {
-    for (int s = i0; s < i1; s += BLK) out.push_back(make_int4(chunk, s, std::min(s + BLK, i1), 0));
+// moment blocks (chunk, i0, i1, second chunk or -1), each with a partial slot per chunk:
+// slot_a is the next slot; slot_b is assigned when the second chunk's slots are laid out
+static void push_mblocks(pasta_cont& c, int chunk, int i0, int i1, int chunk2, int& slot) {
+    for (int s = i0; s < i1; s += BLK) {
+        c.mblocks.push_back(make_int4(chunk, s, std::min(s + BLK, i1), chunk2));
+        c.mslot.push_back(make_int2(slot++, -1));
+    }
+}
+
+// Moment blocks of a chunk continuum.  The pixels of the overlap of chunks ch and ch + 1
+// feed both chunks' moments; their blocks carry both chunks, so that one read of the
+// ratios serves the two products.  A chunk's partial slots are contiguous and in pixel
+// order: the overlap with the previous chunk, its own pixels, the overlap with the next.
+// Two-chunk blocks need a pixel in at most two chunks and the two parameter sets in one
+// 32-column matrix-core product (p1 <= 16); otherwise every chunk gets blocks of its own.
+static void chunk_mblocks(pasta_cont& c) {
+    const int nc = c.n_chunks;
+    const int* ss = c.startstop.data();
+    bool shared = c.p1 <= 16;
+    for (int ch = 1; ch + 1 < nc && shared; ch++)
+        if (ss[2 * (ch + 1)] < ss[2 * (ch - 1) + 1]) shared = false;  // a pixel in 3 chunks
+    c.mblocks.clear();
+    c.mslot.clear();
+    c.mb_off.assign(1, 0);
+    int slot = 0;
+    int right0 = 0, right1 = 0;  // blocks of the previous chunk's overlap with this one
+    for (int ch = 0; ch < nc; ch++) {
+        const int start = ss[2 * ch], stop = ss[2 * ch + 1];
+        if (!shared) {
+            push_mblocks(c, ch, start, stop, -1, slot);
+            c.mb_off.push_back(slot);
+            continue;
+        }
+        for (int b = right0; b < right1; b++) c.mslot[b].y = slot++;
+        const int lo = ch > 0 ? std::max(start, ss[2 * (ch - 1) + 1]) : start;
+        const int next = ch + 1 < nc ? ss[2 * (ch + 1)] : stop;
+        const int hi = std::min(stop, next);
+        push_mblocks(c, ch, lo, hi, -1, slot);
+        right0 = right1 = (int)c.mblocks.size();
+        if (ch + 1 < nc && next < stop) {
+            push_mblocks(c, ch, next, stop, ch + 1, slot);
+            right1 = (int)c.mblocks.size();
+        }
+        c.mb_off.push_back(slot);
+    }
 }
 
 static void host_design_row(double xh, int p1, float* a) {
@@ -416,12 +458,7 @@ int pasta_cont_create_chunk(const double* wl, int n, int n_chunks, int p_order, 
         for (int i = sn; i < stop; i++)
             c->fac[i] = poly_blend(map_range_f((float)wl[i], (float)wl[sn], (float)wl[stop - 1], 1.0f, 0.0f));
     }
-    // moment blocks per chunk
-    c->mb_off.push_back(0);
-    for (int ch = 0; ch < n_chunks; ch++) {
-        push_mblocks(c->mblocks, ch, c->startstop[2 * ch], c->startstop[2 * ch + 1]);
-        c->mb_off.push_back((int)c->mblocks.size());
-    }
+    chunk_mblocks(*c);
     // chi2 segments (src/continuum_fitting.rs:300-358)
     const int* ss = c->startstop.data();
     if (ss[2] < 50) return fail(PASTA_ERR_ARG, "first chunk shorter than the 50-pixel trim");
@@ -465,8 +502,11 @@ int pasta_cont_create_linear(const float* design, int n, int ncols, pasta_cont**
         for (int l = k + 1; l < ncols; l++) G[k * ncols + l] = G[l * ncols + k];
     if (cholesky(G, ncols)) return fail(PASTA_ERR_ARG, "design matrix is rank deficient");
     c->chol = G;
-    push_mblocks(c->mblocks, 0, 0, n);
-    c->mb_off = {0, (int)c->mblocks.size()};
+    {
+        int slot = 0;
+        push_mblocks(*c, 0, 0, n, -1, slot);
+        c->mb_off = {0, slot};
+    }
     push_blocks(c->cblocks, 0, 0, 50, n - 50);
     c->chi2_div = (double)n;
     *out = c.release();
@@ -492,8 +532,11 @@ int pasta_cont_create_constant(int n, pasta_cont** out) {
     auto c = std::make_unique<pasta_cont>();
     c->kind = CONT_CONSTANT;
     c->n = n;
-    push_mblocks(c->mblocks, 0, 0, n);
-    c->mb_off = {0, (int)c->mblocks.size()};
+    {
+        int slot = 0;
+        push_mblocks(*c, 0, 0, n, -1, slot);
+        c->mb_off = {0, slot};
+    }
     push_blocks(c->cblocks, 0, 0, 0, n);
     c->chi2_div = (double)n;
     *out = c.release();
@@ -617,6 +660,7 @@ int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, const fl
         D.ignore = c->ignore;
         D.n_params = c->n_params;
         D.n_mblocks = (int)c->mblocks.size();
+        D.n_mslots = c->mb_off.empty() ? 0 : c->mb_off.back();
         D.n_cblocks = (int)c->cblocks.size();
         D.chi2_div = c->chi2_div;
         CHK(dev_copy(s.get(), c->lohi.data(), c->lohi.size(), &D.chunk_lohi));
@@ -628,6 +672,7 @@ int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, const fl
         CHK(dev_copy(s.get(), c->cdoff.data(), c->cdoff.size(), &D.cdoff));
         CHK(dev_copy(s.get(), c->fixedc.data(), c->fixedc.size(), &D.fixedc));
         CHK(dev_copy(s.get(), c->mblocks.data(), c->mblocks.size(), &D.mblocks));
+        CHK(dev_copy(s.get(), c->mslot.data(), c->mslot.size(), &D.mslot));
         CHK(dev_copy(s.get(), c->mb_off.data(), c->mb_off.size(), &D.mb_off));
         CHK(dev_copy(s.get(), c->cblocks.data(), c->cblocks.size(), &D.cblocks));
         s->n_params = c->n_params;
@@ -668,7 +713,7 @@ int pasta_star_destroy(pasta_star* s) {
 static int64_t star_eval_bytes(const pasta_star* s) {
     const DevStar& D = s->dev;
     int64_t b = (int64_t)D.N * 4 + sizeof(DevEval) + ROT_STRIDE * 4 + 5 * 8 * 4;
-    b += (int64_t)D.n_mblocks * std::max(1, D.p1) * 8 + (int64_t)D.n_cblocks * 8 + D.n_params * 4;
+    b += (int64_t)D.n_mslots * std::max(1, D.p1) * 8 + (int64_t)D.n_cblocks * 8 + D.n_params * 4;
     b += (int64_t)(D.n_tiles + D.n_mblocks + D.n_cblocks + std::max(1, D.n_chunks)) * 8;
     return b;
 }
@@ -715,7 +760,7 @@ int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
             ev.m_off = mo;
             mo += outlen;
             ev.mp_off = mp;
-            mp += (int64_t)D.n_mblocks * std::max(1, D.p1);
+            mp += (int64_t)D.n_mslots * std::max(1, D.p1);
             ev.cp_off = cp;
             cp += D.n_cblocks;
             ev.pf_off = pf;

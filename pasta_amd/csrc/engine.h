@@ -83,7 +83,8 @@ struct DevGrid {
 struct DevStar {
     int N, disp_kind, Kd, cont_kind;
     int n_chunks, p1, ignore, n_params;
-    int n_mblocks, n_cblocks, n_tiles, pad0;
+    int n_mblocks, n_cblocks, n_tiles, n_mslots;  // moment blocks, chi2 blocks, tiles,
+                                                  // moment partial slots
     double chi2_div, wl0, wlN;
     const float* flux;
     const float* ivar;   // 1 / variance (f32), the chi2 weight
@@ -100,8 +101,10 @@ struct DevStar {
     const float* cdesign;     // chunk: per chunk c, p1 rows of f32(powi(x^, k)) from pixel
                               // startstop[2c] & ~3, row stride rounded up to 4, zero padded
     const int* cdoff;         // chunk c's design at cdesign + cdoff[c]
-    const int4* mblocks;      // moment blocks (chunk, i0, i1, 0)
-    const int* mb_off;        // chunk c owns mblocks [mb_off[c], mb_off[c+1])
+    const int4* mblocks;      // moment blocks (chunk, i0, i1, second chunk or -1): a block
+                              // in the overlap of two chunks carries both
+    const int2* mslot;        // per moment block: the partial slots of its chunk(s)
+    const int* mb_off;        // chunk c owns partial slots [mb_off[c], mb_off[c+1])
     const int4* cblocks;      // chi2 blocks (kind, chunk, i0, i1); kind 1 = overlap
 };
 

@@ -136,7 +136,8 @@ struct pasta_cont {
     std::vector<float> cdesign;  // chunk design rows for the matrix-core moments
     std::vector<int> cdoff;
     std::vector<int4> mblocks, cblocks;
-    std::vector<int> mb_off;
+    std::vector<int2> mslot;  // per moment block: partial slots of its chunk(s)
+    std::vector<int> mb_off;  // chunk c owns partial slots [mb_off[c], mb_off[c+1])
     double chi2_div = 1.0;
 };
 

@@ -1010,7 +1010,8 @@ __global__ __launch_bounds__(NT, 4) void k_moments_mfma(const DevStar* __restric
     const int e0 = wv * 16;
     if (e0 >= bd.y) return;  // wavefront-uniform; the kernel has no barrier
     const DevStar& S = stars[evals[bd.x].star];
-    const int4 mb = S.mblocks[bd.z];
+    const int4 mb = S.mblocks[bd.z];  // (chunk, i0, i1, second chunk or -1)
+    const int2 sl = S.mslot[bd.z];     // partial slots of the chunk(s)
     const int j = lane & 15, g = lane >> 4, e = e0 + j;
     const bool ok = e < bd.y && evals[bd.x + e].status == ST_OK;
     const float* m = model + (ok ? evals[bd.x + e].m_off : 0);
@@ -1036,14 +1037,25 @@ __global__ __launch_bounds__(NT, 4) void k_moments_mfma(const DevStar* __restric
     // design rows beyond p1 only feed output rows that are never stored, and an absent
     // evaluation only its own output column: such lanes read any valid row
     const float* drow = dbase + (size_t)min(j, np - 1) * stride - b0;
-    const float* drow2 = dbase + (size_t)min(16 + j, np - 1) * stride - b0;
+    // the second 16-column tile: the second chunk's parameters 0..15 for a block in the
+    // overlap of two chunks, else parameters 16..31 of the same design (p1 > 16)
+    const bool two = mb.w >= 0;
+    const float* drow2;
+    if (two) {
+        const int b0b = S.startstop[2 * mb.w] & ~3;
+        const int strideb = (S.startstop[2 * mb.w + 1] - b0b + 3) & ~3;
+        drow2 = S.cdesign + S.cdoff[mb.w] + (size_t)min(j, np - 1) * strideb - b0b;
+    } else {
+        drow2 = dbase + (size_t)min(16 + j, np - 1) * stride - b0;
+    }
+    const bool tile2 = NTL > 1 && (two || np > 16);  // uniform over the workgroup
     auto round = [&](int a, const float (&y)[4]) {
         const v4 d0 = *(gf4*)(drow + a);
         acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.x, (double)y[0], acc[0][0], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.y, (double)y[1], acc[1][0], 0, 0, 0);
         acc[2][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.z, (double)y[2], acc[2][0], 0, 0, 0);
         acc[3][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.w, (double)y[3], acc[3][0], 0, 0, 0);
-        if (NTL > 1) {
+        if (NTL > 1 && tile2) {
             const v4 d1 = *(gf4*)(drow2 + a);
             acc[0][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.x, (double)y[0], acc[0][NTL - 1], 0, 0, 0);
             acc[1][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.y, (double)y[1], acc[1][NTL - 1], 0, 0, 0);
@@ -1091,16 +1103,20 @@ __global__ __launch_bounds__(NT, 4) void k_moments_mfma(const DevStar* __restric
         round(ac, y);
     }
     if (!ok) return;
-    double* out = mpart + evals[bd.x + e].mp_off + (size_t)bd.z * np;
+    double* out = mpart + evals[bd.x + e].mp_off;
     // output lane layout of the f64 16x16 product: lane (j, g) register r holds row 4r + g
-    // (parameter 16t + 4r + g) of column j (evaluation j)
+    // of column j (evaluation j): parameter 16t + 4r + g of the block's chunk, or for the
+    // second tile of a two-chunk block parameter 4r + g of the second chunk
 #pragma unroll
     for (int t = 0; t < NTL; t++) {
+        if (t == 1 && !tile2) break;
         const d4 s = (acc[0][t] + acc[1][t]) + (acc[2][t] + acc[3][t]);
+        const bool second = t == 1 && two;
+        double* o = out + (size_t)(second ? sl.y : sl.x) * np;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int k = 16 * t + 4 * r + g;
-            if (k < np) out[k] = s[r];
+            const int k = (second ? 0 : 16 * t) + 4 * r + g;
+            if (k < np) o[k] = s[r];
         }
     }
 }
@@ -1579,7 +1595,8 @@ hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const 
         if (K.n_mom <= 0) continue;
         const int4* blk = bmom + K.mom_off;
         if (K.kind == CONT_LINEAR || K.kind == CONT_CHUNK) {
-            if (K.p1 <= 16)
+            // chunk continua: two-tile kernel (blocks in the overlap of two chunks carry both)
+            if (K.kind == CONT_LINEAR && K.p1 <= 16)
                 hipLaunchKernelGGL((k_moments_mfma<1>), dim3(K.n_mom), dim3(NT), 0, st, stars,
                                    evals, blk, model, mpart);
             else
