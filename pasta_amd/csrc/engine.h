@@ -112,13 +112,15 @@ struct DevStar {
 struct DevEval {
     int star;
     int status;
-    int krot;        // rotation taps; 0 = pass-through
+    int krot;        // rotation taps (composed taps when comp); 0 = pass-through
     int tile_base;   // first synthetic tile touched
     int cmax;        // exclusive upper bound of readable model-segment indices
     int active;      // 0 = skip (finished / failed star in the batched path)
     int key;         // storage row of the first present neighbour: groups evaluations
                      // that gather the same rows (n_models for failed ones)
     int rows_hash;   // hash of idx[]: evaluations with equal rows can share their gathers
+    int comp;        // 1: the krot taps are the rotation kernel convolved with the fixed
+                     // instrumental kernel, so k_model has no instrumental stage
     double shift_log;
     long long m_off;  // offset of this evaluation's resampled model in the model buffer
     long long mp_off; // offset of its continuum-moment partials (doubles)

@@ -4,12 +4,13 @@
 //   k_prep     neighbour indices + 64 Lagrange weights, rotation taps, RV shift
 //              (src/cubic.rs:232-243, src/interpolate.rs:424-533,
 //               src/convolve_rv.rs:438-475,532-567)
-//   k_model    per (evaluation, 1792-pixel synthetic tile), all in LDS:
+//   k_model    per (evaluation, synthetic tile of TILE pixels), all in LDS:
 //              interpolate (src/interpolate.rs:930-983, incl. the n%128 tail
 //              quirk) -> rotation convolution (zero-padded "same",
 //              src/convolve_rv.rs:484-529) -> instrumental convolution
-//              (src/convolve_rv.rs:160-211 / 405-431) -> RV shift + linear
-//              resample with the f32-quantised weight (src/convolve_rv.rs:569-595)
+//              (src/convolve_rv.rs:160-211 / 405-431; a fixed instrumental kernel is
+//              folded into the rotation taps away from the grid edges) -> RV shift +
+//              linear resample with the f32-quantised weight (src/convolve_rv.rs:569-595)
 //   k_moments  per-chunk Aᵀ(f/m) in f64 (src/continuum_fitting.rs:216-226)
 //   k_solve    per (evaluation, chunk) f64 Cholesky solve of the normal equations
 //   k_chi2     blended-polynomial chi2 terms, f64 accumulation
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     const double vsini = lab[3];
     const double dvelo = LN10 * G.sg_step;
     __shared__ int s_st;
+    __shared__ int s_comp;
     const bool active = ev.active;
     if (lane == 0) s_st = active ? local_weights(*G.geom, lab[0], lab[1], lab[2], ev.idx, ev.w) : -1;
     __syncthreads();
@@ -125,6 +127,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     const int first = present ? __builtin_amdgcn_readlane(ij, __builtin_ctzll(present)) : 0;
     if (lane == 0) {
         int st = s_st, krot = 0;
+        bool comp = false;
         if (active) {
             const DevStar& S = stars[ev.star];
             if (!st && vsini < 0.0) st = ST_VSINI;
@@ -153,10 +156,21 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
                 ev.cmax = cmax;
                 long long jl = jfloor(S.fidx[S.N - 1] + sl);
                 if (!st && jl + 1 >= cmax) st = ST_INDEX;
-                ev.tile_base = (int)(jfloor(S.fidx[0] + sl) / TILE);
+                const long long j0 = jfloor(S.fidx[0] + sl);
+                ev.tile_base = (int)(j0 / TILE);
+                // Rotation then fixed instrumental broadening, both zero-padded "same"
+                // convolutions, equal one "same" convolution with the composed kernel
+                // wherever the rotation output the instrumental stage reads lies inside
+                // [0, W) (no zero padding between the stages): true for every tile when
+                // the convolved range [j0, jl + 1] stays hd pixels inside the grid.
+                const int hd = (S.Kd - 1) / 2;
+                comp = !st && S.disp_kind == DISP_FIXED && krot > 0 &&
+                       krot + S.Kd - 1 <= KROT_MAX && j0 - hd >= 0 && jl + 1 + hd <= G.W - 1;
             }
         }
         ev.status = st;
+        ev.comp = comp;
+        s_comp = comp;
         ev.krot = krot;
         ev.rows_hash = (int)hsh;
         // storage row of the first present neighbour
@@ -198,19 +212,40 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
         __syncthreads();
         const float s = s_sum;
         for (int i = lane; i < K; i += 64) h[i] = h[i] / s;
-        if (K <= KB_MAX) {
-            // reversed, zero-padded taps tpad[16 + u] = h[K - 1 - u], split into bf16
+        int Kt = K;
+        if (s_comp) {
+            // composed taps c[k] = sum_u g[u] h[k - u] of the normalised rotation taps h and
+            // the instrumental taps g, summed in f64 and rounded once to f32; staged in the
+            // split-tap area, which is written only after this
+            const DevStar& S = stars[ev.star];
+            const int Kd = S.Kd;
+            Kt = K + Kd - 1;
+            float* tmp = h + KROT_MAX;
+            __syncthreads();
+            for (int k = lane; k < Kt; k += 64) {
+                const int u0 = k - (K - 1) > 0 ? k - (K - 1) : 0;
+                const int u1 = k < Kd - 1 ? k : Kd - 1;
+                double a = 0.0;
+                for (int u = u0; u <= u1; u++) a += (double)S.dkern[u] * (double)h[k - u];
+                tmp[k] = (float)a;
+            }
+            __syncthreads();
+            for (int k = lane; k < Kt; k += 64) h[k] = tmp[k];
+            if (lane == 0) ev.krot = Kt;
+        }
+        if (Kt <= KB_MAX) {
+            // reversed, zero-padded taps tpad[16 + u] = h[Kt - 1 - u], split into bf16
             // hi/mid/lo, as pairs tw[part][parity][q] = (tpad[2q + parity], tpad[2q + parity + 1])
             __syncthreads();
             unsigned int* tw = (unsigned int*)(h + KROT_MAX);
-            const int nq = 16 * ((K + 46) >> 5) + 8;  // every pair the MFMA groups read
+            const int nq = 16 * ((Kt + 46) >> 5) + 8;  // every pair the MFMA groups read
             for (int i = lane; i < 2 * nq; i += 64) {
                 const int par = i >= nq, q = i - par * nq;
                 unsigned short v[2][3];
 #pragma unroll
                 for (int z = 0; z < 2; z++) {
                     const int u = 2 * q + par + z - 16;
-                    split3((u >= 0 && u < K) ? h[K - 1 - u] : 0.0f, v[z][0], v[z][1], v[z][2]);
+                    split3((u >= 0 && u < Kt) ? h[Kt - 1 - u] : 0.0f, v[z][0], v[z][1], v[z][2]);
                 }
 #pragma unroll
                 for (int part = 0; part < 3; part++)
@@ -485,6 +520,7 @@ struct TileJob {
     int nC, hd, K, nR;
     bool mask;
     int kind;        // 0 no rotation (tile into Rb), 1 bf16 rotation, 2 f32 rotation
+    int inst;        // instrumental stage: DISP_* of the star, DISP_NONE when composed
     long long g0;    // first interpolated 8-pixel group (multiple of 8)
     int off, ng;     // logical index of rlo - K/2 (or rlo); groups interpolated
 };
@@ -516,12 +552,13 @@ __device__ __forceinline__ bool tile_job(const DevGrid& G, const DevStar* stars,
         chi = (t0 + TILE < G.W ? t0 + TILE : G.W) - 1;
     }
     J.nC = (int)(chi - J.clo + 1);
-    J.hd = (S.disp_kind == DISP_NONE) ? 0 : (S.Kd - 1) / 2;
+    J.inst = ev.comp ? (int)DISP_NONE : S.disp_kind;
+    J.hd = (J.inst == DISP_NONE) ? 0 : (S.Kd - 1) / 2;
     J.K = ev.krot;
     const int hr = J.K > 0 ? J.K / 2 : 0;
     J.rlo = J.clo - J.hd;
     J.nR = J.nC + 2 * J.hd;
-    J.mask = S.disp_kind != DISP_NONE;
+    J.mask = J.inst != DISP_NONE;
     J.kind = J.K == 0 ? 0 : (J.K <= KB_MAX ? 1 : 2);
     const long long first = J.rlo - hr;
     J.g0 = first & ~7LL;
@@ -646,7 +683,7 @@ __device__ __forceinline__ void load_taps(const TileJob& J, const float* rot, fl
         }
         if (tid < 48) Ib[islot(8 * J.ng + tid)] = 0.0f;  // the MFMA steps read past the tile
     }
-    if (S.disp_kind == DISP_FIXED) {
+    if (J.inst == DISP_FIXED) {
         const int Kp = (S.Kd + 7) & ~7;
         for (int i = tid; i < Kp; i += NT) grev[i] = i < S.Kd ? S.dkern[S.Kd - 1 - i] : 0.0f;
     }
@@ -674,11 +711,11 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
     // instrumental broadening -> C (reuses Ib)
     const int nC = J.nC, hd = J.hd;
     const float* Cb = R;
-    if (S.disp_kind == DISP_FIXED) {
+    if (J.inst == DISP_FIXED) {
         conv_tile(R, grev, S.Kd, nC, Ib, false, 0, 0);
         Cb = Ib;
         __syncthreads();
-    } else if (S.disp_kind == DISP_VARIABLE) {
+    } else if (J.inst == DISP_VARIABLE) {
         const int Kd = S.Kd;
         for (int q = tid; q < nC; q += NT) {
             const long long x = J.clo + q;

@@ -12,6 +12,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 RTOL = 1e-5
+FFT_FLOOR = 5e-7  # absolute rounding noise of an f32 FFT convolution, relative to its maximum
 
 
 def _labels_inside(spec, n, seed, vsini=(0.0, 300.0), rv=(-150.0, 150.0)):
@@ -79,6 +80,36 @@ def test_produce_model_matches_oracle(small, gpu_small, disp_kind):
     for k, lab in enumerate(labs):
         ref = O.produce_model(small.og, od, lab)
         assert _rel(out[k], ref) < RTOL, (disp_kind, lab.tolist(), _rel(out[k], ref))
+
+
+@pytest.mark.parametrize("margin", [2, 3, 4, 5, 6, 450])
+def test_composed_instrumental_taps_match_oracle(small, gpu_small, margin):
+    """With a fixed instrumental kernel (7 taps at R = 85,000, half-width hd = 3) the GPU
+    folds it into the rotation taps when the convolved range of the evaluation stays hd
+    pixels inside the grid, and convolves in two stages (zero padding between them at the
+    grid edges) otherwise.  Observed ranges starting 2-6 synthetic pixels inside the grid
+    put evaluations on both sides of that switch; all must match the oracle, which always
+    convolves in two stages.  (At 1 pixel the last resample index reaches the end of the
+    model: the reference's index panic, an error on both sides.)"""
+    from pasta_amd import synth
+    pasta, wlg, interp = gpu_small
+    wl = synth.observed_wavelengths(small.spec, 15_000, margin)
+    d = pasta.FixedResolutionDispersion(wl, 85_000.0, wlg)
+    od = O.Dispersion(small.og, O.DISP_FIXED, wl, resolution=85_000.0)
+    rvs = (0.0, -150.0, 150.0) if margin > 200 else (0.0,)
+    labs = np.array([[6123.0, -0.3, 4.21, v, rv] for v in (1.0, 10.0, 120.0, 300.0) for rv in rvs])
+    print(f"composed taps: margin {margin}, {len(labs)} labels", flush=True)
+    out = interp.produce_model_bulk(d, labs)
+    for k, lab in enumerate(labs):
+        ref = O.produce_model(small.og, od, lab).astype(np.float64)
+        # Near the right edge the observed pixels reach the last W % 128 synthetic pixels,
+        # which the interpolation's tail quirk scales by an extra 1/65535 (values ~1.5e-5).
+        # There the oracle's f32 FFT convolution (like the reference's) carries its absolute
+        # rounding noise, ~1e-7 of the spectrum's maximum, so the bound adds a floor of
+        # 5e-7 of the maximum to the 1e-5 relative bar.
+        err = np.abs(out[k].astype(np.float64) - ref)
+        bound = RTOL * np.abs(ref) + FFT_FLOOR * np.abs(ref).max()
+        assert np.all(err <= bound), (margin, lab.tolist(), float(np.max(err / bound)))
 
 
 def test_interpolate_and_convolve_matches_oracle(small, gpu_small):
