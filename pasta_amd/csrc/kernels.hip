@@ -39,6 +39,8 @@ static_assert((ILDS + KROT_MAX + 48 + RLDS + 2 * HD_MAX + 1 + 8) * 4 <= 40960,
 constexpr double C_KMS = 299792.0;   // src/convolve_rv.rs:440,547
 constexpr double LN10 = 2.302585092994045684017991454684364208;
 
+typedef const __attribute__((address_space(1))) double GlobalD;
+
 __device__ __forceinline__ long long jfloor(double x) {
     double f = floor(x);
     return f > 0.0 ? (long long)f : 0;  // `as usize` saturates at 0
@@ -740,13 +742,29 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
         for (int q = tid; q < nC; q += NT) out[J.clo + q] = Cb[q];
         return;
     }
+    // floor(idx) clamped at 0 (`as usize`) is an integer below 2^31: exact in f64 and in a
+    // 32-bit int, and (float) of it rounds the same integer as (float) of a 64-bit index.
+    // Four float indices in flight per thread (global loads, not waited on by LDS reads).
     const double sl = J.ev->shift_log;
-    for (int i = J.ilo + tid; i < J.ihi; i += NT) {
-        const double idx = S.fidx[i] + sl;
-        const long long j = jfloor(idx);
-        const float w = (float)idx - (float)j;
-        const float a = Cb[j - J.clo], c = Cb[j + 1 - J.clo];
-        out[i] = (1.0f - w) * a + w * c;
+    const int clo = (int)J.clo, ihi = J.ihi;
+    const GlobalD* fx = (const GlobalD*)S.fidx;
+    for (int i0 = J.ilo + tid; i0 < ihi; i0 += 4 * NT) {
+        double idx[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) idx[u] = i0 + u * NT < ihi ? fx[i0 + u * NT] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * NT;
+            if (i < ihi) {
+                const double x = idx[u] + sl;
+                const double f = floor(x);
+                const double jf = f > 0.0 ? f : 0.0;
+                const int j = (int)jf - clo;
+                const float w = (float)x - (float)jf;
+                const float a = Cb[j], c = Cb[j + 1];
+                out[i] = (1.0f - w) * a + w * c;
+            }
+        }
     }
 }
 
