@@ -54,7 +54,7 @@ def measured_utilisation(stars, particles):
     if (stars, particles) != (512, 64) or not os.path.exists(UTIL_FILE):
         return None
     keys = ("mfma_util_k_model", "mfma_tflops_k_model_bf16", "mfma_util_k_moments_mfma",
-            "mfma_tflops_k_moments_mfma_f64", "valu_util_k_model")
+            "mfma_tflops_k_moments_mfma_f64", "valu_util_k_model", "valu_util_k_model_2cyc")
     out = {}
     for line in open(UTIL_FILE):
         k, _, v = line.partition(":")
@@ -472,7 +472,8 @@ def main():
             if traffic else None,
             "iteration_ms": round(iter_ms, 3),
             # PMC: matrix-core busy fraction and rate of the rotation (bf16 x 3) in k_model
-            # and of the f64 continuum moments, VALU busy fraction of k_model
+            # and of the f64 continuum moments, VALU busy fraction of k_model at 4 and at 2
+            # cycles per wave64 instruction
             "mfma": measured_utilisation(len(stars), args.particles),
         },
     }
