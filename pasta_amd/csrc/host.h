@@ -17,7 +17,7 @@
 namespace pasta {
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
-                       int* sort_hist, int* perm, int* pair_eq, int2* items, int* n_items,
+                       int* sort_hist, int* perm, int* pair_eq, int4* items, int* n_items,
                        int n_model,
                        const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
                        const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
@@ -169,7 +169,7 @@ struct Batch {
     DBuf<double> d_labels, d_mpart, d_cpart, d_chi2;
     DBuf<float> d_rot, d_model, d_pf;
     DBuf<int> d_status, d_hist, d_perm, d_tbound, d_pair_eq, d_nitems;
-    DBuf<int2> d_items;
+    DBuf<int4> d_items;
     int tb_stride = 1;
     DBuf<int4> d_bmom, d_bchi2;
     DBuf<int2> d_solve;

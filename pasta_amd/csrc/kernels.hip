@@ -610,33 +610,36 @@ __device__ __forceinline__ void gather_one(const DevGrid& G, const DevEval& ev, 
     }
 }
 
-// two evaluations with the same rows: one gather and conversion, two weight sets
-template <bool FAST>
-__device__ __forceinline__ void gather_rows2(const DevGrid& G, const DevEval& ea, const DevEval& eb,
-                                             int voff, long long x0, float (&aa)[8], float (&ab)[8]) {
+// GN evaluations with the same rows: one gather and conversion, GN weight sets; each
+// evaluation's sum runs over the rows in the same order as alone
+template <int GN, bool FAST>
+__device__ __forceinline__ void gather_rows_g(const DevGrid& G, const DevEval* const (&ev)[GN],
+                                              int voff, long long x0, float (&acc)[GN][8]) {
     const int nrec = (int)(G.pitch * 2);
+    constexpr int RF = GN == 2 ? 8 : 4;  // rows in flight (registers)
 #pragma unroll 1
-    for (int j0 = 0; j0 < 64; j0 += 8) {
-        u32x4 v[8];
+    for (int j0 = 0; j0 < 64; j0 += RF) {
+        u32x4 v[RF];
 #pragma unroll
-        for (int jj = 0; jj < 8; jj++) {
+        for (int jj = 0; jj < RF; jj++) {
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                (void*)ea.rowp[j0 + jj], 0, nrec, 0x00020000);
+                (void*)ev[0]->rowp[j0 + jj], 0, nrec, 0x00020000);
             v[jj] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
         }
 #pragma unroll
-        for (int jj = 0; jj < 8; jj++) {
-            const float fa = __builtin_bit_cast(
-                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, ea.fw[j0 + jj])));
-            const float fb = __builtin_bit_cast(
-                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, eb.fw[j0 + jj])));
+        for (int jj = 0; jj < RF; jj++) {
+            float f[GN];
+#pragma unroll
+            for (int g = 0; g < GN; g++)
+                f[g] = __builtin_bit_cast(
+                    float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, ev[g]->fw[j0 + jj])));
             float u[8];
             unpack8(v[jj], u);
 #pragma unroll
             for (int t = 0; t < 8; t++) {
                 const float q = (FAST || x0 + t < G.Wq) ? u[t] : u[t] / 65535.0f;  // tail quirk :978
-                aa[t] = fmaf(q, fa, aa[t]);
-                ab[t] = fmaf(q, fb, ab[t]);
+#pragma unroll
+                for (int g = 0; g < GN; g++) acc[g][t] = fmaf(q, f[g], acc[g][t]);
             }
         }
     }
@@ -768,10 +771,111 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
     }
 }
 
-__global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restrict__ stars,
+// evaluation e alone on this tile (its job exists; the LDS is free)
+__device__ __forceinline__ void run_solo(const DevGrid& G, const DevStar* stars, const DevEval* evals,
+                                         const float* rot, int e, int tile, const int* tbound,
+                                         int tb_stride, int mode, float* model, float* Ib, float* Rb,
+                                         float* hrev, unsigned int* tw, float* grev) {
+    TileJob B;
+    tile_job(G, stars, evals, e, tile, tbound, tb_stride, mode, B);
+    load_taps(B, rot, Ib, hrev, tw, grev);
+    for (int gg = threadIdx.x; gg < B.ng; gg += NT) {
+        float acc[8];
+        gather_one(G, *B.ev, B.g0 + 8LL * gg, acc);
+        store_group(B, gg, acc, Ib, Rb);
+    }
+    finish_job(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
+}
+
+// A work item of GN (2 or 3) evaluations with the same 64 neighbour rows whose tiles span
+// at most 2 NT 8-pixel groups together: one shared interpolation, A's groups to LDS at once,
+// the others' kept in registers (and for a pair the second one's split taps, read now), then each
+// evaluation's rotation, instrumental broadening and resample in turn.  A's taps are in LDS.
+template <int GN>
+__device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars, const DevEval* evals,
+                                          const float* rot, int tile, const int* tbound,
+                                          int tb_stride, int mode, float* model, const TileJob& A,
+                                          int e1, int e2, long long pg0, int png, float* Ib,
+                                          float* Rb, float* hrev, unsigned int* tw, float* grev) {
+    // with three evaluations the registers go to the kept interpolations instead
+    const DevEval& eb = evals[e1];
+    const bool btw = GN == 2 && eb.krot > 0 && eb.krot <= KB_MAX;
+    const int bq4 = btw ? tap_quads(eb.krot) : 0;
+    u32x4 tkeep[3];
+    if (GN == 2) {
+        const u32x4* src = (const u32x4*)(rot + (size_t)e1 * ROT_STRIDE + KROT_MAX);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const int i = threadIdx.x + r * NT;
+            tkeep[r] = i < 6 * bq4 ? src[tap_quad_index(i, bq4)] : u32x4{0u, 0u, 0u, 0u};
+        }
+    }
+    const DevEval* ev[GN];
+    ev[0] = A.ev;
+    ev[1] = evals + e1;
+    if (GN > 2) ev[GN - 1] = evals + e2;
+    float keep[GN - 1][2][8];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int gg = threadIdx.x + r * NT;
+        float acc[GN][8];
+#pragma unroll
+        for (int g = 0; g < GN; g++)
+#pragma unroll
+            for (int t = 0; t < 8; t++) acc[g][t] = 0.0f;
+        if (gg < png) {
+            const long long x0 = pg0 + 8LL * gg;
+            const bool inside = x0 >= 0 && x0 < G.W;
+            const int voff = inside ? (int)(2 * x0) : OOB_OFFSET;
+            if (__all(x0 + 7 < G.Wq)) gather_rows_g<GN, true>(G, ev, voff, x0, acc);
+            else gather_rows_g<GN, false>(G, ev, voff, x0, acc);
+            if (x0 + 8 > G.W) {
+#pragma unroll
+                for (int t = 0; t < 8; t++)
+                    if (x0 + t >= G.W)
+#pragma unroll
+                        for (int g = 0; g < GN; g++) acc[g][t] = 0.0f;
+            }
+            const long long ga = (x0 - A.g0) >> 3;
+            if (ga >= 0 && ga < A.ng) store_group(A, (int)ga, acc[0], Ib, Rb);
+        }
+#pragma unroll
+        for (int g = 1; g < GN; g++)
+#pragma unroll
+            for (int t = 0; t < 8; t++) keep[g - 1][r][t] = acc[g][t];
+    }
+    finish_job(G, A, rot, model, mode, Ib, Rb, hrev, tw, grev);
+#pragma unroll
+    for (int g = 1; g < GN; g++) {
+        __syncthreads();
+        TileJob B;
+        tile_job(G, stars, evals, g == 1 ? e1 : e2, tile, tbound, tb_stride, mode, B);
+        if (GN == 2) {
+            if (btw) {
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    const int i = threadIdx.x + r * NT;
+                    if (i < 6 * bq4) ((u32x4*)tw)[tap_quad_index(i, bq4)] = tkeep[r];
+                }
+            }
+            load_taps(B, rot, Ib, hrev, tw, grev, false);
+        } else {
+            load_taps(B, rot, Ib, hrev, tw, grev);
+        }
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int gg = threadIdx.x + r * NT;
+            const long long gb = (pg0 + 8LL * gg - B.g0) >> 3;
+            if (gg < png && gb >= 0 && gb < B.ng) store_group(B, (int)gb, keep[g - 1][r], Ib, Rb);
+        }
+        finish_job(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
+    }
+}
+
+__global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __restrict__ stars,
                                               const DevEval* __restrict__ evals,
                                               const float* __restrict__ rot,
-                                              const int2* __restrict__ items,
+                                              const int4* __restrict__ items,
                                               const int* __restrict__ n_items, int n_evals,
                                               const int* __restrict__ tbound, int tb_stride,
                                               float* __restrict__ model, int mode) {
@@ -793,82 +897,42 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     const int k = b - tile * n_evals;
     if (k >= __builtin_amdgcn_readfirstlane(*n_items)) return;
     // the work item's evaluations are uniform: keep them (and so their weight loads) scalar
-    const int2 it = make_int2(__builtin_amdgcn_readfirstlane(items[k].x),
-                              __builtin_amdgcn_readfirstlane(items[k].y));
+    const int ids[3] = {__builtin_amdgcn_readfirstlane(items[k].x),
+                        __builtin_amdgcn_readfirstlane(items[k].y),
+                        __builtin_amdgcn_readfirstlane(items[k].z)};
     TileJob A;
-    const bool hasA = tile_job(G, stars, evals, it.x, tile, tbound, tb_stride, mode, A);
-    bool hasB = false;
-    long long bg0 = 0;
-    int bng = 0;
-    const DevEval* bev = nullptr;
-    if (it.y >= 0) {
+    const bool hasA = tile_job(G, stars, evals, ids[0], tile, tbound, tb_stride, mode, A);
+    // the other evaluations' interpolated ranges (their jobs are formed again when they run)
+    bool hasB = false, hasC = false;
+    long long bg0 = 0, cg0 = 0;
+    int bng = 0, cng = 0;
+    if (ids[1] >= 0) {
         TileJob T;
-        hasB = tile_job(G, stars, evals, it.y, tile, tbound, tb_stride, mode, T);
+        hasB = tile_job(G, stars, evals, ids[1], tile, tbound, tb_stride, mode, T);
         bg0 = T.g0;
         bng = T.ng;
-        bev = evals + it.y;
     }
-    const long long pg0 = min(A.g0, bg0);
-    const int png = (int)((max(A.g0 + 8LL * A.ng, bg0 + 8LL * bng) - pg0) >> 3);
+    if (ids[2] >= 0) {
+        TileJob T;
+        hasC = tile_job(G, stars, evals, ids[2], tile, tbound, tb_stride, mode, T);
+        cg0 = T.g0;
+        cng = T.ng;
+    }
     if (hasA) load_taps(A, rot, Ib, hrev, tw, grev);
-    if (hasA && hasB && png <= 2 * NT) {
-        // shared interpolation: A's groups go to LDS at once, B's wait in registers, and
-        // so do B's split taps (read now, stored after A's rotation)
-        const bool btw = bev->krot > 0 && bev->krot <= KB_MAX;
-        const int bq4 = btw ? tap_quads(bev->krot) : 0;
-        u32x4 tkeep[3];
-        {
-            const u32x4* src = (const u32x4*)(rot + (size_t)it.y * ROT_STRIDE + KROT_MAX);
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const int i = threadIdx.x + r * NT;
-                tkeep[r] = i < 6 * bq4 ? src[tap_quad_index(i, bq4)] : u32x4{0u, 0u, 0u, 0u};
+    if (hasA && hasB) {
+        long long pg0 = min(A.g0, bg0), pg1 = max(A.g0 + 8LL * A.ng, bg0 + 8LL * bng);
+        if (hasC) {
+            const long long q0 = min(pg0, cg0), q1 = max(pg1, cg0 + 8LL * cng);
+            if (((q1 - q0) >> 3) <= 2 * NT) {
+                run_group<3>(G, stars, evals, rot, tile, tbound, tb_stride, mode, model, A, ids[1],
+                             ids[2], q0, (int)((q1 - q0) >> 3), Ib, Rb, hrev, tw, grev);
+                return;
             }
+        } else if (((pg1 - pg0) >> 3) <= 2 * NT) {
+            run_group<2>(G, stars, evals, rot, tile, tbound, tb_stride, mode, model, A, ids[1], -1,
+                         pg0, (int)((pg1 - pg0) >> 3), Ib, Rb, hrev, tw, grev);
+            return;
         }
-        float keep[2][8];
-#pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int gg = threadIdx.x + r * NT;
-            float aa[8], ab[8];
-#pragma unroll
-            for (int t = 0; t < 8; t++) { aa[t] = 0.0f; ab[t] = 0.0f; }
-            if (gg < png) {
-                const long long x0 = pg0 + 8LL * gg;
-                const bool inside = x0 >= 0 && x0 < G.W;
-                const int voff = inside ? (int)(2 * x0) : OOB_OFFSET;
-                if (__all(x0 + 7 < G.Wq)) gather_rows2<true>(G, evals[it.x], evals[it.y], voff, x0, aa, ab);
-                else gather_rows2<false>(G, evals[it.x], evals[it.y], voff, x0, aa, ab);
-                if (x0 + 8 > G.W) {
-#pragma unroll
-                    for (int t = 0; t < 8; t++)
-                        if (x0 + t >= G.W) { aa[t] = 0.0f; ab[t] = 0.0f; }
-                }
-                const long long ga = (x0 - A.g0) >> 3;
-                if (ga >= 0 && ga < A.ng) store_group(A, (int)ga, aa, Ib, Rb);
-            }
-#pragma unroll
-            for (int t = 0; t < 8; t++) keep[r][t] = ab[t];
-        }
-        finish_job(G, A, rot, model, mode, Ib, Rb, hrev, tw, grev);
-        __syncthreads();
-        TileJob B;
-        tile_job(G, stars, evals, it.y, tile, tbound, tb_stride, mode, B);
-        if (btw) {
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const int i = threadIdx.x + r * NT;
-                if (i < 6 * bq4) ((u32x4*)tw)[tap_quad_index(i, bq4)] = tkeep[r];
-            }
-        }
-        load_taps(B, rot, Ib, hrev, tw, grev, false);
-#pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int gg = threadIdx.x + r * NT;
-            const long long gb = (pg0 + 8LL * gg - B.g0) >> 3;
-            if (gg < png && gb >= 0 && gb < B.ng) store_group(B, (int)gb, keep[r], Ib, Rb);
-        }
-        finish_job(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
-        return;
     }
     // one evaluation at a time
     if (hasA) {
@@ -881,15 +945,13 @@ __global__ __launch_bounds__(NT) void k_model(DevGrid G, const DevStar* __restri
     }
     if (hasB) {
         if (hasA) __syncthreads();
-        TileJob B;
-        tile_job(G, stars, evals, it.y, tile, tbound, tb_stride, mode, B);
-        load_taps(B, rot, Ib, hrev, tw, grev);
-        for (int gg = threadIdx.x; gg < B.ng; gg += NT) {
-            float acc[8];
-            gather_one(G, *B.ev, B.g0 + 8LL * gg, acc);
-            store_group(B, gg, acc, Ib, Rb);
-        }
-        finish_job(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
+        run_solo(G, stars, evals, rot, ids[1], tile, tbound, tb_stride, mode, model, Ib, Rb, hrev,
+                 tw, grev);
+    }
+    if (hasC) {
+        if (hasA || hasB) __syncthreads();
+        run_solo(G, stars, evals, rot, ids[2], tile, tbound, tb_stride, mode, model, Ib, Rb, hrev,
+                 tw, grev);
     }
 }
 
@@ -1567,40 +1629,39 @@ __device__ __forceinline__ int block_scan(int v, int* wsum /* 16 */) {
     return r;
 }
 
-// Greedy pairing along each run of equal neighbour rows: positions 0-1, 2-3, ... of a
-// run share; items[k] = (first evaluation, second or -1), in neighbour-row order.
+// Greedy grouping along each run of equal neighbour rows: positions 0-2, 3-5, ... of a
+// run share their gathers; items[k] = (first evaluation, second or -1, third or -1, 0), in
+// neighbour-row order.
 __global__ __launch_bounds__(1024) void k_pair_items(const int* __restrict__ perm,
                                                      const int* __restrict__ eq, int n,
-                                                     int2* __restrict__ items,
+                                                     int4* __restrict__ items,
                                                      int* __restrict__ n_items) {
     __shared__ int wsum[16];
-    __shared__ int s_first[1024];
-    __shared__ int s_carry[3];  // run start of the chunk's last element, its pairing flag, items
+    __shared__ int s_carry[2];  // run start of the chunk's last element, items so far
     const int tid = threadIdx.x;
-    if (tid == 0) { s_carry[0] = 0; s_carry[1] = 0; s_carry[2] = 0; }
+    if (tid == 0) { s_carry[0] = 0; s_carry[1] = 0; }
     __syncthreads();
     for (int base = 0; base < n; base += 1024) {
         const int i = base + tid;
         const bool valid = i < n;
         const int mark = (valid && (i == 0 || !eq[i - 1])) ? i : -1;
         const int start = max(block_scan<true>(mark, wsum), s_carry[0]);
-        const bool first = valid && eq[i] && (((i - start) & 1) == 0);
-        s_first[tid] = first;
-        __syncthreads();
-        const bool second = tid > 0 ? s_first[tid - 1] != 0 : s_carry[1] != 0;
-        const int emit = (valid && !second) ? 1 : 0;
+        const int emit = (valid && (i - start) % 3 == 0) ? 1 : 0;
         const int incl = block_scan<false>(emit, wsum);
-        const int k = s_carry[2] + incl - emit;
-        if (emit) items[k] = make_int2(perm[i], first ? perm[i + 1] : -1);
+        const int k = s_carry[1] + incl - emit;
+        if (emit) {
+            const bool two = eq[i] != 0;
+            const bool three = two && eq[i + 1] != 0;  // eq[i] != 0 implies i + 1 < n
+            items[k] = make_int4(perm[i], two ? perm[i + 1] : -1, three ? perm[i + 2] : -1, 0);
+        }
         __syncthreads();
         if (tid == 1023) {
             s_carry[0] = start;
-            s_carry[1] = first;
-            s_carry[2] = k + emit;
+            s_carry[1] = k + emit;
         }
         __syncthreads();
     }
-    if (tid == 0) *n_items = s_carry[2];
+    if (tid == 0) *n_items = s_carry[1];
 }
 
 hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const int4* bmom,
@@ -1610,7 +1671,7 @@ hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const 
 
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
-                       int* sort_hist, int* perm, int* pair_eq, int2* items, int* n_items,
+                       int* sort_hist, int* perm, int* pair_eq, int4* items, int* n_items,
                        int n_model,
                        const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
                        const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
