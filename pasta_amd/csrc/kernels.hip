@@ -507,9 +507,9 @@ __device__ __forceinline__ void conv_rot_bf16(const unsigned short* Ib3, const u
 
 // ---------------------------------------------------------------------------
 // k_model: one workgroup per (work item, synthetic tile).  A work item is one
-// evaluation, or two evaluations that gather the same 64 neighbour rows: their
-// interpolation shares every 16-byte gather and u16 conversion (one conversion and two
-// FMAs per pixel-row for the pair), then each runs its own rotation, instrumental
+// evaluation, or two or three evaluations that gather the same 64 neighbour rows: their
+// interpolation shares every 16-byte gather and u16 conversion (one conversion and one
+// FMA per evaluation per pixel-row), then each runs its own rotation, instrumental
 // broadening and resample.  Each evaluation's arithmetic is the same as alone.
 // ---------------------------------------------------------------------------
 struct TileJob {
