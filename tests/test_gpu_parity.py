@@ -270,6 +270,34 @@ def test_results_are_deterministic(small, gpu_small):
     assert np.array_equal(a, c[::-1])
 
 
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 7])
+def test_same_cell_groups_match_alone_and_oracle(small, gpu_small, n):
+    """Evaluations whose labels share one grid cell gather the same 64 neighbour rows, and the
+    model kernel interpolates up to three of them from one gather (groups of 3, 3, ..., then 2
+    or 1).  Each one's chi2 must equal, bit for bit, its chi2 evaluated alone, and match the
+    oracle."""
+    pasta, wlg, interp = gpu_small
+    d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
+    star = pasta.Star(interp, d, pasta.ChunkContinuumFitter(small.wl, 5, 4, 0.2), small.flux,
+                      small.var)
+    rng = np.random.default_rng(40 + n)
+    # inside the cell Teff (6000, 6250), [M/H] (-0.5, -0.25), logg (4.0, 4.5): same rows
+    labs = np.stack([rng.uniform(6010, 6240, n), rng.uniform(-0.49, -0.26, n),
+                     rng.uniform(4.01, 4.49, n), rng.uniform(0, 300, n),
+                     rng.uniform(-150, 150, n)], axis=1)
+    idx = [small.og.local_weights(*lab[:3])[0] for lab in labs]
+    assert all(np.array_equal(idx[0], r) for r in idx), "labels must share their neighbour rows"
+    print(f"same-cell groups: {n} evaluations", flush=True)
+    chi2, st, _ = star.chi2(labs)
+    assert not st.any()
+    for k, lab in enumerate(labs):
+        alone, st1, _ = star.chi2(lab[None, :].copy())
+        assert not st1.any()
+        assert chi2[k] == alone[0], (n, k, chi2[k], alone[0])
+        ref, _ = O.cost(small.og, small.odisp, small.ocont, small.flux, small.var, lab)
+        assert abs(chi2[k] - ref) / ref < RTOL, (n, lab.tolist(), chi2[k], ref)
+
+
 def test_chi2_bulk_matches_oracle(small, gpu_small):
     """chi2_bulk (src/lib.rs:1297-1338): per-spectrum wavelengths, NoConvolution and
     ChunkFitter(5, 8, 0.2) hard-coded, one label each, in one batched launch."""
