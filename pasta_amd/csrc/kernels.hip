@@ -164,9 +164,10 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
                 // convolutions, equal one "same" convolution with the composed kernel
                 // wherever the rotation output the instrumental stage reads lies inside
                 // [0, W) (no zero padding between the stages): true for every tile when
-                // the convolved range [j0, jl + 1] stays hd pixels inside the grid.
+                // the convolved range [j0, jl + 1] stays hd pixels inside the grid.  (Both
+                // kernels have odd lengths, so the composed one is centred like its parts.)
                 const int hd = (S.Kd - 1) / 2;
-                comp = !st && S.disp_kind == DISP_FIXED && krot > 0 &&
+                comp = !st && S.disp_kind == DISP_FIXED && krot > 0 && (S.Kd & 1) &&
                        krot + S.Kd - 1 <= KROT_MAX && j0 - hd >= 0 && jl + 1 + hd <= G.W - 1;
             }
         }
