@@ -617,7 +617,8 @@ template <int GN, bool FAST>
 __device__ __forceinline__ void gather_rows_g(const DevGrid& G, const DevEval* const (&ev)[GN],
                                               int voff, long long x0, float (&acc)[GN][8]) {
     const int nrec = (int)(G.pitch * 2);
-    constexpr int RF = GN == 2 ? 8 : 4;  // rows in flight (registers)
+    constexpr int RF = 8;  // rows in flight (registers)
+    static_assert(64 % RF == 0, "whole batches of the 64 neighbour rows");
 #pragma unroll 1
     for (int j0 = 0; j0 < 64; j0 += RF) {
         u32x4 v[RF];
