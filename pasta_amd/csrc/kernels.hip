@@ -852,13 +852,11 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
         __syncthreads();
         TileJob B;
         tile_job(G, stars, evals, g == 1 ? e1 : e2, tile, tbound, tb_stride, mode, B);
-        if (GN == 2) {
-            if (btw) {
+        if (g == 1 && btw) {  // the second evaluation's split taps, read before A's rotation
 #pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    const int i = threadIdx.x + r * NT;
-                    if (i < 6 * bq4) ((u32x4*)tw)[tap_quad_index(i, bq4)] = tkeep[r];
-                }
+            for (int r = 0; r < 3; r++) {
+                const int i = threadIdx.x + r * NT;
+                if (i < 6 * bq4) ((u32x4*)tw)[tap_quad_index(i, bq4)] = tkeep[r];
             }
             load_taps(B, rot, Ib, hrev, tw, grev, false);
         } else {
