@@ -139,6 +139,16 @@ int pasta_star_eval_bytes(const pasta_star *s, int64_t *bytes);
  * parameters.  Returns PASTA_ERR_MODEL if any status is non-zero. */
 int pasta_eval_chi2(pasta_star *s, const double *labels, int P, double *chi2, int *status,
                     float *params);
+/* No counterpart in the reference.  pasta_eval_chi2 for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): same arguments, same kernels and bit-identical results, but every device
+ * buffer beyond the star (itself created beforehand) and the caller's host arrays comes
+ * from one 64 MiB scratch array per GPU, held until the process ends.  An evaluation set
+ * needing more scratch (about P x N > 15 million, e.g. more than 64 particles at
+ * N = 200,000) is rejected with PASTA_ERR_ARG.  Calls in one process are serialised; the
+ * array keeps the last call's contents.  Runs on the null stream. */
+int pasta_eval_chi2_noalloc(pasta_star *s, const double *labels, int P, double *chi2,
+                            int *status, float *params);
 /* chi2 of n stars with one label vector each (labels n x 5), in one batched launch:
  * the evaluation behind chi2_bulk (src/lib.rs:1297-1338), which builds each star from
  * its own wavelengths with NoConvolution and ChunkFitter(5, 8, 0.2).  chi2[k] is NaN

@@ -165,6 +165,7 @@ SIGNATURES = {
     "pasta_star_destroy": [_P],
     "pasta_star_eval_bytes": [_P, _i64p],
     "pasta_eval_chi2": [_P, _dp, C.c_int, _dp, _ip, _fp],
+    "pasta_eval_chi2_noalloc": [_P, _dp, C.c_int, _dp, _ip, _fp],
     "pasta_produce_model": [_P, _P, _dp, C.c_int, _fp, _ip],
     "pasta_interpolate": [_P, _dp, C.c_int, _fp, _ip],
     "pasta_interpolate_and_convolve": [_P, _P, _dp, C.c_int, _fp, _ip],

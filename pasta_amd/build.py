@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpasta_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "api.hip", "binary.hip"]
+SOURCES = ["kernels.hip", "api.hip", "binary.hip", "noalloc.hip"]
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
