@@ -351,7 +351,8 @@ typedef struct pasta_server_config {
 } pasta_server_config;
 int pasta_server_create(pasta_grid *g, const pasta_server_config *cfg, pasta_server **out);
 int pasta_server_destroy(pasta_server *sv);
-/* The server takes ownership of `s` (destroyed when the request is reaped). */
+/* The server takes ownership of `s` on success: its buffers are freed when the request
+ * finishes or is cancelled (the result stays readable through pasta_server_poll). */
 int pasta_server_submit(pasta_server *sv, pasta_star *s, const pasta_fit_settings *st,
                         uint64_t seed, uint64_t *request_id);
 /* Admit waiting requests, run iters_per_step iterations, retire finished stars.
@@ -361,8 +362,11 @@ int pasta_server_step(pasta_server *sv, int *active, int *waiting);
 int pasta_server_poll(pasta_server *sv, uint64_t request_id, int *done, pasta_fit_result *out);
 int pasta_server_cancel(pasta_server *sv, uint64_t request_id);
 /* counters: [0] evals done, [1] requests finished, [2] iterations run,
- * [3] HBM bytes in use by in-flight stars */
-int pasta_server_stats(const pasta_server *sv, int64_t out[4]);
+ * [3] HBM bytes in use by in-flight stars (their evaluation buffers, PSO state and the
+ * stars themselves), [4] device-busy microseconds (first evaluations of admitted stars and
+ * PSO iterations, HIP events on the server's stream), [5] batch rebuilds (admissions and
+ * retirements; the PSO state of stars in flight moves between batches on the device) */
+int pasta_server_stats(const pasta_server *sv, int64_t out[6]);
 /* With timing on: [0] last iteration's k_model ms, [1] sum of k_model ms, [2] last
  * iteration ms (PSO update through best update), [3] sum of iteration ms,
  * [4] iterations summed.  reset != 0 clears the sums after reading. */

@@ -104,11 +104,14 @@ _sig = {
     "or_build_continuum": (C.c_int, [C.c_void_p, _fp, _fp]),
     "or_produce_model": (C.c_int, [C.POINTER(_Model), C.c_void_p, _dp, _fp]),
     "or_produce_model_f64": (C.c_int, [C.POINTER(_Model), C.c_void_p, _dp, _dp]),
+    "or_produce_model_f64q": (C.c_int, [C.POINTER(_Model), C.c_void_p, _dp, _dp]),
     "or_interpolate_and_convolve": (C.c_int, [C.POINTER(_Model), C.c_void_p, _dp, _fp]),
     "or_cost": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, _dp, _fp]),
     "or_cost_batch": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, C.c_int,
                                 C.c_int, _dp]),
     "or_cost_f64": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, _dp]),
+    "or_cost_f64q": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, _dp,
+                               _dp]),
     "or_rng_u64": (C.c_uint64, [C.c_uint64, C.c_uint64]),
     "or_rng_uniform": (C.c_double, [C.c_uint64, C.c_uint64]),
     "or_bounds_outer": (C.c_int, [C.POINTER(_Bounds), _dp, _dp]),
@@ -373,11 +376,14 @@ def fixed_gauss_kernel(resolution, log_step):
 
 
 def produce_model(grid, disp, labels, f64=False):
+    """f64=False: the f32-faithful restatement; True: every step in FP64 with the exact
+    resample weight; "q": FP64 with the reference's f32-quantised resample weight."""
     lab = _d(labels)
     M = grid.model()
     if f64:
         out = np.zeros(disp.n)
-        rc = _L.or_produce_model_f64(C.byref(M), disp._h, _p(lab, C.c_double), _p(out, C.c_double))
+        fn = _L.or_produce_model_f64q if f64 == "q" else _L.or_produce_model_f64
+        rc = fn(C.byref(M), disp._h, _p(lab, C.c_double), _p(out, C.c_double))
     else:
         out = np.zeros(disp.n, np.float32)
         rc = _L.or_produce_model(C.byref(M), disp._h, _p(lab, C.c_double), _p(out, C.c_float))
@@ -395,11 +401,17 @@ def interpolate_and_convolve(grid, disp, labels4):
 
 
 def cost(grid, disp, cont, flux, var, labels, f64=False):
-    """chi2 of one label vector; returns (chi2, params) or raises OracleError."""
+    """chi2 of one label vector; returns (chi2, params) or raises OracleError.  f64 as in
+    produce_model ("q": FP64 with the quantised resample weight, params in f64)."""
     lab = _d(labels)
     flux, var = _f(flux), _f(var)
     M = grid.model()
     c = C.c_double()
+    if f64 == "q":
+        params = np.zeros(max(1, cont.n_params))
+        _chk(_L.or_cost_f64q(C.byref(M), disp._h, cont._h, _p(flux, C.c_float), _p(var, C.c_float),
+                             _p(lab, C.c_double), C.byref(c), _p(params, C.c_double)), "cost")
+        return c.value, params
     if f64:
         _chk(_L.or_cost_f64(C.byref(M), disp._h, cont._h, _p(flux, C.c_float), _p(var, C.c_float),
                             _p(lab, C.c_double), C.byref(c)), "cost")

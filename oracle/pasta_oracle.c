@@ -196,20 +196,28 @@ int or_get_teff_index_limits_at(const or_grid *g, double logg, int *lo, int *hi)
 /* Range::find_neighbors (interpolate.rs:105-159) */
 int or_find_neighbors(const double *v, int n, double x, int left, int right, int out[4]) {
     int r = right_index(v, n, x);
+    /* some[q]: the reference returns Some(index) in slot q (None slots are -1) */
+    int some[4] = {1, 1, 1, 1};
     if (r >= 0) {
         int i = r;
         if (i < left || i > right) return OR_ERR_BOUNDS;
-        if (i == left) { out[0] = -1; out[1] = i; out[2] = i + 1; out[3] = i + 2; }
-        else if (i == right) { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = -1; }
+        if (i == left) { some[0] = 0; out[1] = i; out[2] = i + 1; out[3] = i + 2; }
+        else if (i == right) { out[0] = i - 2; out[1] = i - 1; out[2] = i; some[3] = 0; }
         else if (i == right - 1) { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
         else { out[0] = i - 1; out[1] = i; out[2] = i + 1; out[3] = i + 2; }
-        return OR_OK;
+    } else {
+        int i = -r - 1;
+        if (i <= left || i > right) return OR_ERR_BOUNDS;
+        if (i == left + 1) { some[0] = 0; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
+        else if (i == right) { out[0] = i - 2; out[1] = i - 1; out[2] = i; some[3] = 0; }
+        else { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
     }
-    int i = -r - 1;
-    if (i <= left || i > right) return OR_ERR_BOUNDS;
-    if (i == left + 1) { out[0] = -1; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
-    else if (i == right) { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = -1; }
-    else { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
+    /* Some(index) outside the axis: the reference's value lookup panics
+       (interpolate.rs:483-530; i - 2 below zero wraps past the end first) */
+    for (int q = 0; q < 4; q++) {
+        if (!some[q]) { out[q] = -1; continue; }
+        if (out[q] < 0 || out[q] >= n) return OR_ERR_NEIGHBORS;
+    }
     return OR_OK;
 }
 
@@ -798,7 +806,12 @@ int or_produce_model(const or_model *M, const or_disp *d, const double lab[5], f
 }
 
 /* ---------------- FP64 mode ---------------- */
-int or_produce_model_f64(const or_model *M, const or_disp *d, const double lab[5], double *out) {
+/* qweight != 0: the resample weight is the reference's f32(index) - f32(j)
+ * (convolve_rv.rs:570-572), used in f64; every other step stays f64.  This separates the
+ * f32 arithmetic error of a path from the weight quantisation the reference's own result
+ * carries (up to 7.8e-3 of a pixel step for indices >= 131072). */
+static int produce_model_f64_impl(const or_model *M, const or_disp *d, const double lab[5],
+                                  double *out, int qweight) {
     int W = M->sg.n;
     double *interp = malloc(sizeof(double) * W);
     int rc = or_interpolate_f64(M->g, M->spectra, M->factors, W, lab[0], lab[1], lab[2], interp);
@@ -858,13 +871,21 @@ int or_produce_model_f64(const or_model *M, const or_disp *d, const double lab[5
         double index = d->fidx[i] + shift_log;
         double fl = floor(index);
         long j = fl > 0 ? (long)fl : 0;
-        double w = index - (double)j;
+        double w = qweight ? (double)((float)index - (float)j) : index - (double)j;
         long a = s.start + j, b = a + 1;
         if (b >= s.alen) { free(s.a); return OR_ERR_INDEX; }
         out[i] = (1.0 - w) * s.a[a] + w * s.a[b];
     }
     free(s.a);
     return OR_OK;
+}
+
+int or_produce_model_f64(const or_model *M, const or_disp *d, const double lab[5], double *out) {
+    return produce_model_f64_impl(M, d, lab, out, 0);
+}
+
+int or_produce_model_f64q(const or_model *M, const or_disp *d, const double lab[5], double *out) {
+    return produce_model_f64_impl(M, d, lab, out, 1);
 }
 
 /* ======================================================================
@@ -1385,13 +1406,24 @@ int or_cost(const or_model *M, const or_disp *d, const or_cont *c, const float *
     return rc;
 }
 
-int or_cost_f64(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
-                const float *var, const double lab[5], double *chi2) {
+static int cost_f64_impl(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                         const float *var, const double lab[5], double *chi2, double *params,
+                         int qweight) {
     double *synth = malloc(sizeof(double) * d->n_obs);
-    int rc = or_produce_model_f64(M, d, lab, synth);
-    if (!rc) rc = or_fit_continuum_f64(c, flux, var, synth, NULL, chi2);
+    int rc = produce_model_f64_impl(M, d, lab, synth, qweight);
+    if (!rc) rc = or_fit_continuum_f64(c, flux, var, synth, params, chi2);
     free(synth);
     return rc;
+}
+
+int or_cost_f64(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                const float *var, const double lab[5], double *chi2) {
+    return cost_f64_impl(M, d, c, flux, var, lab, chi2, NULL, 0);
+}
+
+int or_cost_f64q(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                 const float *var, const double lab[5], double *chi2, double *params) {
+    return cost_f64_impl(M, d, c, flux, var, lab, chi2, params, 1);
 }
 
 /* ======================================================================

@@ -149,6 +149,8 @@ typedef struct or_model {
 int or_produce_model(const or_model *M, const or_disp *d, const double lab[5], float *out);
 /* FP64 mode: interpolation with f64 weights, direct f64 convolutions, exact resample weight */
 int or_produce_model_f64(const or_model *M, const or_disp *d, const double lab[5], double *out);
+/* FP64 mode with the reference's f32-quantised resample weight (convolve_rv.rs:570-572) */
+int or_produce_model_f64q(const or_model *M, const or_disp *d, const double lab[5], double *out);
 /* rotation+dispersion only, the synthetic-grid segment (interpolate_and_convolve) */
 int or_interpolate_and_convolve(const or_model *M, const or_disp *d, const double lab[4],
                                 float *out /* W */);
@@ -158,6 +160,10 @@ int or_cost_batch(const or_model *M, const or_disp *d, const or_cont *c, const f
                   const float *var, const double *labels, int n, int n_threads, double *chi2);
 int or_cost_f64(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
                 const float *var, const double lab[5], double *chi2);
+/* or_cost_f64 with the quantised resample weight; params (nullable) the f64 continuum
+ * parameters */
+int or_cost_f64q(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                 const float *var, const double lab[5], double *chi2, double *params);
 
 /* ---------------- PSO (particleswarm.rs, bounds.rs, fitting.rs) ---------------- */
 typedef struct or_pso_settings {

@@ -12,6 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpasta_hip.so")
 
 PASTA_OK = 0
+PASTA_ERR_ARG = -1
+PASTA_ERR_HIP = -2
+PASTA_ERR_NOMEM = -3
 PASTA_ERR_MODEL = -4
 STATUS_TEXT = {
     0: "ok",

@@ -48,6 +48,18 @@ def _raise(rc, what):
     return rc
 
 
+def _batch_rc(rc, what):
+    """Return code of a batched call whose per-item failures are in its status array:
+    PASTA_ERR_MODEL passes (the statuses say which items failed), a bad argument raises
+    ValueError, and a failure of the call itself (HIP error, allocation) raises
+    PastaError, since no status was written."""
+    if rc == 0 or rc == _lib.PASTA_ERR_MODEL:
+        return rc
+    if rc == _lib.PASTA_ERR_ARG:
+        raise ValueError(f"{what}: {_lib.last_error()}")
+    raise _lib.PastaError(f"{what} failed ({rc}): {_lib.last_error()}")
+
+
 class WlGrid:
     """WlGrid(min, step, len, log=False) (src/lib.rs:630-653).  Only log-spaced
     synthetic grids are supported by the HIP path (every grid this path is built for is
@@ -575,8 +587,8 @@ class InMemInterpolator:
         lab = f64(labels).reshape(-1, 3)
         out = np.zeros((len(lab), self.n_pix), np.float32)
         st = np.zeros(len(lab), np.int32)
-        lib.pasta_interpolate(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_float),
-                              ptr(st, C.c_int))
+        _batch_rc(lib.pasta_interpolate(self._h, ptr(lab, C.c_double), len(lab),
+                                        ptr(out, C.c_float), ptr(st, C.c_int)), "interpolate")
         return out, st
 
     def interpolate_linear(self, teff, m, logg):
@@ -590,8 +602,9 @@ class InMemInterpolator:
         lab = f64(labels).reshape(-1, 3)
         out = np.zeros((len(lab), self.n_pix), np.float32)
         st = np.zeros(len(lab), np.int32)
-        lib.pasta_interpolate_linear(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_float),
-                                     ptr(st, C.c_int))
+        _batch_rc(lib.pasta_interpolate_linear(self._h, ptr(lab, C.c_double), len(lab),
+                                               ptr(out, C.c_float), ptr(st, C.c_int)),
+                  "interpolate_linear")
         return out, st
 
     def produce_binary_model_norm(self, continuum_interpolator, dispersion, labels1, labels2,
@@ -656,8 +669,9 @@ class InMemInterpolator:
         n = len(dispersion.wl)
         out = np.zeros((len(lab), n), np.float32)
         st = np.zeros(len(lab), np.int32)
-        lib.pasta_produce_model(self._h, dispersion.handle(self), ptr(lab, C.c_double), len(lab),
-                                ptr(out, C.c_float), ptr(st, C.c_int))
+        _batch_rc(lib.pasta_produce_model(self._h, dispersion.handle(self), ptr(lab, C.c_double),
+                                          len(lab), ptr(out, C.c_float), ptr(st, C.c_int)),
+                  "produce_model")
         return out, st
 
     def chi2_bulk(self, observed_wavelength, observed_flux, observed_var, labels, progress=None,
@@ -752,8 +766,9 @@ class Star:
         out = np.zeros(len(lab))
         st = np.zeros(len(lab), np.int32)
         params = np.zeros((len(lab), max(1, self.n_params)), np.float32)
-        lib.pasta_eval_chi2(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_double),
-                            ptr(st, C.c_int), ptr(params, C.c_float))
+        _batch_rc(lib.pasta_eval_chi2(self._h, ptr(lab, C.c_double), len(lab),
+                                      ptr(out, C.c_double), ptr(st, C.c_int),
+                                      ptr(params, C.c_float)), "chi2")
         return out, st, params
 
 
@@ -883,8 +898,8 @@ class _Binary:
         lab = f64(labels11).reshape(-1, 11)
         out = np.zeros((len(lab), self.n), np.float32)
         st = np.zeros(len(lab), np.int32)
-        lib.pasta_binary_model(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_float),
-                               ptr(st, C.c_int))
+        _batch_rc(lib.pasta_binary_model(self._h, ptr(lab, C.c_double), len(lab),
+                                         ptr(out, C.c_float), ptr(st, C.c_int)), "binary model")
         return out, st
 
     def chi2(self, labels11, params=False):
@@ -892,8 +907,10 @@ class _Binary:
         out = np.zeros(len(lab))
         st = np.zeros(len(lab), np.int32)
         prm = np.zeros((len(lab), max(1, self.n_params)), np.float32) if params else None
-        lib.pasta_binary_eval_chi2(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_double),
-                                   ptr(st, C.c_int), None if prm is None else ptr(prm, C.c_float))
+        _batch_rc(lib.pasta_binary_eval_chi2(self._h, ptr(lab, C.c_double), len(lab),
+                                             ptr(out, C.c_double), ptr(st, C.c_int),
+                                             None if prm is None else ptr(prm, C.c_float)),
+                  "binary chi2")
         return out, st, prm
 
 
@@ -931,8 +948,9 @@ class _BinaryTS:
         lab = f64(labels9).reshape(-1, 9)
         out = np.zeros(len(lab))
         st = np.zeros(len(lab), np.int32)
-        lib.pasta_binary_ts_eval_chi2(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_double),
-                                      ptr(st, C.c_int))
+        _batch_rc(lib.pasta_binary_ts_eval_chi2(self._h, ptr(lab, C.c_double), len(lab),
+                                                ptr(out, C.c_double), ptr(st, C.c_int)),
+                  "time-series chi2")
         return out, st
 
 
@@ -969,8 +987,8 @@ class _BinaryRV:
         lab = f64(rvs).reshape(-1, 2)
         out = np.zeros(len(lab))
         st = np.zeros(len(lab), np.int32)
-        lib.pasta_binary_rv_eval_chi2(self._h, ptr(lab, C.c_double), len(lab), ptr(out, C.c_double),
-                                      ptr(st, C.c_int))
+        _batch_rc(lib.pasta_binary_rv_eval_chi2(self._h, ptr(lab, C.c_double), len(lab),
+                                                ptr(out, C.c_double), ptr(st, C.c_int)), "RV chi2")
         return out, st
 
 
@@ -1170,8 +1188,9 @@ class FitServer:
         _raise(lib.pasta_server_cancel(self._h, int(request_id)), "cancel")
 
     def stats(self):
-        """(evaluations, requests finished, iterations, HBM bytes in use)."""
-        out = (C.c_int64 * 4)()
+        """(evaluations, requests finished, iterations, HBM bytes in use, device-busy
+        microseconds, batch rebuilds)."""
+        out = (C.c_int64 * 6)()
         _raise(lib.pasta_server_stats(self._h, out), "stats")
         return tuple(out)
 

@@ -532,11 +532,10 @@ int pasta_cont_create_constant(int n, pasta_cont** out) {
     auto c = std::make_unique<pasta_cont>();
     c->kind = CONT_CONSTANT;
     c->n = n;
-    {
-        int slot = 0;
-        push_mblocks(*c, 0, 0, n, -1, slot);
-        c->mb_off = {0, slot};
-    }
+    // one moment block over all pixels: the mean is one sequential fold (k_const_fold)
+    c->mblocks.push_back(make_int4(0, 0, n, -1));
+    c->mslot.push_back(make_int2(0, -1));
+    c->mb_off = {0, 1};
     push_blocks(c->cblocks, 0, 0, 0, n);
     c->chi2_div = (double)n;
     *out = c.release();
@@ -616,6 +615,7 @@ static int dev_copy(pasta_star* s, const T* h, size_t n, const T** out) {
     void* p = nullptr;
     HIPCHK(hipMalloc(&p, n * sizeof(T)));
     s->allocs.push_back(p);
+    s->resident_bytes += (int64_t)(n * sizeof(T));
     HIPCHK(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
     *out = (const T*)p;
     return PASTA_OK;
@@ -767,7 +767,7 @@ int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
             pf += std::max(1, D.n_params);
             if (D.cont_kind == CONT_CHUNK)
                 for (int c = 0; c < D.n_chunks; c++) solve.push_back(make_int2(e, c));
-            else if (D.cont_kind == CONT_LINEAR || D.cont_kind == CONT_CONSTANT)
+            else if (D.cont_kind == CONT_LINEAR)
                 solve.push_back(make_int2(e, 0));
         }
     }
@@ -1505,39 +1505,54 @@ int pasta_compute_uncertainty(pasta_star* s, double spec_res, const double label
 
 struct Request {
     uint64_t id = 0;
-    pasta_star* star = nullptr;
+    pasta_star* star = nullptr;  // owned; freed when the request finishes or is cancelled
     pasta_fit_settings st{};
     uint64_t seed = 0;
     int state = 0;  // 0 waiting, 1 running, 2 finished, 3 cancelled
-    int64_t bytes = 0;
+    int64_t bytes = 0;  // HBM while in flight: evaluation buffers, PSO state, the star itself
     std::chrono::steady_clock::time_point t_submit;
     pasta_fit_result res{};
-    // PSO state carried across batch rebuilds (host copies)
-    bool started = false;
-    DevPso pso{};
-    std::vector<double> pos, vel, bpos, bcost;
+    bool started = false;  // its PSO state lives in the current batch, star slot `slot`
+    int slot = -1;
 };
 
+// Two batches alternate: a rebuild fills the idle one and moves the PSO state of the stars
+// that stay in flight from the current one on the device (k_pso_remap), so admission and
+// retirement never copy running stars' state through the host.
 struct pasta_server {
     pasta_grid* g = nullptr;
     pasta_server_config cfg{};
     int64_t budget = 0, in_use = 0;
     uint64_t next_id = 1;
     std::deque<Request*> waiting;
-    std::vector<Request*> running;
+    std::vector<Request*> running;     // in flight (admitted, not finished or cancelled)
+    std::vector<Request*> batch_reqs;  // the requests the current batch holds, by star slot
     std::map<uint64_t, std::unique_ptr<Request>> all;
-    std::unique_ptr<PsoBatch> pb;  // kept across rebuilds: its buffers keep their capacity
-    Batch fin;                     // evaluation of finished stars' best labels (reused)
-    bool active = false;           // pb holds a captured iteration of the running set
-    bool dirty = false;
-    int64_t evals_done = 0, finished = 0, iters_run = 0;
+    std::unique_ptr<PsoBatch> pb[2];   // kept across rebuilds: buffers keep their capacity
+    int cur = 0;                       // pb[cur] is the current batch
+    DBuf<int> d_src_eval, d_src_star;  // remap tables of the last rebuild
+    Batch fin;                         // evaluation of finished stars' best labels (reused)
+    bool active = false;               // pb[cur] holds a captured iteration of batch_reqs
+    bool dirty = false;                // running differs from batch_reqs
+    int64_t evals_done = 0, finished = 0, iters_run = 0, rebuilds = 0;
+    double busy_ms = 0;                // device time of first evaluations and iterations
+    hipEvent_t ev_busy[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool first_eval_timed = false;     // ev_busy[2..3] bracket a first evaluation in flight
     double t_model_last = 0, t_model_sum = 0, t_iter_last = 0, t_iter_sum = 0, t_n = 0;
     std::mutex mu;
     ~pasta_server() {
-        pb.reset();
+        pb[0].reset();
+        pb[1].reset();
+        for (auto& e : ev_busy)
+            if (e) (void)hipEventDestroy(e);
         for (auto& kv : all) delete kv.second->star;
     }
 };
+
+static void request_release_star(Request* r) {
+    delete r->star;
+    r->star = nullptr;
+}
 
 int pasta_server_create(pasta_grid* g, const pasta_server_config* cfg, pasta_server** out) {
     if (!g || !cfg || !out || cfg->max_stars < 1 || cfg->max_particles < 1 || cfg->iters_per_step < 1)
@@ -1552,6 +1567,7 @@ int pasta_server_create(pasta_grid* g, const pasta_server_config* cfg, pasta_ser
         HIPCHK(hipMemGetInfo(&f, &t));
         sv->budget = std::max<int64_t>((int64_t)f - (4LL << 30), 1LL << 28);
     }
+    for (auto& e : sv->ev_busy) HIPCHK(hipEventCreate(&e));
     *out = sv.release();
     return PASTA_OK;
 }
@@ -1572,11 +1588,12 @@ int pasta_server_submit(pasta_server* sv, pasta_star* s, const pasta_fit_setting
     std::lock_guard<std::mutex> lk(sv->mu);
     auto r = std::make_unique<Request>();
     r->id = sv->next_id++;
-    r->star = s;
     r->st = *st;
     r->seed = seed;
-    r->bytes = star_eval_bytes(s) * st->num_particles + 4 * 8 * 5 * st->num_particles;
+    r->bytes = star_eval_bytes(s) * st->num_particles + (int64_t)sizeof(double) * 16 * st->num_particles +
+               (int64_t)sizeof(DevPso) + s->resident_bytes;
     if (r->bytes > sv->budget) return fail(PASTA_ERR_NOMEM, "request larger than the HBM budget");
+    r->star = s;  // owned from here on
     r->t_submit = std::chrono::steady_clock::now();
     *id = r->id;
     sv->waiting.push_back(r.get());
@@ -1584,37 +1601,17 @@ int pasta_server_submit(pasta_server* sv, pasta_star* s, const pasta_fit_setting
     return PASTA_OK;
 }
 
-// Download the PSO state of every running, started star from the current batch.
-static int server_save_state(pasta_server* sv) {
-    if (!sv->active) return PASTA_OK;
-    Batch& B = sv->pb->B;
-    std::vector<DevPso> ps(B.n_stars);
-    HIPCHK(hipMemcpy(ps.data(), B.d_pso.p, sizeof(DevPso) * B.n_stars, hipMemcpyDeviceToHost));
-    std::vector<double> pos((size_t)B.n_evals * 5), vel(pos.size()), bpos(pos.size()), bcost(B.n_evals);
-    HIPCHK(hipMemcpy(pos.data(), B.d_labels.p, sizeof(double) * pos.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(vel.data(), B.d_vel.p, sizeof(double) * vel.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(bpos.data(), B.d_bpos.p, sizeof(double) * bpos.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(bcost.data(), B.d_bcost.p, sizeof(double) * bcost.size(), hipMemcpyDeviceToHost));
-    for (int s = 0; s < B.n_stars; s++) {
-        Request* r = sv->running[s];
-        int f0 = B.first_eval[s], P = B.count[s];
-        r->pso = ps[s];
-        r->started = true;
-        r->pos.assign(pos.begin() + 5 * f0, pos.begin() + 5 * (f0 + P));
-        r->vel.assign(vel.begin() + 5 * f0, vel.begin() + 5 * (f0 + P));
-        r->bpos.assign(bpos.begin() + 5 * f0, bpos.begin() + 5 * (f0 + P));
-        r->bcost.assign(bcost.begin() + f0, bcost.begin() + f0 + P);
-    }
-    return PASTA_OK;
-}
-
-// Build a batch for the running set; started stars resume from their saved state,
-// new stars get their initial population evaluated and sorted.
+// Build the idle batch for the running set.  Stars already in the current batch resume
+// there from their device-resident PSO state (moved by k_pso_remap); new stars get their
+// initial population evaluated and sorted.
 static int server_rebuild(pasta_server* sv) {
+    PsoBatch* old = sv->active ? sv->pb[sv->cur].get() : nullptr;
+    const int nxt = old ? 1 - sv->cur : sv->cur;
     sv->active = false;
+    sv->batch_reqs.clear();
     if (sv->running.empty()) return PASTA_OK;
-    if (!sv->pb) sv->pb = std::make_unique<PsoBatch>();
-    PsoBatch& PB = *sv->pb;
+    if (!sv->pb[nxt]) sv->pb[nxt] = std::make_unique<PsoBatch>();
+    PsoBatch& PB = *sv->pb[nxt];
     Batch& B = PB.B;
     if (B.graph) {
         HIPCHK(hipGraphExecDestroy(B.graph));
@@ -1630,55 +1627,68 @@ static int server_rebuild(pasta_server* sv) {
     }
     int max_it = 0;
     CHK(pso_batch_setup(PB, sv->g, stars, sts, seeds, false, &max_it));
+    std::vector<int> src_eval(B.n_evals, -1), src_star(B.n_stars, -1);
     bool any_new = false, any_started = false;
-    std::vector<double> bpos(PB.h_pos.size(), 0.0), bcost(B.n_evals, 0.0);
     for (int s = 0; s < B.n_stars; s++) {
         Request* r = sv->running[s];
         const int f0 = B.first_eval[s], P = B.count[s];
         if (r->started) {
-            DevPso p = r->pso;
-            p.first_eval = f0;
-            PB.h_pso[s] = p;
-            std::copy(r->pos.begin(), r->pos.end(), PB.h_pos.begin() + 5 * (size_t)f0);
-            std::copy(r->vel.begin(), r->vel.end(), PB.h_vel.begin() + 5 * (size_t)f0);
-            std::copy(r->bpos.begin(), r->bpos.end(), bpos.begin() + 5 * (size_t)f0);
-            std::copy(r->bcost.begin(), r->bcost.end(), bcost.begin() + f0);
-            for (int q = 0; q < P; q++) B.h_evals[f0 + q].active = 0;  // not part of the init pass
+            if (!old || r->slot < 0 || r->slot >= old->B.n_stars)
+                return fail(PASTA_ERR_STATE, "server lost a running star's state");
+            src_star[s] = r->slot;
+            for (int q = 0; q < P; q++) {
+                src_eval[f0 + q] = old->B.first_eval[r->slot] + q;
+                B.h_evals[f0 + q].active = 0;  // not part of the first-evaluation pass
+            }
             any_started = true;
         } else {
             any_new = true;
         }
     }
-    if (any_started) {  // one upload per array for every resumed star
-        CHK(B.d_labels.upload(PB.h_pos));
-        CHK(B.d_vel.upload(PB.h_vel));
-        CHK(B.d_bpos.upload(bpos));
-        CHK(B.d_bcost.upload(bcost));
+    if (any_started) {
+        // the old batch's work has finished: every step ends with a stream synchronisation
+        CHK(sv->d_src_eval.upload(src_eval));
+        CHK(sv->d_src_star.upload(src_star));
+        const Batch& O = old->B;
+        HIPCHK(launch_pso_remap(sv->d_src_eval.p, B.n_evals, sv->d_src_star.p, B.n_stars, O.d_pso.p,
+                                B.d_pso.p, O.d_labels.p, B.d_labels.p, O.d_vel.p, B.d_vel.p,
+                                O.d_bpos.p, B.d_bpos.p, O.d_bcost.p, B.d_bcost.p, B.stream));
+        CHK(B.d_evals.upload(B.h_evals));
     }
-    CHK(B.d_pso.upload(PB.h_pso));
-    CHK(B.d_evals.upload(B.h_evals));
-    if (any_new) CHK(pso_first_eval(PB));
+    if (any_new) {
+        HIPCHK(hipEventRecord(sv->ev_busy[2], B.stream));
+        CHK(pso_first_eval(PB));
+        HIPCHK(hipEventRecord(sv->ev_busy[3], B.stream));
+        sv->first_eval_timed = true;
+    }
     for (auto& e : B.ev)
         if (!e) HIPCHK(hipEventCreate(&e));
     CHK(pso_capture(PB));
+    for (int s = 0; s < B.n_stars; s++) {
+        sv->running[s]->slot = s;
+        sv->running[s]->started = true;
+    }
+    sv->batch_reqs = sv->running;
+    sv->cur = nxt;
     sv->active = true;
+    sv->rebuilds++;
     return PASTA_OK;
 }
 
+// Finished stars (all iterations done, or failed) leave the batch: their continuum
+// parameters come from one more evaluation at the best labels, then their stars are freed.
 static int server_retire(pasta_server* sv) {
-    Batch& B = sv->pb->B;
+    Batch& B = sv->pb[sv->cur]->B;
     std::vector<DevPso> ps(B.n_stars);
     HIPCHK(hipMemcpy(ps.data(), B.d_pso.p, sizeof(DevPso) * B.n_stars, hipMemcpyDeviceToHost));
     std::vector<int> done_idx;
     for (int s = 0; s < B.n_stars; s++)
         if (ps[s].done || ps[s].status) done_idx.push_back(s);
     if (done_idx.empty()) return PASTA_OK;
-    CHK(server_save_state(sv));
-    // continuum parameters at the best labels of the finished stars
     std::vector<pasta_star*> fs;
     std::vector<double> lab;
     for (int s : done_idx) {
-        fs.push_back(sv->running[s]->star);
+        fs.push_back(sv->batch_reqs[s]->star);
         for (int d = 0; d < 5; d++) lab.push_back(ps[s].gbest[d]);
     }
     Batch& F = sv->fin;
@@ -1689,30 +1699,30 @@ static int server_retire(pasta_server* sv) {
     std::vector<float> pf((size_t)std::max<int64_t>(F.pf_len, 1));
     HIPCHK(hipMemcpy(pf.data(), F.d_pf.p, sizeof(float) * pf.size(), hipMemcpyDeviceToHost));
     auto now = std::chrono::steady_clock::now();
-    std::vector<Request*> keep;
-    for (int s = 0, k = 0; s < B.n_stars; s++) {
-        Request* r = sv->running[s];
-        if (k < (int)done_idx.size() && done_idx[k] == s) {
-            pasta_fit_result& res = r->res;
-            std::memset(&res, 0, sizeof(res));
-            res.status = ps[s].status;
-            res.iterations = r->st.max_iters;
-            res.chi2 = ps[s].gcost;
-            std::memcpy(res.label, ps[s].gbest, 40);
-            res.time_s = std::chrono::duration<double>(now - r->t_submit).count();
-            int np = std::min(512, std::max(1, r->star->n_params));
-            res.n_params = np;
-            for (int q = 0; q < np; q++)
-                res.continuum_params[q] =
-                    r->star->cont_kind == CONT_FIXED ? 0.0f : pf[F.h_evals[k].pf_off + q];
-            r->state = 2;
-            sv->in_use -= r->bytes;
-            sv->finished++;
-            k++;
-        } else {
-            keep.push_back(r);
-        }
+    for (int k = 0; k < (int)done_idx.size(); k++) {
+        const int s = done_idx[k];
+        Request* r = sv->batch_reqs[s];
+        pasta_fit_result& res = r->res;
+        std::memset(&res, 0, sizeof(res));
+        res.status = ps[s].status;
+        res.iterations = r->st.max_iters;
+        res.chi2 = ps[s].gcost;
+        std::memcpy(res.label, ps[s].gbest, 40);
+        res.time_s = std::chrono::duration<double>(now - r->t_submit).count();
+        const int np = std::min(512, std::max(1, r->star->n_params));
+        res.n_params = np;
+        for (int q = 0; q < np; q++)
+            res.continuum_params[q] =
+                r->star->cont_kind == CONT_FIXED ? 0.0f : pf[F.h_evals[k].pf_off + q];
+        r->state = 2;
+        sv->in_use -= r->bytes;
+        sv->finished++;
+        // neither the current batch nor `fin` runs again before the next rebuild
+        request_release_star(r);
     }
+    std::vector<Request*> keep;
+    for (Request* r : sv->running)
+        if (r->state == 1) keep.push_back(r);
     sv->running.swap(keep);
     sv->dirty = true;
     return PASTA_OK;
@@ -1727,7 +1737,6 @@ int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
         Request* r = sv->waiting.front();
         if (r->state == 3) { sv->waiting.pop_front(); continue; }
         if (sv->in_use + r->bytes > sv->budget && !sv->running.empty()) break;
-        if (!sv->dirty) CHK(server_save_state(sv));
         sv->waiting.pop_front();
         r->state = 1;
         sv->in_use += r->bytes;
@@ -1739,13 +1748,14 @@ int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
         sv->dirty = false;
     }
     if (sv->active) {
-        Batch& B = sv->pb->B;
+        Batch& B = sv->pb[sv->cur]->B;
+        HIPCHK(hipEventRecord(sv->ev_busy[0], B.stream));
         for (int it = 0; it < sv->cfg.iters_per_step; it++) {
             if (!sv->cfg.timing) {
                 HIPCHK(hipGraphLaunch(B.graph, B.stream));
             } else {  // the same launches without the graph, bracketed by HIP events
                 B.eager_events = true;
-                int rc = pso_iteration_launches(*sv->pb);
+                int rc = pso_iteration_launches(*sv->pb[sv->cur]);
                 B.eager_events = false;
                 CHK(rc);
                 HIPCHK(hipStreamSynchronize(B.stream));
@@ -1759,13 +1769,24 @@ int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
                 sv->t_n += 1;
             }
         }
+        HIPCHK(hipEventRecord(sv->ev_busy[1], B.stream));
         HIPCHK(hipStreamSynchronize(B.stream));
+        float busy = 0.0f;
+        HIPCHK(hipEventElapsedTime(&busy, sv->ev_busy[0], sv->ev_busy[1]));
+        if (sv->first_eval_timed) {
+            float fe = 0.0f;
+            HIPCHK(hipEventElapsedTime(&fe, sv->ev_busy[2], sv->ev_busy[3]));
+            busy += fe;
+            sv->first_eval_timed = false;
+        }
+        sv->busy_ms += busy;
         sv->iters_run += sv->cfg.iters_per_step;
         sv->evals_done += (int64_t)B.n_evals * sv->cfg.iters_per_step;
         CHK(server_retire(sv));
         if (sv->dirty && sv->running.empty()) {
             sv->active = false;
             sv->dirty = false;
+            sv->batch_reqs.clear();
         }
     }
     if (active) *active = (int)sv->running.size();
@@ -1791,24 +1812,33 @@ int pasta_server_cancel(pasta_server* sv, uint64_t id) {
     auto it = sv->all.find(id);
     if (it == sv->all.end()) return fail(PASTA_ERR_STATE, "unknown request");
     Request* r = it->second.get();
-    if (r->state == 0) { r->state = 3; return PASTA_OK; }
+    if (r->state == 0) {  // still queued: admission skips it
+        r->state = 3;
+        request_release_star(r);
+        return PASTA_OK;
+    }
     if (r->state == 1) {
-        CHK(server_save_state(sv));
+        // the star leaves the running set; the current batch, which still holds it, is
+        // rebuilt before the next launch, so its buffers can go now
         sv->running.erase(std::find(sv->running.begin(), sv->running.end(), r));
         sv->in_use -= r->bytes;
         r->state = 3;
+        request_release_star(r);
         sv->dirty = true;
         return PASTA_OK;
     }
+    if (r->state == 3) return fail(PASTA_ERR_STATE, "request was cancelled");
     return fail(PASTA_ERR_STATE, "request already finished");
 }
 
-int pasta_server_stats(const pasta_server* sv, int64_t out[4]) {
-    if (!sv) return fail(PASTA_ERR_ARG, "null server");
+int pasta_server_stats(const pasta_server* sv, int64_t out[6]) {
+    if (!sv || !out) return fail(PASTA_ERR_ARG, "null argument");
     out[0] = sv->evals_done;
     out[1] = sv->finished;
     out[2] = sv->iters_run;
     out[3] = sv->in_use;
+    out[4] = (int64_t)(sv->busy_ms * 1e3);
+    out[5] = sv->rebuilds;
     return PASTA_OK;
 }
 

@@ -51,9 +51,10 @@ struct KVariant {
 };
 __host__ __device__ constexpr KVariant mom_variant(int kind, int p1) {
     // chunk and linear: the matrix-core kernel, 64 evaluations per workgroup; constant: the
-    // vector kernel, 8 evaluations per workgroup
+    // sequential fold of the quotients, 16 evaluations per workgroup
     return (kind == CONT_LINEAR || kind == CONT_CHUNK) ? KVariant{p1 <= 16 ? 16 : 32, 64, 0}
-                                                       : KVariant{1, 8, 0};
+           : kind == CONT_CONSTANT                      ? KVariant{1, 16, 0}
+                                                        : KVariant{1, 8, 0};
 }
 __host__ __device__ constexpr KVariant chi2_variant(int kind, int p1) {
     return (kind == CONT_CONSTANT || kind == CONT_FIXED) ? KVariant{1, 8, 0}

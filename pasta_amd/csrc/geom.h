@@ -106,22 +106,31 @@ PHD int teff_index_limits_at(const GridGeom& g, double logg, int* lo, int* hi) {
     return ST_OK;
 }
 
+// out[q] = -1 where the reference has None.  Where the limits span fewer than three
+// points the rules can name an index outside the axis (i + 2 past the end, i - 2 below
+// 0); the reference then panics on the value lookup (interpolate.rs:483-530), here that
+// is ST_NEIGHBORS.
 PHD int find_neighbors(const double* v, int n, double x, int left, int right, int out[4]) {
     int r = right_index(v, n, x);
+    bool none0 = false, none3 = false;
     if (r >= 0) {
         int i = r;
         if (i < left || i > right) return ST_BOUNDS;
-        if (i == left) { out[0] = -1; out[1] = i; out[2] = i + 1; out[3] = i + 2; }
-        else if (i == right) { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = -1; }
+        if (i == left) { none0 = true; out[1] = i; out[2] = i + 1; out[3] = i + 2; }
+        else if (i == right) { out[0] = i - 2; out[1] = i - 1; out[2] = i; none3 = true; }
         else if (i == right - 1) { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
         else { out[0] = i - 1; out[1] = i; out[2] = i + 1; out[3] = i + 2; }
-        return ST_OK;
+    } else {
+        int i = -r - 1;
+        if (i <= left || i > right) return ST_BOUNDS;
+        if (i == left + 1) { none0 = true; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
+        else if (i == right) { out[0] = i - 2; out[1] = i - 1; out[2] = i; none3 = true; }
+        else { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
     }
-    int i = -r - 1;
-    if (i <= left || i > right) return ST_BOUNDS;
-    if (i == left + 1) { out[0] = -1; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
-    else if (i == right) { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = -1; }
-    else { out[0] = i - 2; out[1] = i - 1; out[2] = i; out[3] = i + 1; }
+    for (int q = none0 ? 1 : 0; q < (none3 ? 3 : 4); q++)
+        if (out[q] < 0 || out[q] >= n) return ST_NEIGHBORS;
+    if (none0) out[0] = -1;
+    if (none3) out[3] = -1;
     return ST_OK;
 }
 

@@ -28,6 +28,11 @@ hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, 
 hipError_t launch_pso_best(DevPso* pso, int n_stars, const DevEval* evals, const double* pos,
                            double* bpos, double* bcost, const double* cost, double* trace,
                            hipStream_t st);
+hipError_t launch_pso_remap(const int* src_eval, int n_evals, const int* src_star, int n_stars,
+                            const DevPso* pso_old, DevPso* pso_new, const double* pos_old,
+                            double* pos_new, const double* vel_old, double* vel_new,
+                            const double* bpos_old, double* bpos_new, const double* bcost_old,
+                            double* bcost_new, hipStream_t st);
 hipError_t launch_pso_init(DevPso* pso, int n_stars, const DevEval* evals, double* pos,
                            double* vel, double* bpos, double* bcost, double* cost, double* trace,
                            hipStream_t st);
@@ -149,6 +154,7 @@ struct pasta_star {
     int n_params = 1, p1 = 1, cont_kind = CONT_CONSTANT, n_chunks = 0;
     DevStar dev{};
     std::vector<void*> allocs;
+    int64_t resident_bytes = 0;  // device memory the star's own buffers hold
     std::unique_ptr<Batch> cache;
     int cache_P = -1;
     std::mutex mu;

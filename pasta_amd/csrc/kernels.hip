@@ -1055,48 +1055,73 @@ __device__ __forceinline__ float dot_na(const float (&a)[PM], const float* b, in
     return r;
 }
 
-// Constant-continuum moment (the sum of f/m) of E evaluations of one star over one block
-// of observed pixels: block descriptor (first evaluation, count, moment block, 0).
-template <int PM, int E, int P1F>
-__global__ __launch_bounds__(NT) void k_moments(const DevStar* __restrict__ stars,
-                                                const DevEval* __restrict__ evals,
-                                                const int4* __restrict__ blocks,
-                                                const float* __restrict__ model,
-                                                double* __restrict__ mpart) {
-    __shared__ double red[4 * E * PM];
+// The constant continuum's parameter as the reference forms it
+// (continuum_fitting.rs:490-492): the f32 quotients f/m, added one after another in pixel
+// order (`target.iter().sum::<f32>()`, a sequential fold), divided by n in f32.  A
+// workgroup covers the E <= 16 evaluations of one star in block descriptor (first
+// evaluation, count, 0, 0).  Waves 1-3 form the quotients of 64-pixel chunks (IEEE
+// divisions, coalesced reads of the flux and of each model row) into a double-buffered
+// LDS tile, one row per evaluation; lane e < E of wave 0 adds row e's 64 values in order
+// while the next chunk is being formed.  Past the last pixel the tile holds zeros: adding
+// +0 leaves a sum unchanged (but for the sign of a zero sum).
+constexpr int FOLD_CH = 64;       // pixels per chunk
+constexpr int FOLD_ROW = 68;      // LDS row stride: 16-byte rows, conflict-free for 16 lanes
+template <int E>
+__global__ __launch_bounds__(NT) void k_const_fold(const DevStar* __restrict__ stars,
+                                                   const DevEval* __restrict__ evals,
+                                                   const int4* __restrict__ blocks,
+                                                   const float* __restrict__ model,
+                                                   float* __restrict__ pf) {
+    static_assert(E <= 16, "one lane of wave 0 per evaluation, <= 16 for conflict-free rows");
+    __shared__ __align__(16) float q[2][E][FOLD_ROW];
     const int4 bd = blocks[blockIdx.x];
     const DevStar& S = stars[evals[bd.x].star];
-    const int4 mb = S.mblocks[bd.z];
-    const float* m[E];
-    bool ok[E];
-    bool any = false;
+    const int N = S.N, cnt = bd.y;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n_ch = (N + FOLD_CH - 1) / FOLD_CH;
+    // per producer thread: its evaluations' model rows (null when failed or absent)
+    const float* rows[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        ok[e] = e < bd.y && evals[bd.x + e].status == ST_OK;
-        m[e] = model + (ok[e] ? evals[bd.x + e].m_off : 0);
-        any |= ok[e];
+        const bool ok = e < cnt && evals[bd.x + e].status == ST_OK;
+        rows[e] = ok ? model + evals[bd.x + e].m_off : nullptr;
     }
-    if (!any) return;
-    double acc[E * PM];
+    auto produce = [&](int c, int buf) {
+        const int t = threadIdx.x - 64;  // 0 .. 191
+        for (int k = t; k < E * FOLD_CH; k += NT - 64) {
+            const int e = k / FOLD_CH, i = k - e * FOLD_CH, px = c * FOLD_CH + i;
+            const float* m = nullptr;
 #pragma unroll
-    for (int k = 0; k < E * PM; k++) acc[k] = 0.0;
-    int np = 1;
-    {  // constant: sum of f/m
-        for (int i = mb.y + threadIdx.x; i < mb.z; i += NT) {
-            const float f = S.flux[i];
-#pragma unroll
-            for (int e = 0; e < E; e++)
-                if (ok[e]) acc[e * PM] += (double)(f * __builtin_amdgcn_rcpf(m[e][i]));
+            for (int j = 0; j < E; j++)
+                if (j == e) m = rows[j];
+            q[buf][e][i] = (m && px < N) ? S.flux[px] / m[px] : 0.0f;
         }
+    };
+    if (wv > 0) produce(0, 0);
+    __syncthreads();
+    float s = -0.0f;
+    for (int c = 0; c < n_ch; c++) {
+        const int buf = c & 1;
+        if (wv == 0) {
+            if (lane < cnt) {
+                const float4* r = (const float4*)&q[buf][lane][0];
+#pragma unroll
+                for (int i = 0; i < FOLD_CH / 4; i++) {
+                    const float4 v = r[i];
+                    s = s + v.x;
+                    s = s + v.y;
+                    s = s + v.z;
+                    s = s + v.w;
+                }
+            }
+        } else if (c + 1 < n_ch) {
+            produce(c + 1, buf ^ 1);
+        }
+        __syncthreads();
     }
-    const double tot = group_sums<E * PM>(acc, red);
-    const int t = threadIdx.x;
-    if (t < E * PM) {
-        const int e = t / PM, k = t - e * PM;
-        if (e < bd.y && k < np) {
-            const DevEval& ev = evals[bd.x + e];
-            if (ev.status == ST_OK) mpart[ev.mp_off + (size_t)bd.z * np + k] = tot;
-        }
+    if (wv == 0 && lane < cnt) {
+        const DevEval& ev = evals[bd.x + lane];
+        if (ev.status == ST_OK) pf[ev.pf_off] = s / (float)N;  // sum / n as FluxFloat
     }
 }
 
@@ -1248,12 +1273,6 @@ __global__ void k_solve(const DevStar* __restrict__ stars, const DevEval* __rest
     DevEval& ev = const_cast<DevEval&>(evals[it.x]);
     if (ev.status != ST_OK) return;
     const DevStar& S = stars[ev.star];
-    if (S.cont_kind == CONT_CONSTANT) {
-        double s = 0.0;
-        for (int mb = 0; mb < S.n_mblocks; mb++) s += mpart[ev.mp_off + mb];
-        pf[ev.pf_off] = (float)s / (float)S.N;  // sum / n as FluxFloat (continuum_fitting.rs:492)
-        return;
-    }
     const int c = it.y, p1 = S.p1;
     int mb0, mb1;
     if (S.cont_kind == CONT_CHUNK) { mb0 = S.mb_off[c]; mb1 = S.mb_off[c + 1]; }
@@ -1535,6 +1554,40 @@ __global__ void k_pso_init(DevPso* __restrict__ pso, int n_stars, const DevEval*
     if (ps.max_iters < 1) ps.done = 1;
 }
 
+// Continuous batching: the PSO state of stars that stay in flight moves from the previous
+// batch into the rebuilt one on the device.  Thread per evaluation of the new batch:
+// src_eval[e] is its particle's evaluation in the old batch (-1: a new star, left as
+// initialised); thread s < n_stars also moves the star record (src_star[s] >= 0),
+// keeping the new batch's first_eval.
+__global__ void k_pso_remap(const int* __restrict__ src_eval, int n_evals,
+                            const int* __restrict__ src_star, int n_stars,
+                            const DevPso* __restrict__ pso_old, DevPso* __restrict__ pso_new,
+                            const double* __restrict__ pos_old, double* __restrict__ pos_new,
+                            const double* __restrict__ vel_old, double* __restrict__ vel_new,
+                            const double* __restrict__ bpos_old, double* __restrict__ bpos_new,
+                            const double* __restrict__ bcost_old, double* __restrict__ bcost_new) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_evals) {
+        const int o = src_eval[t];
+        if (o >= 0) {
+            for (int d = 0; d < 5; d++) {
+                pos_new[5 * (size_t)t + d] = pos_old[5 * (size_t)o + d];
+                vel_new[5 * (size_t)t + d] = vel_old[5 * (size_t)o + d];
+                bpos_new[5 * (size_t)t + d] = bpos_old[5 * (size_t)o + d];
+            }
+            bcost_new[t] = bcost_old[o];
+        }
+    }
+    if (t < n_stars) {
+        const int o = src_star[t];
+        if (o >= 0) {
+            const int f0 = pso_new[t].first_eval;
+            pso_new[t] = pso_old[o];
+            pso_new[t].first_eval = f0;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // counting sort of evaluations by key (order only: results do not depend on it)
 // ---------------------------------------------------------------------------
@@ -1720,13 +1773,9 @@ hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const 
                                    evals, blk, model, mpart);
             continue;
         }
-        const KVariant v = mom_variant(K.kind, K.p1);
-#define PASTA_MOM(PM, E, F)                                                                  \
-    if (v.pm == PM && v.e == E && v.f == F)                                                   \
-        hipLaunchKernelGGL((k_moments<PM, E, F>), dim3(K.n_mom), dim3(NT), 0, st, stars, evals, \
-                           blk, model, mpart);
-        PASTA_MOM(1, 8, 0)
-#undef PASTA_MOM
+        if (K.kind == CONT_CONSTANT)  // the parameter itself: no solve
+            hipLaunchKernelGGL((k_const_fold<16>), dim3(K.n_mom), dim3(NT), 0, st, stars, evals,
+                               blk, model, pf);
     }
     if (n_solve > 0)
         hipLaunchKernelGGL(k_solve, dim3((n_solve + 127) / 128), dim3(128), 0, st, stars, evals,
@@ -2036,6 +2085,19 @@ hipError_t launch_pso_best(DevPso* pso, int n_stars, const DevEval* evals, const
                            hipStream_t st) {
     hipLaunchKernelGGL(k_pso_best, dim3((n_stars + 63) / 64), dim3(64), 0, st, pso, n_stars, evals,
                        pos, bpos, bcost, cost, trace);
+    return hipGetLastError();
+}
+
+hipError_t launch_pso_remap(const int* src_eval, int n_evals, const int* src_star, int n_stars,
+                            const DevPso* pso_old, DevPso* pso_new, const double* pos_old,
+                            double* pos_new, const double* vel_old, double* vel_new,
+                            const double* bpos_old, double* bpos_new, const double* bcost_old,
+                            double* bcost_new, hipStream_t st) {
+    const int n = n_evals > n_stars ? n_evals : n_stars;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pso_remap, dim3((n + 255) / 256), dim3(256), 0, st, src_eval, n_evals,
+                       src_star, n_stars, pso_old, pso_new, pos_old, pos_new, vel_old, vel_new,
+                       bpos_old, bpos_new, bcost_old, bcost_new);
     return hipGetLastError();
 }
 

@@ -64,7 +64,7 @@ struct Layout {
     int64_t pf_stride;  // continuum parameters (floats) per evaluation
     int gm, gc;         // evaluations per moment / chi2 block
     int n_mblocks, n_cblocks;
-    int solve_per_eval;  // chunk: n_chunks (e, c) items; linear / constant: 1; fixed: 0
+    int solve_per_eval;  // chunk: n_chunks (e, c) items; linear: 1; constant, fixed: 0
     int n_bmom, n_bchi2, n_solve;
 };
 
@@ -81,8 +81,7 @@ Layout make_layout(const DevStar& D, int P) {
     L.gc = chi2_variant(D.cont_kind, p1).e;
     L.n_mblocks = D.n_mblocks;
     L.n_cblocks = D.n_cblocks;
-    L.solve_per_eval = D.cont_kind == CONT_CHUNK ? D.n_chunks
-                       : (D.cont_kind == CONT_LINEAR || D.cont_kind == CONT_CONSTANT) ? 1 : 0;
+    L.solve_per_eval = D.cont_kind == CONT_CHUNK ? D.n_chunks : D.cont_kind == CONT_LINEAR ? 1 : 0;
     L.n_bmom = ((P + L.gm - 1) / L.gm) * D.n_mblocks;
     L.n_bchi2 = ((P + L.gc - 1) / L.gc) * D.n_cblocks;
     L.n_solve = P * L.solve_per_eval;

@@ -53,3 +53,28 @@ def gpu_small(small):
     wlg = pasta.WlGrid(small.spec.log10_first, small.spec.log10_step, small.spec.n_pix, log=True)
     interp = pasta.InMemInterpolator.from_arrays(small.spectra, small.labels, wlg)
     return pasta, wlg, interp
+
+
+_full = None
+
+
+@pytest.fixture(scope="session")
+def full():
+    """The full-size grid (25 x 16 x 21 models of 200,000 pixels over 3800-9000 A, 3.36 GB,
+    filled in HBM with the seeded 5,000-line recipe) and the oracle on the same rows read
+    back from HBM."""
+    global _full
+    if _full is None:
+        import pasta_amd as pasta
+        from pasta_amd import synth
+        from oracle import oracle as O
+
+        spec = synth.full_grid_spec()
+        wlg = pasta.WlGrid(spec.log10_first, spec.log10_step, spec.n_pix, log=True)
+        interp = pasta.InMemInterpolator.empty(spec.labels(), wlg)
+        synth.fill_grid_gpu(interp, spec)
+        rows = synth.read_grid(interp)
+        og = O.Grid(spec.labels(), rows, None, spec.log10_first, spec.log10_step)
+        print(f"full grid {rows.shape} in HBM and on the host", flush=True)
+        _full = (pasta, synth, spec, wlg, interp, og)
+    return _full

@@ -124,3 +124,106 @@ def test_server_alone_vs_mix(small, gpu_small):
     # and the server agrees with the plain fit
     one = fitter.fit(interp, obs[2][0], obs[2][1], seed=102)
     assert one.label.as_list() == alone[2][0] and one.chi2 == alone[2][1]
+
+
+def _server_run(pasta, interp, d, c, obs, st, order, submit_at, cancel_at=None, iters_per_step=2,
+                max_stars=3, budget=0):
+    """Steps a server, submitting request k at step submit_at[k] and cancelling request k at
+    step cancel_at[k]; returns {k: (label, chi2, params)} of the finished requests and the
+    request ids."""
+    import ctypes as C
+    from pasta_amd import _lib
+    cfg = _lib.ServerConfig(max_stars, 64, iters_per_step, budget)
+    sv = C.c_void_p()
+    _lib.check(_lib.lib.pasta_server_create(interp._h, C.byref(cfg), C.byref(sv)))
+    ids, out, step, cancelled = {}, {}, 0, set()
+    cancel_at = cancel_at or {}
+    try:
+        while len(out) + len(cancelled) < len(order):
+            for k in order:
+                if k not in ids and submit_at[k] <= step:
+                    s = pasta.Star(interp, d, c, obs[k][0], obs[k][1])
+                    rid = C.c_uint64()
+                    _lib.check(_lib.lib.pasta_server_submit(sv, s.release(), C.byref(st), 100 + k,
+                                                            C.byref(rid)))
+                    ids[k] = rid.value
+                if k in ids and cancel_at.get(k) == step and k not in out and k not in cancelled:
+                    _lib.check(_lib.lib.pasta_server_cancel(sv, ids[k]))
+                    cancelled.add(k)
+            a, w = C.c_int(), C.c_int()
+            _lib.check(_lib.lib.pasta_server_step(sv, C.byref(a), C.byref(w)))
+            for k, rid in ids.items():
+                if k in cancelled or k in out:
+                    continue
+                done = C.c_int()
+                r = _lib.FitResult()
+                _lib.check(_lib.lib.pasta_server_poll(sv, rid, C.byref(done), C.byref(r)))
+                if done.value:
+                    out[k] = (list(r.label), r.chi2, list(r.continuum_params[:r.n_params]))
+            step += 1
+            assert step < 500
+        for k in cancelled:  # a cancelled request reports so
+            done = C.c_int()
+            assert _lib.lib.pasta_server_poll(sv, ids[k], C.byref(done), None) == -6
+    finally:
+        _lib.lib.pasta_server_destroy(sv)
+    return out
+
+
+def test_server_cancel_running_and_queued(small, gpu_small):
+    """A request cancelled while running and one cancelled while queued leave the other
+    requests' results exactly as when each runs alone (the batch is rebuilt around them)."""
+    pasta, wlg, interp = gpu_small
+    fitter, d, c = _fitter(small, pasta, wlg, interp, P=16, iters=20)
+    labs = [[5600.0, -0.5, 4.4, 35.0, -40.0], [6500.0, 0.2, 3.8, 120.0, 60.0],
+            [6123.0, -0.3, 4.21, 10.0, 20.0], [5200.0, 0.4, 4.9, 250.0, 100.0]]
+    obs = [_observed(small, l, 30 + k) for k, l in enumerate(labs)]
+    st = pasta.api._settings_c(pasta.PSOSettings(16, 20), (0.0, 300.0), (-150.0, 150.0), None)
+    print("server: alone", flush=True)
+    alone = {k: _server_run(pasta, interp, d, c, obs, st, [k], {k: 0})[k] for k in (0, 2)}
+    print("server: request 1 cancelled while running, 3 while queued", flush=True)
+    # 2 iterations per step: request 0 retires in step 9; request 1 (18 iterations done) is
+    # cancelled before step 10, right after that retirement, and request 3 while it waits
+    # for a slot; request 2 resumes in the rebuilt batch
+    mixed = _server_run(pasta, interp, d, c, obs, st, [0, 1, 2, 3], {0: 0, 1: 1, 2: 4, 3: 6},
+                        cancel_at={1: 10, 3: 9}, max_stars=3)
+    assert set(mixed) == {0, 2}
+    assert mixed[0] == alone[0] and mixed[2] == alone[2]
+
+
+def test_server_memory_flat_over_many_requests(small, gpu_small):
+    """Finished requests free their stars: free device memory after a second wave of short
+    requests is what it was after the first."""
+    from pasta_amd import _lib
+    import ctypes as C
+    pasta, wlg, interp = gpu_small
+    fitter, d, c = _fitter(small, pasta, wlg, interp, P=8, iters=3)
+    obs = [_observed(small, [6123.0, -0.3, 4.21, 10.0, 20.0], 50 + k) for k in range(3)]
+    srv = pasta.FitServer(interp, max_stars=4, max_particles=8, iters_per_step=3)
+
+    def wave(n):
+        ids = [srv.submit(pasta.Star(interp, d, c, *obs[k % 3]), pasta.PSOSettings(8, 3),
+                          (0.0, 300.0), (-150.0, 150.0), seed=k) for k in range(n)]
+        for _ in range(200):
+            a, w = srv.step()
+            if a == 0 and w == 0:
+                break
+        assert all(srv.poll(i) is not None for i in ids)
+
+    def free():
+        f, t = C.c_uint64(), C.c_uint64()
+        _lib.check(_lib.lib.pasta_device_mem_info(C.byref(f), C.byref(t)))
+        return f.value
+
+    try:
+        wave(12)
+        f0 = free()
+        wave(40)
+        f1 = free()
+        stats = srv.stats()
+        assert stats[1] == 52 and stats[3] == 0, stats
+        print(f"free device memory after 12 / 52 requests: {f0} / {f1} bytes", flush=True)
+        # 40 more stars of ~0.3 MB each would hold ~12 MB if they were not freed
+        assert f1 >= f0 - (4 << 20), (f0, f1)
+    finally:
+        srv.close()

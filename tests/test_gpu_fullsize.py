@@ -21,21 +21,6 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)))
 
 
-@pytest.fixture(scope="module")
-def full():
-    import pasta_amd as pasta
-    from pasta_amd import synth
-
-    spec = synth.full_grid_spec()
-    wlg = pasta.WlGrid(spec.log10_first, spec.log10_step, spec.n_pix, log=True)
-    interp = pasta.InMemInterpolator.empty(spec.labels(), wlg)
-    synth.fill_grid_gpu(interp, spec)
-    rows = synth.read_grid(interp)
-    og = O.Grid(spec.labels(), rows, None, spec.log10_first, spec.log10_step)
-    print(f"full grid {rows.shape} in HBM and on the host", flush=True)
-    return pasta, synth, spec, wlg, interp, og
-
-
 def _star(full, n, seed):
     pasta, synth, spec, wlg, interp, og = full
     wl = synth.observed_wavelengths(spec, n, 600)

@@ -205,17 +205,10 @@ def test_chi2_matches_oracle(small, gpu_small, kind):
     for k, lab in enumerate(labs):
         ref, rp = O.cost(small.og, small.odisp, ocont, small.flux, small.var, lab)
         if kind == "constant":
-            # The reference sums flux/model sequentially in f32 (continuum_fitting.rs:492);
-            # the GPU sums in f64.  Pin the GPU against the same arithmetic with the mean
-            # summed exactly, and bound its distance to the f32-sequential sum.
-            m = O.produce_model(small.og, small.odisp, lab)
-            q = (small.flux / m).astype(np.float32)
-            mean = np.float32(q.astype(np.float64).sum()) / np.float32(len(q))
-            r = (small.flux - m * mean).astype(np.float32)
-            exact = float(((r / small.var) * r).astype(np.float64).sum() / len(q))
-            assert abs(chi2[k] - exact) / exact < RTOL, (lab.tolist(), chi2[k], exact)
-            assert abs(chi2[k] - ref) / ref < 1e-4, (lab.tolist(), chi2[k], ref)
-            continue
+            # The mean of f/m is the reference's sequential f32 fold (continuum_fitting.rs:
+            # 490-492) on both sides; it differs from the exactly summed mean by ~1e-5 here,
+            # which moves chi2 by ~1e-4, so this pins the fold order itself.
+            assert abs(params[k][0] - rp[0]) <= 2e-6 * abs(rp[0]), (params[k][0], rp[0])
         assert abs(chi2[k] - ref) / ref < RTOL, (kind, lab.tolist(), chi2[k], ref)
         if kind in ("chunk", "chunk10x8", "linear", "linear30", "constant"):
             np.testing.assert_allclose(params[k][: len(rp)], rp, rtol=2e-4, atol=1e-3 * np.abs(rp).max())
