@@ -90,6 +90,7 @@ _sig = {
     "or_disp_new": (C.c_void_p, [C.c_int, _dp, C.c_int, C.POINTER(_WlGrid), C.c_double, _fp]),
     "or_disp_free": (None, [C.c_void_p]),
     "or_disp_kernel_len": (C.c_int, [C.c_void_p]),
+    "or_disp_kernels": (C.c_long, [C.c_void_p, _fp, C.c_long]),
     "or_cont_new_chunk": (C.c_void_p, [_dp, C.c_int, C.c_int, C.c_int, C.c_double]),
     "or_cont_new_linear": (C.c_void_p, [_fp, C.c_int, C.c_int]),
     "or_cont_new_fixed": (C.c_void_p, [_fp, C.c_int, C.c_int]),
@@ -284,6 +285,15 @@ class Dispersion:
     @property
     def K(self):
         return _L.or_disp_kernel_len(self._h)
+
+    def kernels(self, W):
+        """fixed: the K taps; variable: the W x K per-pixel kernels."""
+        K = self.K
+        out = np.zeros(max(1, K * W), np.float32)
+        n = _L.or_disp_kernels(self._h, _p(out, C.c_float), len(out))
+        if n < 0:
+            raise OracleError(n, "kernels")
+        return out[:K] if self.kind == DISP_FIXED else out[:n].reshape(W, K)
 
 
 class Continuum:

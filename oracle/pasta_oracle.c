@@ -685,6 +685,20 @@ void or_disp_free(or_disp *d) {
 
 int or_disp_kernel_len(const or_disp *d) { return d->K; }
 
+/* fixed: the K taps; variable: W rows of K taps (row i = synthetic pixel i's kernel) */
+long or_disp_kernels(const or_disp *d, float *out, long cap) {
+    if (d->kind == OR_DISP_FIXED) {
+        if (cap < d->K) return OR_ERR_ARG;
+        memcpy(out, d->kernel, sizeof(float) * d->K);
+        return d->K;
+    }
+    if (d->kind != OR_DISP_VARIABLE) return 0;
+    long n = (long)d->K * d->W;
+    if (cap < n) return OR_ERR_ARG;
+    memcpy(out, d->kernels, sizeof(float) * (size_t)n);
+    return n;
+}
+
 /* ======================================================================
  * produce_model -- interpolate.rs:1068-1090 composed of
  * convolve_rotation (convolve_rv.rs:484-529), convolve_segment, and

@@ -102,6 +102,9 @@ or_disp *or_disp_new(int kind, const double *wl, int n, const or_wlgrid *sg, dou
                      const float *disp /* variable: per observed pixel */);
 void or_disp_free(or_disp *d);
 int or_disp_kernel_len(const or_disp *d);
+/* fixed: the K taps; variable: W rows of K taps (row i = synthetic pixel i); returns the
+ * number of values written */
+long or_disp_kernels(const or_disp *d, float *out, long cap);
 
 typedef struct or_cont {
     int kind;

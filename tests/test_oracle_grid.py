@@ -222,3 +222,33 @@ def test_factors_scale_each_model():
     out = g.interpolate(5400.0, -0.6, 3.9, f64=True)
     exp = sum(w64[q] * fac[idx[q]] * spectra[idx[q]].astype(np.float64) for q in range(64) if idx[q] >= 0)
     np.testing.assert_allclose(out[:128], exp[:128], rtol=1e-12)
+
+
+def test_neighbour_index_outside_the_axis_is_an_error():
+    """Where an axis' limits span two points, the neighbour rules name an index outside the
+    axis (i + 2 past the end at the left limit, i - 2 below zero at the right, or at the
+    middle node of three, src/interpolate.rs:124-132): the reference's value lookup then
+    panics (:483-530; i - 2 wraps first), the oracle returns "not enough neighbours".
+    Four points are enough everywhere."""
+    def grid(ms):
+        return O.Grid(staircase_labels(TEFF[:4], np.array(ms), LOGG, [(0, 4)] * 4))
+
+    def ok(g, m):
+        idx, w32, w64, *_ = g.local_weights(5300.0, m, 4.1)
+        assert abs(w64.sum() - 1.0) < 1e-12 and (idx < g.n_models).all()
+
+    def err(g, m):
+        with pytest.raises(O.OracleError) as e:
+            g.local_weights(5300.0, m, 4.1)
+        assert e.value.code == -2, m
+
+    two = grid([-0.5, 0.0])
+    for m in (-0.5, -0.2, 0.0):
+        err(two, m)
+    three = grid([-0.5, 0.0, 0.5])
+    for m in (-0.5, -0.2, 0.3, 0.5):
+        ok(three, m)
+    err(three, 0.0)  # exact middle node: i == right - 1 = 1, neighbour i - 2 = -1
+    four = grid([-0.5, 0.0, 0.5, 1.0])
+    for m in (-0.5, -0.2, 0.0, 0.3, 0.5, 0.7, 1.0):
+        ok(four, m)
