@@ -386,8 +386,9 @@ def fixed_gauss_kernel(resolution, log_step):
 
 
 def produce_model(grid, disp, labels, f64=False):
-    """f64=False: the f32-faithful restatement; True: every step in FP64 with the exact
-    resample weight; "q": FP64 with the reference's f32-quantised resample weight."""
+    """f64=False: the f32-faithful restatement; True: every step in FP64 with exact
+    weights; "q": FP64 arithmetic over the values the reference stores in f32 (the 64
+    interpolation weights, the rotation and instrumental taps, the resample weight)."""
     lab = _d(labels)
     M = grid.model()
     if f64:
@@ -412,7 +413,7 @@ def interpolate_and_convolve(grid, disp, labels4):
 
 def cost(grid, disp, cont, flux, var, labels, f64=False):
     """chi2 of one label vector; returns (chi2, params) or raises OracleError.  f64 as in
-    produce_model ("q": FP64 with the quantised resample weight, params in f64)."""
+    produce_model ("q": FP64 over the reference's f32-stored values, params in f64)."""
     lab = _d(labels)
     flux, var = _f(flux), _f(var)
     M = grid.model()

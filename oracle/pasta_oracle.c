@@ -398,12 +398,15 @@ int or_interpolate_f32(const or_grid *g, const uint16_t *spectra, const float *f
     return OR_OK;
 }
 
-int or_interpolate_f64(const or_grid *g, const uint16_t *spectra, const float *factors, int W,
-                       double teff, double m, double logg, double *out) {
+/* f32w != 0: the 64 weights rounded to f32 as the reference stores them (cubic.rs:241) */
+static int interpolate_f64_impl(const or_grid *g, const uint16_t *spectra, const float *factors,
+                                int W, double teff, double m, double logg, double *out, int f32w) {
     int idx[64];
     double w[64];
     int rc = or_local_weights(g, teff, m, logg, idx, NULL, w, NULL, NULL, NULL);
     if (rc) return rc;
+    if (f32w)
+        for (int j = 0; j < 64; j++) w[j] = (double)(float)w[j];
     double fac[64];
     for (int j = 0; j < 64; j++)
         fac[j] = (idx[j] >= 0 && factors) ? (double)factors[idx[j]] : 1.0 / 65535.0;
@@ -418,6 +421,11 @@ int or_interpolate_f64(const or_grid *g, const uint16_t *spectra, const float *f
         out[x] = y;
     }
     return OR_OK;
+}
+
+int or_interpolate_f64(const or_grid *g, const uint16_t *spectra, const float *factors, int W,
+                       double teff, double m, double logg, double *out) {
+    return interpolate_f64_impl(g, spectra, factors, W, teff, m, logg, out, 0);
 }
 
 /* ======================================================================
@@ -820,15 +828,18 @@ int or_produce_model(const or_model *M, const or_disp *d, const double lab[5], f
 }
 
 /* ---------------- FP64 mode ---------------- */
-/* qweight != 0: the resample weight is the reference's f32(index) - f32(j)
- * (convolve_rv.rs:570-572), used in f64; every other step stays f64.  This separates the
- * f32 arithmetic error of a path from the weight quantisation the reference's own result
- * carries (up to 7.8e-3 of a pixel step for indices >= 131072). */
+/* qweight != 0: FP64 arithmetic over the values the reference stores in f32: the 64
+ * interpolation weights (cubic.rs:241), the rotation and instrumental taps
+ * (convolve_rv.rs:107-118,468-474) and the resample weight f32(index) - f32(j) (:570-572).
+ * This separates the f32 arithmetic error of a path from those quantisations, which the
+ * reference's own result carries (the resample weight alone up to 7.8e-3 of a pixel step
+ * for indices >= 131072). */
 static int produce_model_f64_impl(const or_model *M, const or_disp *d, const double lab[5],
                                   double *out, int qweight) {
     int W = M->sg.n;
     double *interp = malloc(sizeof(double) * W);
-    int rc = or_interpolate_f64(M->g, M->spectra, M->factors, W, lab[0], lab[1], lab[2], interp);
+    int rc = interpolate_f64_impl(M->g, M->spectra, M->factors, W, lab[0], lab[1], lab[2], interp,
+                                  qweight);
     if (rc) { free(interp); return rc; }
     double vsini = lab[3];
     if (vsini < 0.0) { free(interp); return OR_ERR_VSINI; }
@@ -840,7 +851,14 @@ static int produce_model_f64_impl(const or_model *M, const or_disp *d, const dou
     } else {
         if (n > FFTSIZE) { free(interp); return OR_ERR_KERNEL; }
         double *k = malloc(sizeof(double) * n);
-        or_rotation_kernel_f64(vsini, dvelo, k, n);
+        if (qweight) {  /* the reference's f32 taps, normalised by their f32 sum */
+            float *k32 = malloc(sizeof(float) * n);
+            or_rotation_kernel(vsini, dvelo, k32, n);
+            for (int i = 0; i < n; i++) k[i] = (double)k32[i];
+            free(k32);
+        } else {
+            or_rotation_kernel_f64(vsini, dvelo, k, n);
+        }
         int ylen = or_oa_out_len(W, n);
         double *y = calloc((size_t)ylen, sizeof(double));
         or_direct_convolve_f64(interp, W, k, n, y); /* W+n-1 <= ylen; rest zero */
@@ -854,12 +872,16 @@ static int produce_model_f64_impl(const or_model *M, const or_disp *d, const dou
         if (d->kind == OR_DISP_FIXED) {
             int K = d->K, h = (K - 1) / 2;
             double *kk = malloc(sizeof(double) * K), ks = 0.0;
-            for (int i = 0; i < K; i++) {
-                double x = ((double)i - (double)h) / d->sigma;
-                kk[i] = exp(-0.5 * x * x);
-                ks += kk[i];
+            if (qweight) {  /* the reference's f32 taps (make_gaussian) */
+                for (int i = 0; i < K; i++) kk[i] = (double)d->kernel[i];
+            } else {
+                for (int i = 0; i < K; i++) {
+                    double x = ((double)i - (double)h) / d->sigma;
+                    kk[i] = exp(-0.5 * x * x);
+                    ks += kk[i];
+                }
+                for (int i = 0; i < K; i++) kk[i] /= ks;
             }
-            for (int i = 0; i < K; i++) kk[i] /= ks;
             double *y = malloc(sizeof(double) * (W + K - 1));
             or_direct_convolve_f64(flat, W, kk, K, y);
             memcpy(res, y + h, sizeof(double) * W);

@@ -2,8 +2,11 @@
 factor spanning 1e3-1e7, flux/max stored as uint16), wavelength-dependent
 resolution R = 60,000 -> 90,000 (per-pixel Gaussian kernels), 16 chunks of degree
 10.  The GPU (FP32 with f64 reductions and FP64 solves) is held to 1e-5 of the
-f32-faithful oracle, and its error against the FP64 oracle is measured and
-bounded; both figures are printed."""
+f32-faithful oracle.  Against the FP64 oracle over the values the reference stores in
+f32 (interpolation weights, taps, resample weight: f64="q") the GPU's error is the FP32
+arithmetic alone; it is bounded at what it measures and set beside the f32 oracle's own
+distance to the same FP64 result (the reference's FP32 arithmetic); all figures are
+printed."""
 import numpy as np
 import pytest
 
@@ -43,37 +46,64 @@ def test_factor_range(prec):
 
 
 def test_models_vs_f32_and_fp64_oracles(prec):
+    """Errors relative to the spectrum's scale (max |flux|): with per-model factors the
+    interpolation cancels large terms, so pointwise relative error is not meaningful.
+    Against FP64 with the reference's quantised resample weight ("q") the difference is
+    the FP32 arithmetic alone; against FP64 with the exact weight it is dominated by the
+    reference's own f32(index) - f32(j) weight (src/convolve_rv.rs:570-572)."""
     out = prec["interp"].produce_model_bulk(prec["d"], prec["labs"])
-    worst32 = worst64 = 0.0
+    worst32 = worst64q = worst_ref64q = worst64 = 0.0
     for k, lab in enumerate(prec["labs"]):
         m32 = O.produce_model(prec["og"], prec["od"], lab)
+        m64q = O.produce_model(prec["og"], prec["od"], lab, f64="q")
         m64 = O.produce_model(prec["og"], prec["od"], lab, f64=True)
-        scale = np.abs(m64).max()
-        # errors relative to the spectrum's scale (max |flux|): with per-model factors the
-        # interpolation cancels large terms, so pointwise relative error is not meaningful
+        scale = np.abs(m64q).max()
         worst32 = max(worst32, float(np.max(np.abs(out[k] - m32)) / np.abs(m32).max()))
+        worst64q = max(worst64q, float(np.max(np.abs(out[k] - m64q)) / scale))
+        worst_ref64q = max(worst_ref64q, float(np.max(np.abs(m32 - m64q)) / scale))
         worst64 = max(worst64, float(np.max(np.abs(out[k] - m64)) / scale))
-    print(f"model: GPU vs f32 oracle max err / max {worst32:.2e}; GPU vs FP64 oracle max err / max "
-          f"{worst64:.2e}", flush=True)
+    print(f"model (max err / max |flux|): GPU vs f32 oracle {worst32:.2e}; vs FP64 with the "
+          f"quantised weight: GPU {worst64q:.2e}, f32 oracle {worst_ref64q:.2e}; GPU vs FP64 with "
+          f"the exact weight {worst64:.2e}", flush=True)
     assert worst32 < 1e-5
-    # FP32 arithmetic plus the reference's f32-quantised resample weight (<= 1.5e-3 of a
-    # pixel-to-pixel flux step at these indices)
+    # FP32 arithmetic (~5e-7 of scale measured), of the order of the reference's own
+    assert worst64q < 2e-6 and worst64q < 2 * worst_ref64q
+    # the quantised weight: <= 1.5e-3 of a pixel-to-pixel flux step at these indices
     assert worst64 < 2e-3
 
 
-def test_chi2_vs_f32_and_fp64_oracles(prec):
-    chi, st, _ = prec["star"].chi2(prec["labs"])
+@pytest.mark.parametrize("cont", ["chunk16x10", "constant"])
+def test_chi2_vs_f32_and_fp64_oracles(prec, cont):
+    """chi2 against the f32-faithful oracle (1e-5) and against FP64 with the quantised
+    resample weight.  For the constant continuum the mean of f/m is the reference's
+    sequential f32 fold (continuum_fitting.rs:490-492) on the GPU and in the f32 oracle;
+    its distance to the FP64 mean is the reference's own rounding, reported beside."""
+    import pasta_amd as pasta
+    if cont == "constant":
+        n = len(prec["flux"])
+        star = pasta.Star(prec["interp"], prec["d"], pasta.ConstantContinuum(), prec["flux"],
+                          prec["var"])
+        oc = O.Continuum("constant", n)
+    else:
+        star, oc = prec["star"], prec["oc"]
+    chi, st, _ = star.chi2(prec["labs"])
     assert not st.any()
-    worst32 = worst64 = 0.0
+    worst32 = worst64q = worst_ref64q = 0.0
     for k, lab in enumerate(prec["labs"]):
-        c32, _ = O.cost(prec["og"], prec["od"], prec["oc"], prec["flux"], prec["var"], lab)
-        c64, _ = O.cost(prec["og"], prec["od"], prec["oc"], prec["flux"], prec["var"], lab, f64=True)
+        c32, _ = O.cost(prec["og"], prec["od"], oc, prec["flux"], prec["var"], lab)
+        c64q, _ = O.cost(prec["og"], prec["od"], oc, prec["flux"], prec["var"], lab, f64="q")
         worst32 = max(worst32, abs(chi[k] - c32) / c32)
-        worst64 = max(worst64, abs(chi[k] - c64) / c64)
-    print(f"chi2: GPU vs f32 oracle max rel {worst32:.2e}; GPU vs FP64 oracle max rel "
-          f"{worst64:.2e}", flush=True)
+        worst64q = max(worst64q, abs(chi[k] - c64q) / c64q)
+        worst_ref64q = max(worst_ref64q, abs(c32 - c64q) / c64q)
+    print(f"chi2[{cont}]: GPU vs f32 oracle max rel {worst32:.2e}; vs FP64 (quantised weight): "
+          f"GPU {worst64q:.2e}, f32 oracle {worst_ref64q:.2e}", flush=True)
     assert worst32 < 1e-5
-    assert worst64 < 5e-2
+    if cont == "constant":
+        assert worst64q < 2 * worst_ref64q + 1e-6  # the f32 fold's own error, on both sides
+    else:
+        # ~2.3e-6 measured on both sides: the continuum parameters are f32 values
+        # (src/continuum_fitting.rs:223) in the GPU path and the f32 oracle alike
+        assert worst64q < 5e-6 and worst64q < 2 * worst_ref64q + 1e-7
 
 
 def test_fit_best_labels_vs_fp64(prec):
