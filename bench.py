@@ -29,38 +29,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 R_INSTR = 85_000.0
-# HBM bytes per batched k_model launch (512 stars x 64 particles), measured with
-# rocprofv3 FETCH_SIZE / WRITE_SIZE and scaled by a plain copy kernel; written there
-# as "traffic_bytes_per_launch: <bytes>"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_model_batched_512x64_hbm_traffic.md")
+# PMC figures of k_model (rocprofv3 counter passes of this script's batched leg), written in
+# profiles/ as "<key>: <value>" lines under "window: <stars>x<particles> warmup <W> steps <K>"
+# (the k_model launches of the timed iterations of that command), so that they are only used
+# beside kernel times of the same iterations
+PMC_FILES = [os.path.join(ROOT, "profiles", "k_model_batched_512x64_hbm_traffic.md"),
+             os.path.join(ROOT, "profiles", "k_model_k_moments_batched_512x64_mfma_valu.md")]
 
 
-def measured_traffic(stars, particles):
-    if (stars, particles) != (512, 64) or not os.path.exists(TRAFFIC_FILE):
-        return None
-    for line in open(TRAFFIC_FILE):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
-
-
-# matrix-core / VALU utilisation of the same launches (rocprofv3 PMC), written there as
-# "<key>: <value>" lines
-UTIL_FILE = os.path.join(ROOT, "profiles", "k_model_k_moments_batched_512x64_mfma_valu.md")
-
-
-def measured_utilisation(stars, particles):
-    if (stars, particles) != (512, 64) or not os.path.exists(UTIL_FILE):
-        return None
-    keys = ("mfma_util_k_model", "mfma_tflops_k_model_bf16", "mfma_util_k_moments_mfma",
-            "mfma_tflops_k_moments_mfma_f64", "valu_util_k_model", "valu_util_k_model_2cyc")
-    out = {}
-    for line in open(UTIL_FILE):
-        k, _, v = line.partition(":")
-        if k in keys:
-            out[k] = float(v)
-    return out or None
+def measured_pmc(stars, particles, warmup, steps):
+    want = f"{stars}x{particles} warmup {warmup} steps {steps}"
+    out, on = {}, False
+    for path in PMC_FILES:
+        if not os.path.exists(path):
+            continue
+        for line in open(path):
+            k, _, v = line.strip().partition(":")
+            if k == "window":
+                on = v.strip() == want
+            elif on and k and v.strip():
+                try:
+                    out[k.strip()] = float(v)
+                except ValueError:
+                    pass
+    return out
 
 
 def log(*a):
@@ -113,7 +107,7 @@ def run_batched(pasta, _lib, interp, stars, meta, P, warmup, steps, timing_every
             f"{time.perf_counter() - t_fit:.2f} s")
         for _ in range(warmup):
             _lib.check(_lib.lib.pasta_server_step(sv, C.byref(a), C.byref(w)), "step")
-        tm = (C.c_double * 5)()
+        tm = (C.c_double * 7)()
         _lib.lib.pasta_server_timing(sv, 1, tm)
         t0 = time.perf_counter()
         for k in range(steps):
@@ -130,7 +124,8 @@ def run_batched(pasta, _lib, interp, stars, meta, P, warmup, steps, timing_every
         while a.value > 0:  # the rest of the fits
             _lib.check(_lib.lib.pasta_server_step(sv, C.byref(a), C.byref(w)), "step")
         fit_s = time.perf_counter() - t_fit
-        return elapsed, tm[1] / tm[4], tm[3] / tm[4], fit_s, iters
+        return dict(elapsed=elapsed, model_ms=tm[1] / tm[4], iter_ms=tm[3] / tm[4], fit_s=fit_s,
+                    iters=iters, flops=tm[5] / tm[4], bytes=tm[6] / tm[4], timed=int(tm[4]))
     finally:
         _lib.lib.pasta_server_destroy(sv)
 
@@ -199,9 +194,10 @@ def run_binary(pasta, synth, interp, wlg, spec, P=128, reps=6, fit_iters=20, see
 
 
 def serving_requests(pasta, synth, interp, wlg, spec, n, seed=5000):
-    """Serving-config requests: lengths lognormal (median 80,000, clipped 20,000-200,000),
-    labels uniform in the grid, vsini U(0,300), RV U(-150,150), SNR U(30,300), continuum
-    modes cycling chunk(10, 8) / linear (30 smooth columns) / constant."""
+    """Serving-config requests (the client's data): lengths lognormal (median 80,000, clipped
+    20,000-200,000), labels uniform in the grid, vsini U(0,300), RV U(-150,150), SNR
+    U(30,300), continuum modes cycling chunk(10, 8) / linear (30 smooth columns) /
+    constant.  Only wavelengths, flux, variance and the mode travel with a request."""
     rng = np.random.default_rng(seed)
     reqs = []
     for k in range(n):
@@ -213,7 +209,7 @@ def serving_requests(pasta, synth, interp, wlg, spec, n, seed=5000):
         flux, var = synth.observe(interp.produce_model(disp, *lab), wl, rng.uniform(30, 300),
                                   seed + 1 + k)
         mode = ("chunk", "linear", "constant")[k % 3]
-        reqs.append(dict(wl=wl, disp=disp, flux=flux, var=var, mode=mode, n=npx,
+        reqs.append(dict(wl=wl, flux=flux, var=var, mode=mode, n=npx,
                          design=synth.smooth_design(npx, 30, seed + k) if mode == "linear" else None))
     return reqs
 
@@ -226,30 +222,43 @@ def _continuum(pasta, r):
     return pasta.ConstantContinuum()
 
 
+def prepare_star(pasta, interp, wlg, r):
+    """A request's server-side preparation: dispersion (float indices, instrumental taps),
+    continuum model (chunk bounds, design rows, Cholesky factors) and the star's upload."""
+    disp = pasta.FixedResolutionDispersion(r["wl"], R_INSTR, wlg)
+    return pasta.Star(interp, disp, _continuum(pasta, r), r["flux"], r["var"])
+
+
 def pct(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else None
 
 
-def run_serving(pasta, interp, reqs, rate, n_req, P, iters, iters_per_step, n_plain, seed):
+def run_serving(pasta, interp, wlg, reqs, rate, n_req, P, iters, iters_per_step, n_plain, seed,
+                workers=8):
     """Poisson arrivals at `rate` req/s of the first n_req requests into the server (continuous
     batching), then the first n_plain of the same arrivals through the plain path, one fit at
-    a time in arrival order.  Latency = finish - scheduled arrival."""
+    a time in arrival order.  Each request is prepared on a pool of host threads from its
+    arrival on (the C calls release the GIL, so preparation overlaps the device's work);
+    latency = result time - scheduled arrival, preparation included."""
+    from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(seed)
     arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
     settings = pasta.PSOSettings(P, iters)
-    t0 = time.perf_counter()
-    stars = [pasta.Star(interp, reqs[k]["disp"], _continuum(pasta, reqs[k]), reqs[k]["flux"],
-                        reqs[k]["var"]) for k in range(n_req)]
-    prep_ms = (time.perf_counter() - t0) / n_req * 1e3
     srv = pasta.FitServer(interp, max_stars=1024, max_particles=P, iters_per_step=iters_per_step)
+    pool = ThreadPoolExecutor(max_workers=workers)
     try:
-        ids, lat, sub, steps, busy = {}, {}, 0, 0, 0
+        pending, ids, lat, prep, sub, steps, busy = {}, {}, {}, [], 0, 0, 0
         t0 = time.perf_counter()
         while len(lat) < n_req:
             now = time.perf_counter() - t0
             while sub < n_req and arr[sub] <= now:
-                ids[sub] = srv.submit(stars[sub], settings, (0.0, 300.0), (-150.0, 150.0), seed=sub)
+                pending[sub] = (pool.submit(prepare_star, pasta, interp, wlg, reqs[sub]),
+                                time.perf_counter())
                 sub += 1
+            for k in [k for k, (f, _) in pending.items() if f.done()]:
+                f, tq = pending.pop(k)
+                prep.append(time.perf_counter() - tq)
+                ids[k] = srv.submit(f.result(), settings, (0.0, 300.0), (-150.0, 150.0), seed=k)
             active, waiting = srv.step()
             steps += 1
             busy = max(busy, active)
@@ -257,35 +266,42 @@ def run_serving(pasta, interp, reqs, rate, n_req, P, iters, iters_per_step, n_pl
             for k, rid in ids.items():
                 if k not in lat and srv.poll(rid) is not None:
                     lat[k] = now - arr[k]
-            if active == 0 and waiting == 0 and sub < n_req and len(lat) == sub:
+            if active == 0 and waiting == 0 and not pending and sub < n_req and len(lat) == sub:
                 time.sleep(max(0.0, arr[sub] - (time.perf_counter() - t0)))
+            elif active == 0 and waiting == 0 and pending:
+                time.sleep(0.0005)  # only preparation in flight
         makespan = time.perf_counter() - t0
-        evals = srv.stats()[0]
+        stats = srv.stats()
     finally:
+        pool.shutdown()
         srv.close()
+    evals, busy_us, rebuilds, rebuild_us = stats[0], stats[4], stats[5], stats[6]
     lat_s = [lat[k] for k in range(n_req)]
-    # the plain path, same arrivals, one fit at a time: the service time of each of the
-    # first n_plain requests is measured; FIFO latencies over all n_req arrivals follow
-    # from finish_k = max(arrival_k, finish_(k-1)) + service_(k mod n_plain)
+    # the plain path, same arrivals, one fit at a time: the service time (preparation + fit)
+    # of each of the first n_plain requests is measured; FIFO latencies over all n_req
+    # arrivals follow from finish_k = max(arrival_k, finish_(k-1)) + service_(k mod n_plain)
     from pasta_amd import _lib
     n_plain = min(n_plain, n_req)
-    pstars = [pasta.Star(interp, reqs[k]["disp"], _continuum(pasta, reqs[k]), reqs[k]["flux"],
-                         reqs[k]["var"]) for k in range(n_plain)]
     st = pasta.api._settings_c(settings, (0.0, 300.0), (-150.0, 150.0), None)
     res = _lib.FitResult()
     service = []
     for k in range(n_plain):
-        t0 = time.perf_counter()
-        _lib.check(_lib.lib.pasta_fit(pstars[k]._h, C.byref(st), k, None, C.byref(res), None), "fit")
-        service.append(time.perf_counter() - t0)
+        t1 = time.perf_counter()
+        star = prepare_star(pasta, interp, wlg, reqs[k])
+        _lib.check(_lib.lib.pasta_fit(star._h, C.byref(st), k, None, C.byref(res), None), "fit")
+        service.append(time.perf_counter() - t1)
     plain_lat, fin = [], 0.0
     for k in range(n_req):
         fin = max(arr[k], fin) + service[k % n_plain]
         plain_lat.append(fin - arr[k])
     return dict(rate=rate, requests=n_req, p50_s=pct(lat_s, 50), p99_s=pct(lat_s, 99),
                 evals_per_s=evals / makespan, makespan_s=makespan, steps=steps, max_active=busy,
-                star_prep_ms=prep_ms, plain_service_ms=1e3 * float(np.mean(service)),
-                plain_measured=n_plain, plain_p50_s=pct(plain_lat, 50), plain_p99_s=pct(plain_lat, 99))
+                gpu_busy_s=busy_us * 1e-6, gpu_idle_frac=max(0.0, 1.0 - busy_us * 1e-6 / makespan),
+                batch_rebuilds=rebuilds, rebuild_host_s=rebuild_us * 1e-6,
+                prep_p50_ms=1e3 * pct(prep, 50),
+                prep_p99_ms=1e3 * pct(prep, 99), prep_workers=workers,
+                plain_service_ms=1e3 * float(np.mean(service)), plain_measured=n_plain,
+                plain_p50_s=pct(plain_lat, 50), plain_p99_s=pct(plain_lat, 99))
 
 
 def cpu_baseline(interp, spec, plain, budget_s):
@@ -300,7 +316,9 @@ def cpu_baseline(interp, spec, plain, budget_s):
     od = O.Dispersion(og, O.DISP_FIXED, plain["wl"], resolution=R_INSTR)
     oc = O.Continuum("chunk", len(plain["wl"]), wl=plain["wl"], n_chunks=10, p_order=8, overlap=0.2)
     log(f"[bench] oracle set-up {time.perf_counter() - t0:.1f} s")
-    threads = min(len(os.sched_getaffinity(0)), 16)
+    # the host's CPU share for one GPU on the GPU box is 16 threads: the pool is capped there
+    affinity = len(os.sched_getaffinity(0))
+    threads = min(affinity, 16)
     rng = np.random.default_rng(77)
     pool = 16384
     labels = np.stack([rng.uniform(4200, 9800, pool), rng.uniform(-0.95, 0.45, pool),
@@ -328,19 +346,39 @@ def cpu_baseline(interp, spec, plain, budget_s):
     ref = np.concatenate([first, rest])
     ok = st == 0
     rels = np.abs(gpu[ok] - ref[ok]) / np.abs(ref[ok])
+    # both against FP64 over the reference's f32-stored values, on a subsample: which side
+    # the differences come from
+    nq = min(len(lab_all), 8 * threads)
+    t0 = time.perf_counter()
+    ref64 = O.cost_batch(og, od, oc, plain["flux"], plain["var"], lab_all[:nq], threads, f64="q")
+    okq = ok[:nq]
+    e_gpu = np.abs(gpu[:nq][okq] - ref64[okq]) / ref64[okq]
+    e_ref = np.abs(ref[:nq][okq] - ref64[okq]) / ref64[okq]
+    log(f"[bench] FP64 oracle on {nq} evaluations in {time.perf_counter() - t0:.1f} s")
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return dict(value=n / dt, cores=threads, n=n, seconds=dt, parity_max_rel=float(rels.max()),
                 parity_p50_rel=float(np.percentile(rels, 50)),
                 parity_p99_rel=float(np.percentile(rels, 99)),
                 parity_over_1e5=int((rels > 1e-5).sum()), n_parity=int(ok.sum()),
-                one_core=n1 / dt1, n1=n1)
+                one_core=n1 / dt1, n1=n1, nproc=os.cpu_count(), affinity=affinity, cpu_model=model,
+                fp64_n=int(okq.sum()), gpu_vs_fp64_max=float(e_gpu.max()),
+                gpu_vs_fp64_p50=float(np.percentile(e_gpu, 50)),
+                ref_vs_fp64_max=float(e_ref.max()), ref_vs_fp64_p50=float(np.percentile(e_ref, 50)))
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=190,
+    ap.add_argument("--steps", type=int, default=20,
                     help="timed PSO iterations (the fit has max(200, warmup + steps + 3))")
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--stars", type=int, default=512)
     ap.add_argument("--particles", type=int, default=64)
     ap.add_argument("--n-range", type=int, nargs=2, default=(40_000, 200_000),
@@ -383,16 +421,19 @@ def main():
     log(f"[bench] {len(stars)} stars set up in {time.perf_counter() - t0:.1f} s "
         f"(N {min(m['n'] for m in meta)}..{max(m['n'] for m in meta)})")
 
-    elapsed, model_ms, iter_ms, fit_s, fit_iters = run_batched(
-        pasta, _lib, interp, stars, meta, args.particles, args.warmup, args.steps)
+    rb = run_batched(pasta, _lib, interp, stars, meta, args.particles, args.warmup, args.steps)
+    elapsed, model_ms, iter_ms, fit_s, fit_iters = (rb["elapsed"], rb["model_ms"], rb["iter_ms"],
+                                                    rb["fit_s"], rb["iters"])
     n_evals_step = args.particles * len(stars)
     value = n_evals_step * args.steps / elapsed
-    bytes_launch = float(sum(args.particles * (64 * 2 * m["span"] + 8 * m["n"]) for m in meta))
-    achieved = bytes_launch / (model_ms * 1e-3) / 1e9
-    traffic = measured_traffic(len(stars), args.particles)
+    tflops = rb["flops"] / (model_ms * 1e-3) / 1e12
+    pmc = measured_pmc(len(stars), args.particles, args.warmup, args.steps)
+    traffic = pmc.get("traffic_bytes_per_launch")
     log(f"[bench] batched: {value:,.0f} evals/s, {elapsed / args.steps * 1e3:.1f} ms/step, "
-        f"k_model {model_ms:.2f} ms ({achieved:,.0f} GB/s algorithmic), iteration {iter_ms:.2f} ms; "
-        f"{len(stars)} fits of {fit_iters} iterations in {fit_s:.2f} s")
+        f"k_model {model_ms:.2f} ms ({tflops:.1f} TFLOP/s FP32 algorithmic, "
+        f"{rb['bytes'] / (model_ms * 1e-3) / 1e9:,.0f} GB/s algorithmic bytes; timed over "
+        f"{rb['timed']} eager launches), iteration {iter_ms:.2f} ms; {len(stars)} fits of "
+        f"{fit_iters} iterations in {fit_s:.2f} s")
 
     plain = None
     if not args.no_plain:
@@ -415,22 +456,26 @@ def main():
         reqs = serving_requests(pasta, synth, interp, wlg, spec, max(counts))
         log(f"[bench] {len(reqs)} serving requests generated in {time.perf_counter() - t0:.1f} s")
         for k, (r, n) in enumerate(zip(rates, counts)):
-            sv = run_serving(pasta, interp, reqs, r, n, args.serve_particles, args.serve_iters,
-                             2, 40, 6000 + k)
+            sv = run_serving(pasta, interp, wlg, reqs, r, n, args.serve_particles,
+                             args.serve_iters, 2, 40, 6000 + k)
             serving.append(sv)
             log(f"[bench] serving {r:g} req/s: {n} requests, p50 {sv['p50_s']:.3f} s, p99 "
                 f"{sv['p99_s']:.3f} s, {sv['evals_per_s']:,.0f} evals/s, max {sv['max_active']} "
-                f"stars in flight; plain path one at a time (service {sv['plain_service_ms']:.1f} ms): "
-                f"p50 {sv['plain_p50_s']:.3f} s, p99 {sv['plain_p99_s']:.3f} s")
+                f"stars in flight, GPU idle {sv['gpu_idle_frac']:.2f}, preparation p50 "
+                f"{sv['prep_p50_ms']:.1f} ms; plain path one at a time (service "
+                f"{sv['plain_service_ms']:.1f} ms): p50 {sv['plain_p50_s']:.3f} s, p99 "
+                f"{sv['plain_p99_s']:.3f} s")
 
     cpu = None
     if not args.no_cpu_baseline and plain is not None:
         cpu = cpu_baseline(interp, spec, plain, args.cpu_seconds)
-        log(f"[bench] cpu oracle: {cpu['value']:.1f} evals/s on {cpu['cores']} threads, "
-            f"{cpu['one_core']:.1f} evals/s on 1 ({cpu['n1']} evals) "
-            f"({cpu['n']} evals, {cpu['seconds']:.1f} s); GPU-vs-oracle chi2 max rel "
-            f"{cpu['parity_max_rel']:.2e} (p50 {cpu['parity_p50_rel']:.1e}, p99 "
-            f"{cpu['parity_p99_rel']:.1e}) over {cpu['n_parity']} full-size evaluations")
+        log(f"[bench] cpu oracle: {cpu['value']:.1f} evals/s on {cpu['cores']} threads "
+            f"({cpu['cpu_model']}, nproc {cpu['nproc']}), {cpu['one_core']:.1f} evals/s on 1 "
+            f"({cpu['n1']} evals) ({cpu['n']} evals, {cpu['seconds']:.1f} s); GPU-vs-oracle chi2 "
+            f"max rel {cpu['parity_max_rel']:.2e} (p50 {cpu['parity_p50_rel']:.1e}, p99 "
+            f"{cpu['parity_p99_rel']:.1e}) over {cpu['n_parity']} full-size evaluations; against "
+            f"FP64 ({cpu['fp64_n']}): GPU max {cpu['gpu_vs_fp64_max']:.2e}, f32 oracle max "
+            f"{cpu['ref_vs_fp64_max']:.2e}")
 
     out = {
         # value is the batched throughput; the per-star fit latencies (p50/p99) of the
@@ -460,25 +505,33 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm",
+            # k_model is bound on chip (its HBM traffic is ~0.1 of peak): its roofline is the
+            # FP32 compute one, with the algorithmic flops of the reference's direct count
+            # (interpolation 128 W, rotation + instrumental 2 (K_rot + K_instr) W with each
+            # evaluation's own K_rot, resample 6 N) over the FP32 peak
+            "bound": "mfma",
             "kernel": "k_model",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "achieved": round(tflops, 2),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": bytes_launch,
+            "flops_per_launch": rb["flops"],
             "kernel_ms": round(model_ms, 3),
-            # algorithmic bytes exceed HBM peak through L2 / MALL reuse across evaluations;
-            # the HBM bytes actually moved (PMC) against the same peak:
-            "measured_hbm_GBs": round(traffic / (model_ms * 1e-3) / 1e9, 1) if traffic else None,
-            "measured_hbm_frac": round(traffic / (model_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            "timed_launches": rb["timed"],
+            "hbm_GBs": round(traffic / (model_ms * 1e-3) / 1e9, 1) if traffic else None,
+            "hbm_frac": round(traffic / (model_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
             if traffic else None,
+            # algorithmic bytes (64 u16 neighbour rows per synthetic pixel + 8 B per observed
+            # pixel) over the HBM bytes moved: the reuse the L2 / MALL provide
+            "algorithmic_bytes_per_launch": rb["bytes"],
+            "reuse_factor": round(rb["bytes"] / traffic, 2) if traffic else None,
+            "l2_to_cu_bytes_per_launch": pmc.get("l2_to_cu_bytes_per_launch"),
             "iteration_ms": round(iter_ms, 3),
-            # PMC: matrix-core busy fraction and rate of the rotation (bf16 x 3) in k_model
-            # and of the f64 continuum moments, VALU busy fraction of k_model at 4 and at 2
-            # cycles per wave64 instruction
-            "mfma": measured_utilisation(len(stars), args.particles),
+            # PMC of the same iterations: matrix-core busy fraction and rate of the rotation
+            # (bf16 x 3) in k_model, VALU busy fraction of k_model
+            "pmc": {k: v for k, v in pmc.items() if k not in
+                    ("traffic_bytes_per_launch", "l2_to_cu_bytes_per_launch")} or None,
         },
     }
     out["batched_fit_latency_s"] = round(fit_s, 3)  # all stars' fits, submit to last result
@@ -496,14 +549,23 @@ def main():
             "cores": cpu["cores"],
             "kind": "port",
             "sample": f"{cpu['n']} chi2 evaluations of the N=150000 star on the full grid "
-                      f"(C oracle, {cpu['cores']} threads)",
+                      f"(C oracle, {cpu['cores']} threads: the host's CPU share for one GPU)",
             "one_core_value": round(cpu["one_core"], 2),
             "one_core_sample": f"{cpu['n1']} evaluations on 1 thread",
+            "nproc": cpu["nproc"],
+            "affinity_cpus": cpu["affinity"],
+            "cpu_model": cpu["cpu_model"],
         }
-        out["parity_full_size"] = {"max_rel_chi2": cpu["parity_max_rel"],
-                                   "p50_rel_chi2": cpu["parity_p50_rel"],
-                                   "p99_rel_chi2": cpu["parity_p99_rel"],
-                                   "over_1e-5": cpu["parity_over_1e5"], "n": cpu["n_parity"]}
+        out["parity_full_size"] = {
+            "max_rel_chi2": cpu["parity_max_rel"], "p50_rel_chi2": cpu["parity_p50_rel"],
+            "p99_rel_chi2": cpu["parity_p99_rel"], "over_1e-5": cpu["parity_over_1e5"],
+            "n": cpu["n_parity"],
+            # against FP64 over the reference's f32-stored values (weights, taps, resample
+            # weight): the GPU's error and the f32 restatement's own, on the first fp64_n
+            "fp64_n": cpu["fp64_n"],
+            "gpu_vs_fp64_max_rel": cpu["gpu_vs_fp64_max"], "gpu_vs_fp64_p50_rel": cpu["gpu_vs_fp64_p50"],
+            "f32_oracle_vs_fp64_max_rel": cpu["ref_vs_fp64_max"],
+            "f32_oracle_vs_fp64_p50_rel": cpu["ref_vs_fp64_p50"]}
     if binary is not None:
         out["binary"] = {
             "workload": f"binary stars (BinaryCostFunction): full grid + full continuum grid "
@@ -517,8 +579,10 @@ def main():
             "workload": f"Poisson arrivals, lognormal N (median 80k, 20k-200k), continuum modes "
                         f"chunk(10,8)/linear(30 columns)/constant, PSO {args.serve_particles} "
                         f"particles x {args.serve_iters} iterations per request; latency from "
-                        f"scheduled arrival to result; each request's star (upload, continuum "
-                        f"factorisation) is prepared before the arrivals start (star_prep_ms)",
+                        f"scheduled arrival to result, including each request's preparation "
+                        f"(dispersion, continuum factorisation, upload) on a pool of host "
+                        f"threads from its arrival on; gpu_idle_frac = 1 - device-busy time "
+                        f"(HIP events around first evaluations and iterations) / makespan",
             "rates": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in sv.items()}
                       for sv in serving],
         }

@@ -365,12 +365,17 @@ int pasta_server_cancel(pasta_server *sv, uint64_t request_id);
  * [3] HBM bytes in use by in-flight stars (their evaluation buffers, PSO state and the
  * stars themselves), [4] device-busy microseconds (first evaluations of admitted stars and
  * PSO iterations, HIP events on the server's stream), [5] batch rebuilds (admissions and
- * retirements; the PSO state of stars in flight moves between batches on the device) */
-int pasta_server_stats(const pasta_server *sv, int64_t out[6]);
+ * retirements; the PSO state of stars in flight moves between batches on the device),
+ * [6] host microseconds spent in those rebuilds */
+int pasta_server_stats(const pasta_server *sv, int64_t out[7]);
 /* With timing on: [0] last iteration's k_model ms, [1] sum of k_model ms, [2] last
  * iteration ms (PSO update through best update), [3] sum of iteration ms,
- * [4] iterations summed.  reset != 0 clears the sums after reading. */
-int pasta_server_timing(pasta_server *sv, int reset, double out[5]);
+ * [4] iterations summed, [5] sum of the timed k_model launches' algorithmic FP32 flops,
+ * [6] and of their algorithmic bytes (per evaluation, as the reference's algorithm
+ * counts them: 128 W + 2 (K_rot + K_instr) W + 6 N flops, 128 W + 8 N bytes, W the
+ * synthetic span, N the observed pixels, K_rot from the evaluation's vsini).
+ * reset != 0 clears the sums after reading. */
+int pasta_server_timing(pasta_server *sv, int reset, double out[7]);
 /* Switch between graph replays (0, the default) and eager launches timed with
  * HIP events (1); both run the same kernels on the server's stream. */
 int pasta_server_set_timing(pasta_server *sv, int on);

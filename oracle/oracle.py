@@ -111,6 +111,8 @@ _sig = {
     "or_cost_batch": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, C.c_int,
                                 C.c_int, _dp]),
     "or_cost_f64": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, _dp]),
+    "or_cost_batch_f64q": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp,
+                                     C.c_int, C.c_int, _dp]),
     "or_cost_f64q": (C.c_int, [C.POINTER(_Model), C.c_void_p, C.c_void_p, _fp, _fp, _dp, _dp,
                                _dp]),
     "or_rng_u64": (C.c_uint64, [C.c_uint64, C.c_uint64]),
@@ -508,14 +510,16 @@ def rng_u64(key, ctr):
     return _L.or_rng_u64(key, ctr)
 
 
-def cost_batch(grid, disp, cont, flux, var, labels, n_threads=1):
-    """chi2 of many label vectors on n_threads threads (errors raise)."""
+def cost_batch(grid, disp, cont, flux, var, labels, n_threads=1, f64=False):
+    """chi2 of many label vectors on n_threads threads (errors raise); f64="q" for the
+    FP64 mode over the reference's f32-stored values."""
     lab = _d(labels).reshape(-1, 5)
     flux, var = _f(flux), _f(var)
     M = grid.model()
     out = np.zeros(len(lab))
-    _chk(_L.or_cost_batch(C.byref(M), disp._h, cont._h, _p(flux, C.c_float), _p(var, C.c_float),
-                          _p(lab, C.c_double), len(lab), n_threads, _p(out, C.c_double)), "cost_batch")
+    fn = _L.or_cost_batch_f64q if f64 == "q" else _L.or_cost_batch
+    _chk(fn(C.byref(M), disp._h, cont._h, _p(flux, C.c_float), _p(var, C.c_float),
+            _p(lab, C.c_double), len(lab), n_threads, _p(out, C.c_double)), "cost_batch")
     return out
 
 

@@ -1650,6 +1650,7 @@ typedef struct {
     const or_model *M; const or_disp *d; const or_cont *c;
     const float *flux, *var;
     int n_threads;
+    int f64q;  /* 1: FP64 over the reference's f32-stored values (or_cost_f64q) */
 } fit_ctx;
 
 typedef struct { fit_ctx *fc; const double *pos; double *cost; int lo, hi, rc; } job_t;
@@ -1657,8 +1658,11 @@ typedef struct { fit_ctx *fc; const double *pos; double *cost; int lo, hi, rc; }
 static void *cost_worker(void *arg) {
     job_t *j = arg;
     for (int q = j->lo; q < j->hi; q++) {
-        int rc = or_cost(j->fc->M, j->fc->d, j->fc->c, j->fc->flux, j->fc->var, j->pos + 5 * q,
-                         j->cost + q, NULL);
+        int rc = j->fc->f64q
+                     ? or_cost_f64q(j->fc->M, j->fc->d, j->fc->c, j->fc->flux, j->fc->var,
+                                    j->pos + 5 * q, j->cost + q, NULL)
+                     : or_cost(j->fc->M, j->fc->d, j->fc->c, j->fc->flux, j->fc->var,
+                               j->pos + 5 * q, j->cost + q, NULL);
         if (rc && !j->rc) j->rc = rc;
     }
     return NULL;
@@ -1690,7 +1694,7 @@ int or_fit(const or_model *M, const or_disp *d, const or_cont *c, const float *f
            const float *var, const or_bounds *b, const or_pso_settings *s, uint64_t seed,
            const double *init_pos, int n_threads, double best[5], double *best_cost,
            float *params, long *n_evals) {
-    fit_ctx fc = {M, d, c, flux, var, n_threads};
+    fit_ctx fc = {M, d, c, flux, var, n_threads, 0};
     int rc = or_pso_run(b, s, seed, init_pos, NULL, model_cost_fn, &fc, best, best_cost, NULL,
                         n_evals);
     if (rc) return rc;
@@ -1704,7 +1708,14 @@ int or_fit(const or_model *M, const or_disp *d, const or_cont *c, const float *f
  * reference's rayon par_iter over particles, src/fitting.rs:328-330). */
 int or_cost_batch(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
                   const float *var, const double *labels, int n, int n_threads, double *chi2) {
-    fit_ctx fc = {M, d, c, flux, var, n_threads};
+    fit_ctx fc = {M, d, c, flux, var, n_threads, 0};
+    return model_cost_fn(&fc, n, labels, chi2);
+}
+
+int or_cost_batch_f64q(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                       const float *var, const double *labels, int n, int n_threads,
+                       double *chi2) {
+    fit_ctx fc = {M, d, c, flux, var, n_threads, 1};
     return model_cost_fn(&fc, n, labels, chi2);
 }
 

@@ -161,6 +161,10 @@ int or_cost(const or_model *M, const or_disp *d, const or_cont *c, const float *
             const float *var, const double lab[5], double *chi2, float *params);
 int or_cost_batch(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
                   const float *var, const double *labels, int n, int n_threads, double *chi2);
+/* or_cost_batch in the FP64 mode over the reference's f32-stored values (or_cost_f64q) */
+int or_cost_batch_f64q(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
+                       const float *var, const double *labels, int n, int n_threads,
+                       double *chi2);
 int or_cost_f64(const or_model *M, const or_disp *d, const or_cont *c, const float *flux,
                 const float *var, const double lab[5], double *chi2);
 /* or_cost_f64 with the quantised resample weight; params (nullable) the f64 continuum

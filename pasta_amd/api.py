@@ -1189,8 +1189,8 @@ class FitServer:
 
     def stats(self):
         """(evaluations, requests finished, iterations, HBM bytes in use, device-busy
-        microseconds, batch rebuilds)."""
-        out = (C.c_int64 * 6)()
+        microseconds, batch rebuilds, host microseconds in rebuilds)."""
+        out = (C.c_int64 * 7)()
         _raise(lib.pasta_server_stats(self._h, out), "stats")
         return tuple(out)
 

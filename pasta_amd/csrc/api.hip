@@ -21,6 +21,11 @@ int fail(int code, const std::string& msg) {
 }
 
 const char* pasta_last_error(void) { return g_err.c_str(); }
+
+std::shared_mutex& capture_mutex() {
+    static std::shared_mutex m;
+    return m;
+}
 int pasta_version(void) { return 100; }
 
 int pasta_device_mem_info(uint64_t* free_b, uint64_t* total_b) {
@@ -609,20 +614,36 @@ int pasta_cont_build(const pasta_cont* c, const float* params, int n_params, flo
 // ---------------------------------------------------------------------------
 // star
 // ---------------------------------------------------------------------------
+// Each host thread uploads stars on a stream of its own (default priority, non-blocking):
+// a copy then waits only for itself, not for the batches running on other streams.
+namespace {
+struct UploadStream {
+    hipStream_t s = nullptr;
+    ~UploadStream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+thread_local UploadStream t_upload;
+}  // namespace
+
 template <typename T>
 static int dev_copy(pasta_star* s, const T* h, size_t n, const T** out) {
     if (n == 0) { *out = nullptr; return PASTA_OK; }
     void* p = nullptr;
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    if (!t_upload.s) HIPCHK(hipStreamCreateWithFlags(&t_upload.s, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&p, n * sizeof(T)));
     s->allocs.push_back(p);
     s->resident_bytes += (int64_t)(n * sizeof(T));
-    HIPCHK(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, t_upload.s));
+    HIPCHK(hipStreamSynchronize(t_upload.s));
     *out = (const T*)p;
     return PASTA_OK;
 }
 
 pasta_star::~pasta_star() {
     cache.reset();
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
     for (void* p : allocs) (void)hipFree(p);
 }
 
@@ -644,8 +665,10 @@ int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, const fl
         D.wlN = d->wl.back();
         double f0 = d->fidx.front(), fN = d->fidx.back();
         D.n_tiles = (int)std::floor((fN - f0 + 1.0) / TILE) + 2;
+        s->span = (int)(std::floor(fN) - std::floor(f0)) + 2;
     } else {
         D.n_tiles = (g->W + TILE - 1) / TILE;
+        s->span = g->W;
     }
     if (flux) CHK(dev_copy(s.get(), flux, n, &D.flux));
     if (var) {  // chi2 terms are r * r * (1 / var): one multiply per evaluation and pixel
@@ -741,7 +764,7 @@ int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
         ne += counts[s];
     }
     B.n_evals = ne;
-    B.h_evals.assign(ne, DevEval{});
+    B.h_evals.assign(ne, EvalInit{});
     int64_t mo = 0, mp = 0, cp = 0, pf = 0;
     std::vector<int4> bmom, bchi2;
     std::vector<int2> solve;
@@ -752,7 +775,7 @@ int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
         const int outlen = mode == 0 ? D.N : g->W;
         for (int q = 0; q < counts[s]; q++) {
             int e = B.first_eval[s] + q;
-            DevEval& ev = B.h_evals[e];
+            EvalInit& ev = B.h_evals[e];
             ev.star = s;
             ev.active = 1;
             if (D.cont_kind == CONT_LINEAR || D.cont_kind == CONT_CHUNK)
@@ -810,7 +833,10 @@ int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
     std::vector<DevStar> ds(B.n_stars);
     for (int s = 0; s < B.n_stars; s++) ds[s] = stars[s]->dev;
     CHK(B.d_stars.upload(ds));
-    CHK(B.d_evals.upload(B.h_evals));
+    if (!B.stream) HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+    CHK(B.d_init.upload(B.h_evals));
+    CHK(B.d_evals.alloc((size_t)std::max(ne, 1)));
+    HIPCHK(launch_eval_init(B.d_init.p, ne, B.d_evals.p, B.stream));
     CHK(B.d_labels.alloc((size_t)ne * 5));
     CHK(B.d_rot.alloc((size_t)ne * ROT_STRIDE));
     CHK(B.d_model.alloc((size_t)std::max<int64_t>(mo, 1)));
@@ -829,7 +855,7 @@ int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
     CHK(B.d_bmom.upload(bmom));
     CHK(B.d_solve.upload(solve));
     CHK(B.d_bchi2.upload(bchi2));
-    if (!B.stream) HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamSynchronize(B.stream));  // the records are in place for any stream
     return PASTA_OK;
 }
 
@@ -986,6 +1012,17 @@ int pasta_interpolate_and_convolve(pasta_grid* g, const pasta_disp* d, const dou
         for (int k = 0; k < 4; k++) l5[5 * i + k] = labels[4 * i + k];
     pasta_disp dd = *d;  // synthetic-grid output ignores the observed sampling
     return model_only(g, &dd, l5.data(), P, out, status, 1);
+}
+
+// taps of build_rotation_kernel (src/convolve_rv.rs:438-449); 0 for the pass-through of
+// vsini == 0 (:509-511)
+static int rotation_taps(double vsini, double step) {
+    if (!(vsini > 0.0)) return 0;
+    const double eps = 0.6, dvelo = LN_10 * step, vrot = vsini / C_KMS;
+    const double a = 4.0 * (eps - 1.0) / (M_PI * eps);
+    const double v = std::ceil(2.0 * vrot / dvelo * std::sqrt(1.0 + a * a));
+    long long nme = (v >= 0.0 && v < 1e9) ? (long long)v + 2 : 2;
+    return (int)(nme + 1 - nme % 2);
 }
 
 int pasta_vsini_kernel(const pasta_grid* g, double vsini, float* out, int cap, int* len) {
@@ -1235,9 +1272,14 @@ static int pso_iteration_launches(PsoBatch& PB) {
 static int pso_capture(PsoBatch& PB) {
     Batch& B = PB.B;
     hipGraph_t gr;
-    HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-    int rc = pso_iteration_launches(PB);
-    hipError_t e = hipStreamEndCapture(B.stream, &gr);
+    int rc;
+    hipError_t e;
+    {
+        std::unique_lock<std::shared_mutex> lk(capture_mutex());
+        HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        rc = pso_iteration_launches(PB);
+        e = hipStreamEndCapture(B.stream, &gr);
+    }
     if (rc) return rc;
     HIPCHK(e);
     HIPCHK(hipGraphInstantiate(&B.graph, gr, nullptr, nullptr, 0));
@@ -1536,9 +1578,12 @@ struct pasta_server {
     bool dirty = false;                // running differs from batch_reqs
     int64_t evals_done = 0, finished = 0, iters_run = 0, rebuilds = 0;
     double busy_ms = 0;                // device time of first evaluations and iterations
+    double rebuild_ms = 0;             // host time in batch rebuilds (build, remap, capture)
     hipEvent_t ev_busy[4] = {nullptr, nullptr, nullptr, nullptr};
     bool first_eval_timed = false;     // ev_busy[2..3] bracket a first evaluation in flight
     double t_model_last = 0, t_model_sum = 0, t_iter_last = 0, t_iter_sum = 0, t_n = 0;
+    double t_flops_sum = 0, t_bytes_sum = 0;  // algorithmic work of the timed k_model launches
+    std::vector<double> lab_host;
     std::mutex mu;
     ~pasta_server() {
         pb[0].reset();
@@ -1636,10 +1681,7 @@ static int server_rebuild(pasta_server* sv) {
             if (!old || r->slot < 0 || r->slot >= old->B.n_stars)
                 return fail(PASTA_ERR_STATE, "server lost a running star's state");
             src_star[s] = r->slot;
-            for (int q = 0; q < P; q++) {
-                src_eval[f0 + q] = old->B.first_eval[r->slot] + q;
-                B.h_evals[f0 + q].active = 0;  // not part of the first-evaluation pass
-            }
+            for (int q = 0; q < P; q++) src_eval[f0 + q] = old->B.first_eval[r->slot] + q;
             any_started = true;
         } else {
             any_new = true;
@@ -1650,10 +1692,10 @@ static int server_rebuild(pasta_server* sv) {
         CHK(sv->d_src_eval.upload(src_eval));
         CHK(sv->d_src_star.upload(src_star));
         const Batch& O = old->B;
-        HIPCHK(launch_pso_remap(sv->d_src_eval.p, B.n_evals, sv->d_src_star.p, B.n_stars, O.d_pso.p,
-                                B.d_pso.p, O.d_labels.p, B.d_labels.p, O.d_vel.p, B.d_vel.p,
-                                O.d_bpos.p, B.d_bpos.p, O.d_bcost.p, B.d_bcost.p, B.stream));
-        CHK(B.d_evals.upload(B.h_evals));
+        HIPCHK(launch_pso_remap(sv->d_src_eval.p, B.n_evals, B.d_evals.p, sv->d_src_star.p,
+                                B.n_stars, O.d_pso.p, B.d_pso.p, O.d_labels.p, B.d_labels.p,
+                                O.d_vel.p, B.d_vel.p, O.d_bpos.p, B.d_bpos.p, O.d_bcost.p,
+                                B.d_bcost.p, B.stream));
     }
     if (any_new) {
         HIPCHK(hipEventRecord(sv->ev_busy[2], B.stream));
@@ -1728,6 +1770,33 @@ static int server_retire(pasta_server* sv) {
     return PASTA_OK;
 }
 
+// Algorithmic work of one k_model launch over the batch's current labels, counted as the
+// reference's algorithm does it for each evaluation (synthetic span W, observed pixels N):
+// interpolation 64 products and sums per pixel (128 W flops, 64 u16 reads = 128 W bytes),
+// the rotation and instrumental "same" convolutions with their own tap counts
+// (2 (K_rot + K_instr) W flops, the rotation taps from the evaluation's vsini), the
+// resample (6 N flops, an 8-byte float index read per observed pixel).
+static int server_algorithmic_work(pasta_server* sv, Batch& B, double* flops, double* bytes) {
+    sv->lab_host.resize((size_t)B.n_evals * 5);
+    HIPCHK(hipMemcpy(sv->lab_host.data(), B.d_labels.p, sizeof(double) * sv->lab_host.size(),
+                     hipMemcpyDeviceToHost));
+    double f = 0.0, b = 0.0;
+    for (int s = 0; s < B.n_stars; s++) {
+        const pasta_star* st = B.stars[s];
+        const double W = st->span, N = st->N;
+        const int kg = st->dev.disp_kind == DISP_NONE ? 0 : st->dev.Kd;
+        for (int q = 0; q < B.count[s]; q++) {
+            const double vsini = sv->lab_host[5 * (size_t)(B.first_eval[s] + q) + 3];
+            const int kr = rotation_taps(vsini, sv->g->step);
+            f += 128.0 * W + 2.0 * (double)(kr + kg) * W + 6.0 * N;
+            b += 128.0 * W + 8.0 * N;
+        }
+    }
+    *flops = f;
+    *bytes = b;
+    return PASTA_OK;
+}
+
 int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
     if (!sv) return fail(PASTA_ERR_ARG, "null server");
     std::lock_guard<std::mutex> lk(sv->mu);
@@ -1744,7 +1813,10 @@ int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
         sv->dirty = true;
     }
     if (sv->dirty) {
+        const auto t0 = std::chrono::steady_clock::now();
         CHK(server_rebuild(sv));
+        sv->rebuild_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         sv->dirty = false;
     }
     if (sv->active) {
@@ -1767,6 +1839,10 @@ int pasta_server_step(pasta_server* sv, int* active, int* waiting) {
                 sv->t_iter_last = t;
                 sv->t_iter_sum += t;
                 sv->t_n += 1;
+                double fl = 0.0, by = 0.0;
+                CHK(server_algorithmic_work(sv, B, &fl, &by));
+                sv->t_flops_sum += fl;
+                sv->t_bytes_sum += by;
             }
         }
         HIPCHK(hipEventRecord(sv->ev_busy[1], B.stream));
@@ -1831,8 +1907,9 @@ int pasta_server_cancel(pasta_server* sv, uint64_t id) {
     return fail(PASTA_ERR_STATE, "request already finished");
 }
 
-int pasta_server_stats(const pasta_server* sv, int64_t out[6]) {
+int pasta_server_stats(const pasta_server* sv, int64_t out[7]) {
     if (!sv || !out) return fail(PASTA_ERR_ARG, "null argument");
+    out[6] = (int64_t)(sv->rebuild_ms * 1e3);
     out[0] = sv->evals_done;
     out[1] = sv->finished;
     out[2] = sv->iters_run;
@@ -1842,7 +1919,7 @@ int pasta_server_stats(const pasta_server* sv, int64_t out[6]) {
     return PASTA_OK;
 }
 
-int pasta_server_timing(pasta_server* sv, int reset, double out[5]) {
+int pasta_server_timing(pasta_server* sv, int reset, double out[7]) {
     if (!sv || !out) return fail(PASTA_ERR_ARG, "null argument");
     std::lock_guard<std::mutex> lk(sv->mu);
     out[0] = sv->t_model_last;
@@ -1850,7 +1927,9 @@ int pasta_server_timing(pasta_server* sv, int reset, double out[5]) {
     out[2] = sv->t_iter_last;
     out[3] = sv->t_iter_sum;
     out[4] = sv->t_n;
-    if (reset) sv->t_model_sum = sv->t_iter_sum = sv->t_n = 0;
+    out[5] = sv->t_flops_sum;
+    out[6] = sv->t_bytes_sum;
+    if (reset) sv->t_model_sum = sv->t_iter_sum = sv->t_n = sv->t_flops_sum = sv->t_bytes_sum = 0;
     return PASTA_OK;
 }
 

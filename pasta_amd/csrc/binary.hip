@@ -223,7 +223,7 @@ int pasta_binary_model(pasta_binary* b, const double* labels, int P, float* out,
     CHK(binary_launch(b, labels, P, false));
     std::vector<int> st(P);
     CHK(binary_status(b, P, st.data()));
-    const std::vector<DevEval>& he = b->Bo->h_evals;
+    const std::vector<EvalInit>& he = b->Bo->h_evals;
     int bad = 0;
     for (int q = 0; q < P; q++) {
         if (status) status[q] = st[q];

@@ -133,6 +133,14 @@ struct DevEval {
     const uint16_t* rowp[64];  // its grid row (row 0 where absent)
 };
 
+// What the host sets in an evaluation record (the rest is written by k_prep); expanded
+// into DevEval on the device by k_eval_init, so a batch uploads 40 bytes per evaluation.
+struct EvalInit {
+    int star;
+    int active;
+    long long m_off, mp_off, cp_off, pf_off;
+};
+
 // Batched PSO state per star (device).
 struct DevPso {
     Bounds b;

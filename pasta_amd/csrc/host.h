@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -28,8 +29,9 @@ hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, 
 hipError_t launch_pso_best(DevPso* pso, int n_stars, const DevEval* evals, const double* pos,
                            double* bpos, double* bcost, const double* cost, double* trace,
                            hipStream_t st);
-hipError_t launch_pso_remap(const int* src_eval, int n_evals, const int* src_star, int n_stars,
-                            const DevPso* pso_old, DevPso* pso_new, const double* pos_old,
+hipError_t launch_eval_init(const EvalInit* in, int n, DevEval* out, hipStream_t st);
+hipError_t launch_pso_remap(const int* src_eval, int n_evals, DevEval* evals, const int* src_star,
+                            int n_stars, const DevPso* pso_old, DevPso* pso_new, const double* pos_old,
                             double* pos_new, const double* vel_old, double* vel_new,
                             const double* bpos_old, double* bpos_new, const double* bcost_old,
                             double* bcost_new, hipStream_t st);
@@ -42,6 +44,11 @@ using namespace pasta;
 
 // sets the thread-local message of pasta_last_error and returns code
 int fail(int code, const std::string& msg);
+
+// Stream capture (the batched and served PSO iteration graphs) holds this exclusively;
+// device allocations, frees and synchronous copies hold it shared, so that another
+// thread's memory call never lands inside a capture (which HIP then invalidates).
+std::shared_mutex& capture_mutex();
 
 #define HIPCHK(x)                                                                         \
     do {                                                                                  \
@@ -71,7 +78,10 @@ struct DBuf {
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            std::shared_lock<std::shared_mutex> lk(capture_mutex());
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -79,12 +89,14 @@ struct DBuf {
         if (count <= n && p) return PASTA_OK;
         release();
         size_t c = count ? count : 1;
+        std::shared_lock<std::shared_mutex> lk(capture_mutex());
         HIPCHK(hipMalloc((void**)&p, c * sizeof(T)));
         n = c;
         return PASTA_OK;
     }
     int upload(const T* h, size_t count) {
         CHK(alloc(count));
+        std::shared_lock<std::shared_mutex> lk(capture_mutex());
         if (count) HIPCHK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
         return PASTA_OK;
     }
@@ -155,6 +167,7 @@ struct pasta_star {
     DevStar dev{};
     std::vector<void*> allocs;
     int64_t resident_bytes = 0;  // device memory the star's own buffers hold
+    int span = 0;                // synthetic pixels the observed range spans
     std::unique_ptr<Batch> cache;
     int cache_P = -1;
     std::mutex mu;
@@ -168,7 +181,8 @@ struct Batch {
     pasta_grid* grid = nullptr;
     std::vector<pasta_star*> stars;
     std::vector<int> first_eval, count;
-    std::vector<DevEval> h_evals;
+    std::vector<EvalInit> h_evals;  // host part of the evaluation records
+    DBuf<EvalInit> d_init;
     int64_t model_len = 0, mpart_len = 0, cpart_len = 0, pf_len = 0;
     DBuf<DevStar> d_stars;
     DBuf<DevEval> d_evals;

@@ -1059,13 +1059,15 @@ __device__ __forceinline__ float dot_na(const float (&a)[PM], const float* b, in
 // (continuum_fitting.rs:490-492): the f32 quotients f/m, added one after another in pixel
 // order (`target.iter().sum::<f32>()`, a sequential fold), divided by n in f32.  A
 // workgroup covers the E <= 16 evaluations of one star in block descriptor (first
-// evaluation, count, 0, 0).  Waves 1-3 form the quotients of 64-pixel chunks (IEEE
-// divisions, coalesced reads of the flux and of each model row) into a double-buffered
-// LDS tile, one row per evaluation; lane e < E of wave 0 adds row e's 64 values in order
-// while the next chunk is being formed.  Past the last pixel the tile holds zeros: adding
-// +0 leaves a sum unchanged (but for the sign of a zero sum).
-constexpr int FOLD_CH = 64;       // pixels per chunk
-constexpr int FOLD_ROW = 68;      // LDS row stride: 16-byte rows, conflict-free for 16 lanes
+// evaluation, count, 0, 0).  Waves 1-3 form the quotients of 256-pixel chunks (IEEE
+// divisions; each thread issues all its flux and model reads of a chunk before its first
+// division, so ~22 reads are in flight per thread) into a double-buffered LDS tile, one row
+// per evaluation; lane e < E of wave 0 adds row e's 256 values in order while the next
+// chunk is being formed.  Past the last pixel the tile holds zeros: adding +0 leaves a sum
+// unchanged (but for the sign of a zero sum).
+constexpr int FOLD_CH = 256;      // pixels per chunk
+constexpr int FOLD_ROW = 260;     // LDS row stride: 16-byte rows, 260 = 4 mod 64 banks, so
+                                  // 16 lanes' 16-byte reads cover the 64 banks once
 template <int E>
 __global__ __launch_bounds__(NT) void k_const_fold(const DevStar* __restrict__ stars,
                                                    const DevEval* __restrict__ evals,
@@ -1073,28 +1075,42 @@ __global__ __launch_bounds__(NT) void k_const_fold(const DevStar* __restrict__ s
                                                    const float* __restrict__ model,
                                                    float* __restrict__ pf) {
     static_assert(E <= 16, "one lane of wave 0 per evaluation, <= 16 for conflict-free rows");
+    constexpr int NP = NT - 64;                       // producer threads
+    constexpr int PER = (E * FOLD_CH + NP - 1) / NP;  // quotients per producer and chunk
     __shared__ __align__(16) float q[2][E][FOLD_ROW];
+    __shared__ long long s_off[E];
+    __shared__ int s_ok[E];
     const int4 bd = blocks[blockIdx.x];
     const DevStar& S = stars[evals[bd.x].star];
     const int N = S.N, cnt = bd.y;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n_ch = (N + FOLD_CH - 1) / FOLD_CH;
-    // per producer thread: its evaluations' model rows (null when failed or absent)
-    const float* rows[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) {
+    if (threadIdx.x < E) {
+        const int e = threadIdx.x;
         const bool ok = e < cnt && evals[bd.x + e].status == ST_OK;
-        rows[e] = ok ? model + evals[bd.x + e].m_off : nullptr;
+        s_ok[e] = ok;
+        s_off[e] = ok ? evals[bd.x + e].m_off : 0;
     }
+    __syncthreads();
+    const float* __restrict__ flux = S.flux;
     auto produce = [&](int c, int buf) {
-        const int t = threadIdx.x - 64;  // 0 .. 191
-        for (int k = t; k < E * FOLD_CH; k += NT - 64) {
-            const int e = k / FOLD_CH, i = k - e * FOLD_CH, px = c * FOLD_CH + i;
-            const float* m = nullptr;
+        const int t = threadIdx.x - 64;
+        float fv[PER], mv[PER];
 #pragma unroll
-            for (int j = 0; j < E; j++)
-                if (j == e) m = rows[j];
-            q[buf][e][i] = (m && px < N) ? S.flux[px] / m[px] : 0.0f;
+        for (int r = 0; r < PER; r++) {
+            const int k = t + r * NP;
+            const int e = k / FOLD_CH, i = k - e * FOLD_CH, px = c * FOLD_CH + i;
+            const bool in = k < E * FOLD_CH && px < N && s_ok[e < E ? e : 0];
+            fv[r] = in ? flux[px] : 0.0f;
+            mv[r] = in ? model[s_off[e] + px] : 1.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int k = t + r * NP;
+            if (k < E * FOLD_CH) {
+                const int e = k / FOLD_CH, i = k - e * FOLD_CH;
+                q[buf][e][i] = fv[r] / mv[r];  // 0 / 1 where there is no pixel
+            }
         }
     };
     if (wv > 0) produce(0, 0);
@@ -1105,7 +1121,7 @@ __global__ __launch_bounds__(NT) void k_const_fold(const DevStar* __restrict__ s
         if (wv == 0) {
             if (lane < cnt) {
                 const float4* r = (const float4*)&q[buf][lane][0];
-#pragma unroll
+#pragma unroll 16
                 for (int i = 0; i < FOLD_CH / 4; i++) {
                     const float4 v = r[i];
                     s = s + v.x;
@@ -1119,10 +1135,7 @@ __global__ __launch_bounds__(NT) void k_const_fold(const DevStar* __restrict__ s
         }
         __syncthreads();
     }
-    if (wv == 0 && lane < cnt) {
-        const DevEval& ev = evals[bd.x + lane];
-        if (ev.status == ST_OK) pf[ev.pf_off] = s / (float)N;  // sum / n as FluxFloat
-    }
+    if (wv == 0 && lane < cnt && s_ok[lane]) pf[evals[bd.x + lane].pf_off] = s / (float)N;
 }
 
 // Continuum moments Aᵀ(f/m) of the chunk and linear continua on f64 matrix cores
@@ -1554,13 +1567,35 @@ __global__ void k_pso_init(DevPso* __restrict__ pso, int n_stars, const DevEval*
     if (ps.max_iters < 1) ps.done = 1;
 }
 
+// Evaluation records from their host part: one workgroup per record zeroes it and sets the
+// star, the active flag and the buffer offsets.
+__global__ __launch_bounds__(64) void k_eval_init(const EvalInit* __restrict__ in, int n,
+                                                  DevEval* __restrict__ out) {
+    const int e = blockIdx.x;
+    if (e >= n) return;
+    int* w = (int*)(out + e);
+    for (int i = threadIdx.x; i < (int)(sizeof(DevEval) / sizeof(int)); i += blockDim.x) w[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const EvalInit v = in[e];
+        DevEval& ev = out[e];
+        ev.star = v.star;
+        ev.active = v.active;
+        ev.m_off = v.m_off;
+        ev.mp_off = v.mp_off;
+        ev.cp_off = v.cp_off;
+        ev.pf_off = v.pf_off;
+    }
+}
+
 // Continuous batching: the PSO state of stars that stay in flight moves from the previous
 // batch into the rebuilt one on the device.  Thread per evaluation of the new batch:
 // src_eval[e] is its particle's evaluation in the old batch (-1: a new star, left as
 // initialised); thread s < n_stars also moves the star record (src_star[s] >= 0),
 // keeping the new batch's first_eval.
 __global__ void k_pso_remap(const int* __restrict__ src_eval, int n_evals,
-                            const int* __restrict__ src_star, int n_stars,
+                            DevEval* __restrict__ evals, const int* __restrict__ src_star,
+                            int n_stars,
                             const DevPso* __restrict__ pso_old, DevPso* __restrict__ pso_new,
                             const double* __restrict__ pos_old, double* __restrict__ pos_new,
                             const double* __restrict__ vel_old, double* __restrict__ vel_new,
@@ -1570,6 +1605,7 @@ __global__ void k_pso_remap(const int* __restrict__ src_eval, int n_evals,
     if (t < n_evals) {
         const int o = src_eval[t];
         if (o >= 0) {
+            evals[t].active = 0;  // resumed: not part of the first-evaluation pass
             for (int d = 0; d < 5; d++) {
                 pos_new[5 * (size_t)t + d] = pos_old[5 * (size_t)o + d];
                 vel_new[5 * (size_t)t + d] = vel_old[5 * (size_t)o + d];
@@ -2088,16 +2124,22 @@ hipError_t launch_pso_best(DevPso* pso, int n_stars, const DevEval* evals, const
     return hipGetLastError();
 }
 
-hipError_t launch_pso_remap(const int* src_eval, int n_evals, const int* src_star, int n_stars,
-                            const DevPso* pso_old, DevPso* pso_new, const double* pos_old,
-                            double* pos_new, const double* vel_old, double* vel_new,
-                            const double* bpos_old, double* bpos_new, const double* bcost_old,
-                            double* bcost_new, hipStream_t st) {
+hipError_t launch_eval_init(const EvalInit* in, int n, DevEval* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_eval_init, dim3(n), dim3(64), 0, st, in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pso_remap(const int* src_eval, int n_evals, DevEval* evals, const int* src_star,
+                            int n_stars, const DevPso* pso_old, DevPso* pso_new,
+                            const double* pos_old, double* pos_new, const double* vel_old,
+                            double* vel_new, const double* bpos_old, double* bpos_new,
+                            const double* bcost_old, double* bcost_new, hipStream_t st) {
     const int n = n_evals > n_stars ? n_evals : n_stars;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_pso_remap, dim3((n + 255) / 256), dim3(256), 0, st, src_eval, n_evals,
-                       src_star, n_stars, pso_old, pso_new, pos_old, pos_new, vel_old, vel_new,
-                       bpos_old, bpos_new, bcost_old, bcost_new);
+                       evals, src_star, n_stars, pso_old, pso_new, pos_old, pos_new, vel_old,
+                       vel_new, bpos_old, bpos_new, bcost_old, bcost_new);
     return hipGetLastError();
 }
 

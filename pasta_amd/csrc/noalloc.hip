@@ -173,6 +173,8 @@ int pasta_eval_chi2_noalloc(pasta_star* s, const double* labels, int P, double* 
                         std::to_string(PASTA_NOALLOC_SCRATCH_BYTES) +
                         " the allocation-free path holds (fewer particles or pixels per call)");
     std::lock_guard<std::mutex> lk(g_scratch_mu);
+    // null-stream work must not land inside another thread's stream capture
+    std::shared_lock<std::shared_mutex> cap(capture_mutex());
     void* base = nullptr;
     CHK(scratch_address(&base));
     Carve c{(char*)base};
