@@ -86,7 +86,7 @@ def settings(pasta, P, iters):
     return st
 
 
-def run_batched(pasta, _lib, interp, stars, meta, P, warmup, steps, timing_every=10):
+def run_batched(pasta, _lib, interp, stars, meta, P, warmup, steps, timing_every=5):
     """Every star runs one PSO fit of max(200, warmup + steps + 3) iterations in the server.
     The timed steps are graph replays, except every timing_every-th, which runs the same
     launches eagerly with HIP events around k_model and the whole iteration."""
