@@ -205,6 +205,14 @@ typedef struct pasta_fit_result {
 int pasta_fit(pasta_star *s, const pasta_fit_settings *st, uint64_t seed, const double *init_pos,
               pasta_fit_result *out, double *trace);
 
+/* pasta_fit with the reference's PSOobserver (src/fitting.rs:217-256; trace_directory of
+ * Python fit, src/lib.rs:1476-1507,838-848): population receives, for every iteration
+ * it = 0 .. max_iters-1, the swarm after that iteration as P x 6 doubles (position,
+ * cost) in particle order, the content of the reference's it_{it}.json. */
+int pasta_fit_observed(pasta_star *s, const pasta_fit_settings *st, uint64_t seed,
+                       const double *init_pos, pasta_fit_result *out, double *trace,
+                       double *population);
+
 /* The batched path (replaces fit_bulk, src/lib.rs:1511-1540): every particle of
  * every star in one device-driven, hipGraph-captured PSO iteration. */
 int pasta_fit_bulk(pasta_star *const *stars, int n_stars, const pasta_fit_settings *st,

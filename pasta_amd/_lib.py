@@ -174,6 +174,7 @@ SIGNATURES = {
     "pasta_interpolate_and_convolve": [_P, _P, _dp, C.c_int, _fp, _ip],
     "pasta_vsini_kernel": [_P, C.c_double, _fp, C.c_int, _ip],
     "pasta_fit": [_P, C.POINTER(FitSettings), C.c_uint64, _dp, C.POINTER(FitResult), _dp],
+    "pasta_fit_observed": [_P, C.POINTER(FitSettings), C.c_uint64, _dp, C.POINTER(FitResult), _dp, _dp],
     "pasta_fit_bulk": [_PP, C.c_int, C.POINTER(FitSettings), _u64p, C.POINTER(FitResult), _dp],
     "pasta_compute_uncertainty": [_P, C.c_double, _dp, _dp, _dp, _ip],
     "pasta_eval_chi2_multi": [_PP, C.c_int, _dp, _dp, _ip],

@@ -800,12 +800,24 @@ class InMemSingleFitter:
         keep = None if init_pos is None else f64(init_pos)
         if keep is not None:
             ip = ptr(keep, C.c_double)
-        rc = lib.pasta_fit(star._h, C.byref(s), int(seed), ip, C.byref(res), ptr(trace, C.c_double))
+        if trace_directory is None:
+            rc = lib.pasta_fit(star._h, C.byref(s), int(seed), ip, C.byref(res),
+                               ptr(trace, C.c_double))
+        else:
+            pop = np.zeros((self.settings.max_iters, self.settings.num_particles, 6))
+            rc = lib.pasta_fit_observed(star._h, C.byref(s), int(seed), ip, C.byref(res),
+                                        ptr(trace, C.c_double), ptr(pop, C.c_double))
         _raise(rc, "fit")
         out = OptimizationResult._from_c(res)
+        # the reference's observers (src/fitting.rs:217-296) see the state after each PSO
+        # iteration (argmin's observe_iter, iteration numbers from 0), not the initialisation
         if best_particle_file is not None:
             with open(best_particle_file, "w") as f:
-                json.dump([{"position": t[1:].tolist(), "cost": t[0]} for t in trace], f)
+                json.dump([{"position": t[1:].tolist(), "cost": t[0]} for t in trace[1:]], f)
+        if trace_directory is not None:
+            for it in range(self.settings.max_iters):
+                with open(os.path.join(trace_directory, f"it_{it}.json"), "w") as f:
+                    json.dump([{"position": p[:5].tolist(), "cost": p[5]} for p in pop[it]], f)
         out._trace = trace
         return out
 
@@ -1030,6 +1042,9 @@ class BinaryFitter:
         """PSO fit of a binary (src/lib.rs:1637-1665); `seed` selects the random stream."""
         if trace_directory is not None and best_particle_file is not None:
             raise ValueError("Cannot specify both trace_directory and best_particle_file")
+        if trace_directory is not None:
+            raise NotImplementedError("the per-iteration population files (trace_directory) are "
+                                      "written by single-star fits only")
         b = self._binary(interpolator, continuum_interpolator, observed_flux, observed_var)
         s = _binary_settings_c(self.settings, self.vsini_range, self.rv_range, constraints)
         res = _lib.BinaryResult()
@@ -1042,7 +1057,7 @@ class BinaryFitter:
         out = BinaryOptimizationResult._from_c(res)
         if best_particle_file is not None:
             with open(best_particle_file, "w") as f:
-                json.dump([{"position": t[1:].tolist(), "cost": t[0]} for t in trace], f)
+                json.dump([{"position": t[1:].tolist(), "cost": t[0]} for t in trace[1:]], f)
         out._trace = trace
         return out
 
@@ -1073,6 +1088,9 @@ class BinaryRVFitter:
             trace_directory=None, best_particle_file=None, constraints=None, seed=0, init_pos=None):
         if trace_directory is not None and best_particle_file is not None:
             raise ValueError("Cannot specify both trace_directory and best_particle_file")
+        if trace_directory is not None:
+            raise NotImplementedError("the per-iteration population files (trace_directory) are "
+                                      "written by single-star fits only")
         r = self._rv(model1, model2, continuum1, continuum2, light_ratio, observed_flux, observed_var)
         s = _binary_settings_c(self.settings, (0.0, 0.0), self.rv_range, constraints, 2)
         res = _lib.RvResult()
@@ -1084,7 +1102,7 @@ class BinaryRVFitter:
         out = RVOptimizationResult(res.rv1, res.rv2, res.chi2, res.iterations, res.time_s)
         if best_particle_file is not None:
             with open(best_particle_file, "w") as f:
-                json.dump([{"position": t[1:].tolist(), "cost": t[0]} for t in trace], f)
+                json.dump([{"position": t[1:].tolist(), "cost": t[0]} for t in trace[1:]], f)
         out._trace = trace
         return out
 
@@ -1115,6 +1133,9 @@ class BinaryTimeseriesKnownRVFitter:
             seed=0, init_pos=None):
         if trace_directory is not None and best_particle_file is not None:
             raise ValueError("Cannot specify both trace_directory and best_particle_file")
+        if trace_directory is not None:
+            raise NotImplementedError("the per-iteration population files (trace_directory) are "
+                                      "written by single-star fits only")
         t = self._ts(interpolator, continuum_interpolator, observed_fluxes, observed_vars, rvs)
         s = _binary_settings_c(self.settings, self.vsini_range, (0.0, 0.0), constraints, 9)
         res = _lib.TsResult()

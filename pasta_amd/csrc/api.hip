@@ -1126,8 +1126,9 @@ static int final_params(pasta_star* s, const double best[5], pasta_fit_result* o
 // ---------------------------------------------------------------------------
 // plain fit: host PSO, one device round trip per iteration
 // ---------------------------------------------------------------------------
-int pasta_fit(pasta_star* s, const pasta_fit_settings* st, uint64_t seed, const double* init_pos,
-              pasta_fit_result* out, double* trace) {
+static int fit_impl(pasta_star* s, const pasta_fit_settings* st, uint64_t seed,
+                    const double* init_pos, pasta_fit_result* out, double* trace,
+                    double* population) {
     if (!s || !out) return fail(PASTA_ERR_ARG, "null argument");
     if (s->cont_kind < 0) return fail(PASTA_ERR_ARG, "star has no continuum model");
     CHK(settings_check(st));
@@ -1194,6 +1195,12 @@ int pasta_fit(pasta_star* s, const pasta_fit_settings* st, uint64_t seed, const 
             }
         }
         if (trace) { trace[6 * it] = gc; std::memcpy(trace + 6 * it + 1, gb, sizeof(gb)); }
+        if (population)  // the population after iteration it: (position, cost) per particle
+            for (int q = 0; q < P; q++) {
+                double* o = population + ((size_t)(it - 1) * P + q) * 6;
+                std::memcpy(o, &pos[5 * q], 40);
+                o[5] = cost[q];
+            }
     }
     std::memcpy(out->label, gb, sizeof(gb));
     out->chi2 = gc;
@@ -1201,6 +1208,18 @@ int pasta_fit(pasta_star* s, const pasta_fit_settings* st, uint64_t seed, const 
     CHK(final_params(s, gb, out));
     out->time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     return PASTA_OK;
+}
+
+int pasta_fit(pasta_star* s, const pasta_fit_settings* st, uint64_t seed, const double* init_pos,
+              pasta_fit_result* out, double* trace) {
+    return fit_impl(s, st, seed, init_pos, out, trace, nullptr);
+}
+
+int pasta_fit_observed(pasta_star* s, const pasta_fit_settings* st, uint64_t seed,
+                       const double* init_pos, pasta_fit_result* out, double* trace,
+                       double* population) {
+    if (!population) return fail(PASTA_ERR_ARG, "null population buffer");
+    return fit_impl(s, st, seed, init_pos, out, trace, population);
 }
 
 // ---------------------------------------------------------------------------

@@ -227,3 +227,34 @@ def test_server_memory_flat_over_many_requests(small, gpu_small):
         assert f1 >= f0 - (4 << 20), (f0, f1)
     finally:
         srv.close()
+
+
+def test_fit_observers_write_reference_files(small, gpu_small, tmp_path):
+    """trace_directory writes it_{k}.json per PSO iteration (the swarm after that iteration:
+    position and cost of every particle, src/fitting.rs:217-256) and best_particle_file the
+    best particle after each iteration (:260-296); the fit itself is unchanged."""
+    import json
+    pasta, wlg, interp = gpu_small
+    fitter, d, c = _fitter(small, pasta, wlg, interp, P=12, iters=6)
+    plain = fitter.fit(interp, small.flux, small.var, seed=5)
+    res = fitter.fit(interp, small.flux, small.var, seed=5, trace_directory=str(tmp_path))
+    assert res.label.as_list() == plain.label.as_list() and res.chi2 == plain.chi2
+    files = sorted(tmp_path.glob("it_*.json"), key=lambda p: int(p.stem[3:]))
+    assert [p.name for p in files] == [f"it_{k}.json" for k in range(6)]
+    star = pasta.Star(interp, d, c, small.flux, small.var)
+    for k, p in enumerate(files):
+        pop = json.load(open(p))
+        assert len(pop) == 12
+        pos = np.array([q["position"] for q in pop])
+        cost = np.array([q["cost"] for q in pop])
+        chi, st, _ = star.chi2(pos)
+        np.testing.assert_array_equal(chi, cost)  # the costs of those positions
+        # every global best is at most every personal best, so the new global best is the
+        # lower of the old one and this swarm's lowest cost
+        assert plain._trace[k + 1, 0] == min(plain._trace[k, 0], cost.min())
+    bp = tmp_path / "best.json"
+    fitter.fit(interp, small.flux, small.var, seed=5, best_particle_file=str(bp))
+    best = json.load(open(bp))
+    assert len(best) == 6
+    np.testing.assert_array_equal([b["cost"] for b in best], plain._trace[1:, 0])
+    np.testing.assert_array_equal([b["position"] for b in best], plain._trace[1:, 1:])
