@@ -81,20 +81,22 @@ def make_stars(pasta, synth, interp, wlg, spec, n_stars, seed=4000, n_range=(40_
     return stars, meta
 
 
-def settings(pasta, P, iters):
-    st = pasta.api._settings_c(pasta.PSOSettings(P, iters), (0.0, 300.0), (-150.0, 150.0), None)
+def settings(pasta, P, iters, vsini_range=(0.0, 300.0), constraints=None):
+    st = pasta.api._settings_c(pasta.PSOSettings(P, iters), vsini_range, (-150.0, 150.0),
+                               constraints)
     return st
 
 
-def run_batched(pasta, _lib, interp, stars, meta, P, warmup, steps, timing_every=5):
-    """Every star runs one PSO fit of max(200, warmup + steps + 3) iterations in the server.
-    The timed steps are graph replays, except every timing_every-th, which runs the same
-    launches eagerly with HIP events around k_model and the whole iteration."""
+def run_batched(pasta, _lib, interp, stars, meta, P, warmup, steps, timing_every=5,
+                min_iters=200, st=None):
+    """Every star runs one PSO fit of max(min_iters, warmup + steps + 3) iterations in the
+    server.  The timed steps are graph replays, except every timing_every-th, which runs the
+    same launches eagerly with HIP events around k_model and the whole iteration."""
     cfg = _lib.ServerConfig(len(stars), P, 1, 0, 0)
     sv = C.c_void_p()
     _lib.check(_lib.lib.pasta_server_create(interp._h, C.byref(cfg), C.byref(sv)), "server")
-    iters = max(200, warmup + steps + 3)
-    st = settings(pasta, P, iters)
+    iters = max(min_iters, warmup + steps + 3)
+    st = st or settings(pasta, P, iters)
     try:
         for k, s in enumerate(stars):
             rid = C.c_uint64()
@@ -128,6 +130,43 @@ def run_batched(pasta, _lib, interp, stars, meta, P, warmup, steps, timing_every
                     iters=iters, flops=tm[5] / tm[4], bytes=tm[6] / tm[4], timed=int(tm[4]))
     finally:
         _lib.lib.pasta_server_destroy(sv)
+
+
+def rotation_taps(vsini, log10_step):
+    """Taps of the reference's rotation kernel (src/convolve_rv.rs:438-449)."""
+    eps, vrot = 0.6, vsini / 299792.0
+    a = 4.0 * (eps - 1.0) / (np.pi * eps)
+    n = int(np.ceil(2.0 * vrot / (np.log(10.0) * log10_step) * np.sqrt(1.0 + a * a))) + 2
+    return n + 1 - n % 2
+
+
+def run_rotation_sweep(pasta, _lib, synth, interp, wlg, spec, taps, n_stars=128, P=64):
+    """k_model time per rotation-kernel size on the direct (matrix-core) and FFT paths:
+    n_stars of the batched workload's stars, P particles each, vsini fixed (FixConstraint)
+    so that every evaluation's kernel has the bucket's taps (plus the 6 of the composed
+    R = 85,000 instrumental kernel); 2 warm-up and 4 timed iterations, each timed with HIP
+    events around k_model."""
+    rows = []
+    for K in taps:
+        lo, hi = 0.1, 3000.0
+        for _ in range(60):
+            mid = 0.5 * (lo + hi)
+            lo, hi = (lo, mid) if rotation_taps(mid, spec.log10_step) >= K else (mid, hi)
+        v = hi
+        row = {"taps": rotation_taps(v, spec.log10_step), "vsini": round(v, 3)}
+        for name, fft_taps in (("direct", 1 << 20), ("fft", 3)):
+            interp.set_fft_taps(fft_taps)
+            try:
+                stars, meta = make_stars(pasta, synth, interp, wlg, spec, n_stars)
+                st = settings(pasta, P, 9, (0.0, 3000.0), [pasta.FixConstraint(3, v)])
+                rb = run_batched(pasta, _lib, interp, stars, meta, P, 2, 4, timing_every=1,
+                                 min_iters=9, st=st)
+            finally:
+                interp.set_fft_taps(0)
+            row[name + "_ms"] = round(rb["model_ms"], 3)
+        log(f"[bench] rotation sweep: {row}")
+        rows.append(row)
+    return rows
 
 
 def run_plain(pasta, synth, interp, wlg, spec, iters=200, P=100, seed=2000):
@@ -394,6 +433,9 @@ def main():
                     help="arrival window per rate (requests = rate x seconds, at most 1000)")
     ap.add_argument("--serve-particles", type=int, default=32)
     ap.add_argument("--serve-iters", type=int, default=100)
+    ap.add_argument("--rotation-sweep", default="",
+                    help="comma-separated rotation-kernel taps: time k_model on the direct and "
+                         "FFT paths per size, print the table and exit")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="also run one plain copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -414,6 +456,12 @@ def main():
     log(f"[bench] grid {interp.n_models} x {interp.n_pix} filled in HBM in {time.perf_counter() - t0:.1f} s")
     if args.copy_probe_gib > 0:
         synth.copy_probe(int(args.copy_probe_gib * (1 << 30)))
+
+    if args.rotation_sweep:
+        taps = [int(x) for x in args.rotation_sweep.split(",") if x]
+        rows = run_rotation_sweep(pasta, _lib, synth, interp, wlg, spec, taps)
+        print(json.dumps({"rotation_sweep": rows}), flush=True)
+        return
 
     t0 = time.perf_counter()
     stars, meta = make_stars(pasta, synth, interp, wlg, spec, args.stars,

@@ -76,6 +76,12 @@ int pasta_grid_device_rows(const pasta_grid *g, uint16_t **rows, int64_t *pitch)
 int pasta_grid_read_rows(const pasta_grid *g, int row0, int nrows, uint16_t *out);
 /* Upload per-row factors after creation (storage order), or NULL to clear. */
 int pasta_grid_set_factors(pasta_grid *g, const float *factors_storage_order);
+/* Rotation-stage crossover (no counterpart in the reference, which always convolves by
+ * FFT, src/convolve_rv.rs:484-529): kernels of at least min_taps taps are convolved by a
+ * 4096-point FFT where their tiles fit one transform, shorter ones directly on the matrix
+ * cores.  min_taps <= 0 restores the default; a value above 2048 turns the FFT path off.
+ * Applies to evaluations launched (and graphs captured) after the call. */
+int pasta_grid_set_fft_taps(pasta_grid *g, int min_taps);
 /* get_local_grid + calculate_interpolation_coefficients (src/interpolate.rs:424-533,
  * src/cubic.rs:232-243): storage row of each of the 64 neighbours (-1 absent) and
  * the f32 weights, in the reference's neighbour order. */

@@ -148,6 +148,7 @@ SIGNATURES = {
     "pasta_grid_axes": [_P, _dp, _dp, _dp, _ip, _ip, _ip],
     "pasta_grid_device_rows": [_P, C.POINTER(_u16p), _i64p],
     "pasta_grid_set_factors": [_P, _fp],
+    "pasta_grid_set_fft_taps": [_P, C.c_int],
     "pasta_grid_local_weights": [_P, C.c_double, C.c_double, C.c_double, _ip, _fp],
     "pasta_grid_clamp_1d": [_P, C.c_double, C.c_double, C.c_double, C.c_int, _dp],
     "pasta_grid_is_teff_logg_between_bounds": [_P, C.c_double, C.c_double],

@@ -545,6 +545,13 @@ class InMemInterpolator:
     def is_teff_logg_between_bounds(self, teff, logg):
         return bool(lib.pasta_grid_is_teff_logg_between_bounds(self._h, teff, logg))
 
+    def set_fft_taps(self, min_taps=0):
+        """Rotation kernels of at least `min_taps` taps are convolved by FFT (where the
+        tiles fit one 4096-point transform), shorter ones directly on the matrix cores; 0
+        restores the measured default, a value above 2048 turns the FFT path off.  The
+        reference always convolves by FFT (src/convolve_rv.rs:484-529)."""
+        _raise(lib.pasta_grid_set_fft_taps(self._h, int(min_taps)), "set_fft_taps")
+
     def clamp_1d(self, teff, m, logg, index):
         out = C.c_double()
         _raise(lib.pasta_grid_clamp_1d(self._h, teff, m, logg, index, C.byref(out)), "clamp_1d")

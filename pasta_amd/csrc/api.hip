@@ -165,6 +165,12 @@ int pasta_grid_read_rows(const pasta_grid* g, int row0, int nrows, uint16_t* out
     return PASTA_OK;
 }
 
+int pasta_grid_set_fft_taps(pasta_grid* g, int min_taps) {
+    if (!g) return fail(PASTA_ERR_ARG, "null grid");
+    g->fft_min_taps = min_taps <= 0 ? FFT_MIN_TAPS : min_taps;
+    return PASTA_OK;
+}
+
 int pasta_grid_set_factors(pasta_grid* g, const float* f) {
     if (!g) return fail(PASTA_ERR_ARG, "null grid");
     if (!f) { g->has_factors = false; return PASTA_OK; }
@@ -859,14 +865,50 @@ int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
     return PASTA_OK;
 }
 
+// Whether any evaluation of vsini <= max_vsini on these stars can reach the grid's FFT
+// crossover: the rotation kernel's taps, plus the fixed instrumental kernel's when it is
+// composed into them.  Launches without that possibility run the kernels built without the
+// FFT stage (the path choice is then made with the crossover off, so it never depends on
+// this bound being tight).
+bool may_fft(const pasta_grid* g, double max_vsini, int extra_taps) {
+    if (g->fft_min_taps > KROT_MAX) return false;
+    if (!(max_vsini < 1e30)) return true;
+    if (!(max_vsini > 0.0)) return false;
+    const double eps = 0.6, dvelo = LN_10 * g->step, vrot = max_vsini / C_KMS;
+    const double a = 4.0 * (eps - 1.0) / (M_PI * eps);
+    const double v = std::ceil(2.0 * vrot / dvelo * std::sqrt(1.0 + a * a));
+    if (!(v < 1e6)) return true;
+    return (long long)v + 3 + extra_taps >= g->fft_min_taps;
+}
+
+int composed_extra_taps(const pasta_star* s) {
+    return s->dev.disp_kind == DISP_FIXED ? s->dev.Kd - 1 : 0;
+}
+
+bool may_fft(const pasta_grid* g, double max_vsini, const std::vector<pasta_star*>& stars) {
+    int extra = 0;
+    for (const pasta_star* s : stars) extra = std::max(extra, composed_extra_taps(s));
+    return may_fft(g, max_vsini, extra);
+}
+
+double max_label_vsini(const double* labels5, int n) {
+    double m = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double v = labels5[5 * (size_t)i + 3];
+        if (!(v <= m)) m = std::isnan(v) ? HUGE_VAL : v;
+    }
+    return m;
+}
+
 int batch_launch(Batch& B) {
     DevGrid G = B.grid->dev();
+    const bool fft = may_fft(B.grid, B.max_vsini, B.stars);
     HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p,
                        B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.d_pair_eq.p, B.d_items.p,
                        B.d_nitems.p, B.n_bm, B.d_bmom.p,
                        B.d_bchi2.p, B.cls.data(), (int)B.cls.size(), B.d_solve.p, B.n_solve,
                        B.d_model.p, B.d_mpart.p, B.d_pf.p, B.d_cpart.p, B.d_chi2.p,
-                       B.d_status.p, B.mode, B.eager_events ? B.ev[0] : nullptr,
+                       B.d_status.p, B.mode, fft, B.eager_events ? B.ev[0] : nullptr,
                        B.eager_events ? B.ev[1] : nullptr, B.stream));
     return PASTA_OK;
 }
@@ -874,6 +916,7 @@ int batch_launch(Batch& B) {
 int batch_eval(Batch& B, const double* labels, double* chi2, int* status) {
     HIPCHK(hipMemcpyAsync(B.d_labels.p, labels, sizeof(double) * 5 * B.n_evals,
                           hipMemcpyHostToDevice, B.stream));
+    B.max_vsini = max_label_vsini(labels, B.n_evals);
     CHK(batch_launch(B));
     if (chi2)
         HIPCHK(hipMemcpyAsync(chi2, B.d_chi2.p, sizeof(double) * B.n_evals, hipMemcpyDeviceToHost,
@@ -1245,10 +1288,14 @@ static int pso_batch_setup(PsoBatch& PB, pasta_grid* g, const std::vector<pasta_
     vel.assign((size_t)B.n_evals * 5, 0.0);
     PB.h_pso.assign(ns, DevPso{});
     *max_it = 0;
+    B.max_vsini = 0.0;
     for (int s = 0; s < ns; s++) {
         const pasta_fit_settings* st = sts[s];
         DevPso& P = PB.h_pso[s];
         P.b = make_bounds(st);
+        double lo[5], hi[5];
+        bounds_outer(g->geom, P.b, lo, hi);  // the PSO keeps its particles inside these
+        B.max_vsini = std::max(B.max_vsini, std::isnan(hi[3]) ? HUGE_VAL : hi[3]);
         P.s = PsoParams{st->inertia_factor, st->cognitive_factor, st->social_factor};
         P.key_iter = rng_u64(seeds[s], KEY_ITER);
         P.P = st->num_particles;

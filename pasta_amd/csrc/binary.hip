@@ -129,7 +129,8 @@ static int binary_launch(pasta_binary* b, const double* labels, int P, bool chi2
                        Bc.d_tbound.p, Bc.tb_stride, Bc.d_hist.p, Bc.d_perm.p, Bc.d_pair_eq.p,
                        Bc.d_items.p, Bc.d_nitems.p, Bc.n_bm, Bc.d_bmom.p, Bc.d_bchi2.p,
                        Bc.cls.data(), 0, Bc.d_solve.p, 0, Bc.d_model.p, Bc.d_mpart.p, Bc.d_pf.p,
-                       Bc.d_cpart.p, Bc.d_chi2.p, Bc.d_status.p, 1, nullptr, nullptr, st));
+                       Bc.d_cpart.p, Bc.d_chi2.p, Bc.d_status.p, 1,
+                       may_fft(b->g, HUGE_VAL, Bc.stars), nullptr, nullptr, st));
     HIPCHK(hipStreamWaitEvent(st, b->join, 0));
     HIPCHK(launch_binary(G, Bo.d_stars.p, Bo.d_evals.p, P * b->S, b->d_lab11.p, Bc.d_evals.p, P,
                          Bc.d_model.p, b->d_cstat.p, b->d_cont.p, b->d_cmean.p, b->N,

@@ -29,8 +29,22 @@ constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:43
 // split; their taps are kept as 3 split parts x 2 alignments x TW_MAX packed bf16 pairs.
 constexpr int KB_MAX = 640;
 constexpr int TW_MAX = (32 * ((KB_MAX + 15 + 31) / 32) + 32) / 2;  // 352
-// per-evaluation rotation buffer: f32 taps, then the split pairs
+// per-evaluation rotation buffer: f32 taps, then the split pairs (or the kernel's spectrum,
+// 2049 complex values, for the FFT path)
 constexpr int ROT_STRIDE = KROT_MAX + 6 * TW_MAX;
+// Default crossover of the rotation stage: kernels of at least this many taps are convolved
+// by FFT (if the tile and kernel fit one 4096-point transform), shorter ones directly on the
+// matrix cores.  Measured on the batched workload (128 stars x 64 particles, full grid): the
+// FFT stage costs about the same at any size, the bf16 x 3 products (up to KB_MAX taps) stay
+// cheaper, and the f32 products above KB_MAX are overtaken at ~677 taps.  Set per grid with
+// pasta_grid_set_fft_taps.
+constexpr int FFT_MIN_TAPS = 677;
+constexpr int FFT_N = 4096;       // real points of the FFT path's transforms
+constexpr int FFT_M = FFT_N / 2;  // their complex half-length
+// A K-tap kernel whose tiles carry hd instrumental half-width pixels on each side fits one
+// transform: every tile's interpolated range (<= 7 + TILE + 1 + 2 hd + K - 1, rounded up to
+// 8) stays within FFT_N samples, so no output index of the circular convolution wraps.
+__host__ __device__ constexpr bool fft_fits(int K, int hd) { return K + 2 * hd <= FFT_N - TILE - 14; }
 constexpr int CH_MAX = 32;        // max chunks
 constexpr int P1_MAX = 32;        // max continuum parameters per chunk / linear columns
 constexpr int BLK = PASTA_CONT_BLOCK;  // observed pixels per reduction block
@@ -78,7 +92,7 @@ struct DevGrid {
     double sg_first, sg_step; // log10 of the first wavelength, log10 step
     double synth_first_wl, synth_last_wl;  // 10^first, 10^(first + step*W)
     int n_models_bins;        // n_models + 1 (sort bins; the last holds failed evaluations)
-    int pad0;
+    int fft_min_taps;         // rotation kernels of at least this many taps use the FFT path
 };
 
 struct DevStar {
@@ -122,6 +136,9 @@ struct DevEval {
     int rows_hash;   // hash of idx[]: evaluations with equal rows can share their gathers
     int comp;        // 1: the krot taps are the rotation kernel convolved with the fixed
                      // instrumental kernel, so k_model has no instrumental stage
+    int fft;         // 1: the rotation runs as an FFT convolution; the tap slot holds the
+                     // kernel's spectrum (k_rot_spectrum)
+    int pad1;
     double shift_log;
     long long m_off;  // offset of this evaluation's resampled model in the model buffer
     long long mp_off; // offset of its continuum-moment partials (doubles)

@@ -22,7 +22,7 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
                        int n_model,
                        const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
                        const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
-                       double* cpart, double* chi2, int* status, int mode,
+                       double* cpart, double* chi2, int* status, int mode, bool fft,
                        hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st);
 hipError_t launch_pso_update(const GridGeom* geom, DevPso* pso, DevEval* evals, int n, double* pos,
                              double* vel, const double* bpos, hipStream_t st);
@@ -115,6 +115,7 @@ struct pasta_grid {
     long long pitch = 0;
     int W = 0;
     double first = 0, step = 0, first_wl = 0, last_wl = 0;
+    int fft_min_taps = FFT_MIN_TAPS;
     std::mutex mu;
     DevGrid dev() const {
         DevGrid g;
@@ -129,7 +130,7 @@ struct pasta_grid {
         g.synth_first_wl = first_wl;
         g.synth_last_wl = last_wl;
         g.n_models_bins = geom.n_models + 1;
-        g.pad0 = 0;
+        g.fft_min_taps = fft_min_taps;
         return g;
     }
 };
@@ -176,6 +177,9 @@ struct pasta_star {
 
 struct Batch {
     int mode = 0;  // 0: resample to the observed pixels; 1: synthetic-grid output
+    // largest vsini of the batch's evaluations (from the labels or the PSO bounds; infinite
+    // when unknown): selects whether its launches carry the FFT rotation path
+    double max_vsini = HUGE_VAL;
     int n_evals = 0, n_stars = 0;
     std::vector<ContClass> cls;
     pasta_grid* grid = nullptr;
@@ -216,6 +220,11 @@ int star_build(pasta_grid* g, const pasta_disp* d, const pasta_cont* c, const fl
 // observed pixels, mode 1 keeps the synthetic grid
 int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
                 const std::vector<int>& counts, int mode);
+// whether evaluations of vsini <= max_vsini on these stars can reach the FFT crossover
+bool may_fft(const pasta_grid* g, double max_vsini, const std::vector<pasta_star*>& stars);
+bool may_fft(const pasta_grid* g, double max_vsini, int extra_taps);
+int composed_extra_taps(const pasta_star* s);  // taps a composed instrumental kernel adds
+double max_label_vsini(const double* labels5, int n);  // NaN counts as unbounded
 int batch_launch(Batch& B);
 int batch_eval(Batch& B, const double* labels, double* chi2, int* status);
 const char* status_text(int st);

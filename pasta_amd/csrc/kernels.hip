@@ -111,6 +111,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     const double dvelo = LN10 * G.sg_step;
     __shared__ int s_st;
     __shared__ int s_comp;
+    __shared__ int s_fft;
     const bool active = ev.active;
     if (lane == 0) s_st = active ? local_weights(*G.geom, lab[0], lab[1], lab[2], ev.idx, ev.w) : -1;
     __syncthreads();
@@ -175,6 +176,17 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
         ev.comp = comp;
         s_comp = comp;
         ev.krot = krot;
+        // rotation path: the FFT for kernels of at least fft_min_taps taps that fit one
+        // transform with their tiles, the matrix cores otherwise
+        bool fft = false;
+        if (st == ST_OK && krot > 0) {
+            const DevStar& S = stars[ev.star];
+            const int Kt = comp ? krot + S.Kd - 1 : krot;
+            const int hd = (comp || S.disp_kind == DISP_NONE) ? 0 : (S.Kd - 1) / 2;
+            fft = Kt >= G.fft_min_taps && fft_fits(Kt, hd);
+        }
+        ev.fft = fft;
+        s_fft = fft;
         ev.rows_hash = (int)hsh;
         // storage row of the first present neighbour
         ev.key = (st == ST_OK && present) ? first : G.geom->n_models;
@@ -236,7 +248,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
             for (int k = lane; k < Kt; k += 64) h[k] = tmp[k];
             if (lane == 0) ev.krot = Kt;
         }
-        if (Kt <= KB_MAX) {
+        if (Kt <= KB_MAX && !s_fft) {
             // reversed, zero-padded taps tpad[16 + u] = h[Kt - 1 - u], split into bf16
             // hi/mid/lo, as pairs tw[part][parity][q] = (tpad[2q + parity], tpad[2q + parity + 1])
             __syncthreads();
@@ -312,6 +324,15 @@ __device__ __forceinline__ void gather_rows(const DevGrid& G, const DevEval& ev,
             acc8<FAST>(acc, v[jj], fw, x0, G.Wq);
         }
     }
+}
+
+// Workgroup barrier that orders LDS accesses only: k_model's threads share nothing through
+// global memory, so outstanding global loads and stores (the model's resampled pixels) need
+// not drain at every barrier, as the fence of __syncthreads() would make them.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 __device__ __forceinline__ int islot(int k) { return k + ((k >> 4) << 2); }
@@ -507,6 +528,261 @@ __device__ __forceinline__ void conv_rot_bf16(const unsigned short* Ib3, const u
 }
 
 // ---------------------------------------------------------------------------
+// FFT convolution for wide rotation kernels (the reference convolves with an f32 FFT,
+// src/convolve_rv.rs:121-157,484-529).  The linear convolution y = x * a of a tile's
+// interpolated range x (at most FFT_N samples) with the kernel a (K taps) is read off a
+// circular one of length FFT_N wherever no index wraps: y[n] for K - 1 <= n < FFT_N.  The
+// real transforms of length FFT_N run as complex transforms of FFT_M = FFT_N / 2 points
+// z[n] = x[2n] + i x[2n+1] (the standard split into even and odd samples), in place in LDS
+// on NT = 256 threads: a Stockham transform with radices 8, 8, 8, 4, every thread holding
+// one radix-8 (or two radix-4) butterflies per pass, the passes separated by LDS barriers.
+// Twiddles come from sincospif (correctly rounded argument scaling), the higher powers of
+// a pass's twiddle from one or two products of accurate ones.  The kernel's spectrum is
+// computed once per evaluation (k_rot_spectrum) into its tap slot.
+// ---------------------------------------------------------------------------
+constexpr int FFT_SLOTS = FFT_M + FFT_M / 16;  // padded complex slots
+static_assert(2 * FFT_SLOTS <= ILDS, "FFT buffer within the rotation stage's LDS");
+static_assert(2 * (FFT_M + 1) <= ROT_STRIDE, "kernel spectrum within an evaluation's tap slot");
+static_assert(NT == 256 && FFT_M == 8 * NT, "one radix-8 butterfly per thread and pass");
+
+struct cf {
+    float x, y;
+};
+__device__ __forceinline__ cf cadd(cf a, cf b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cf csub(cf a, cf b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cf cmul(cf a, cf b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ cf cconj(cf a) { return {a.x, -a.y}; }
+__device__ __forceinline__ cf cmi(cf a) { return {a.y, -a.x}; }  // a * (-i)
+__device__ __forceinline__ cf cscale(cf a, float s) { return {a.x * s, a.y * s}; }
+// e^{i pi t}
+__device__ __forceinline__ cf cexpi(float t) {
+    float s, c;
+    sincospif(t, &s, &c);
+    return {c, s};
+}
+// padded slot of complex index i: one pad slot after every 16 spreads the strided
+// butterfly stores over the LDS banks
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
+
+__device__ __forceinline__ void dft2(cf& a, cf& b) {
+    const cf t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+// forward 8-point DFT in place (decimation in frequency), natural-order output
+__device__ __forceinline__ void dft8(cf (&v)[8]) {
+    const float r = 0.707106781186547524f;
+    dft2(v[0], v[4]); dft2(v[1], v[5]); dft2(v[2], v[6]); dft2(v[3], v[7]);
+    v[5] = {(v[5].x + v[5].y) * r, (v[5].y - v[5].x) * r};     // * e^{-i pi/4}
+    v[6] = cmi(v[6]);                                          // * e^{-i pi/2}
+    v[7] = {(v[7].y - v[7].x) * r, -(v[7].x + v[7].y) * r};    // * e^{-3i pi/4}
+    dft2(v[0], v[2]); dft2(v[1], v[3]); dft2(v[4], v[6]); dft2(v[5], v[7]);
+    v[3] = cmi(v[3]);
+    v[7] = cmi(v[7]);
+    dft2(v[0], v[1]); dft2(v[2], v[3]); dft2(v[4], v[5]); dft2(v[6], v[7]);
+    const cf o1 = v[4], o2 = v[2], o3 = v[6], o4 = v[1], o6 = v[3];
+    v[1] = o1; v[2] = o2; v[3] = o3; v[4] = o4; v[6] = o6;
+}
+
+__device__ __forceinline__ void dft4(cf (&v)[4]) {
+    dft2(v[0], v[2]); dft2(v[1], v[3]);
+    v[3] = cmi(v[3]);
+    dft2(v[0], v[1]); dft2(v[2], v[3]);
+    const cf t = v[1];
+    v[1] = v[2];
+    v[2] = t;
+}
+
+// Forward DFT of FFT_M points in place: input buf[i] (buf[fpad(i)] when PADDED_IN), output
+// buf[fpad(k)], natural order.  Passes (Ns, R) = (1, 8), (8, 8), (64, 8), (512, 4): thread j
+// reads in[j + r FFT_M / R], applies e^{-2 pi i (j mod Ns) r / (Ns R)}, transforms, and
+// writes out[(j / Ns) Ns R + j mod Ns + r Ns].  Starts and ends with an LDS barrier.
+template <bool PADDED_IN>
+__device__ void fft_m(cf* buf) {
+    const int j = threadIdx.x;
+    cf v[8];
+    lds_barrier();
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[r] = buf[PADDED_IN ? fpad(j + NT * r) : j + NT * r];
+    dft8(v);
+    lds_barrier();
+#pragma unroll
+    for (int r = 0; r < 8; r++) buf[fpad(8 * j + r)] = v[r];
+#pragma unroll 1
+    for (int Ns = 8; Ns <= 64; Ns *= 8) {
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = buf[fpad(j + NT * r)];
+        const int jm = j & (Ns - 1);
+        const float t = -2.0f * (float)jm / (float)(8 * Ns);  // exact: a dyadic rational
+        const cf w1 = cexpi(t), w2 = cexpi(2.0f * t), w4 = cexpi(4.0f * t);
+        const cf w3 = cmul(w1, w2);
+        v[1] = cmul(v[1], w1);
+        v[2] = cmul(v[2], w2);
+        v[3] = cmul(v[3], w3);
+        v[4] = cmul(v[4], w4);
+        v[5] = cmul(v[5], cmul(w1, w4));
+        v[6] = cmul(v[6], cmul(w2, w4));
+        v[7] = cmul(v[7], cmul(w3, w4));
+        dft8(v);
+        lds_barrier();
+        const int d = (j / Ns) * Ns * 8 + jm;
+#pragma unroll
+        for (int r = 0; r < 8; r++) buf[fpad(d + Ns * r)] = v[r];
+    }
+    lds_barrier();
+    cf u[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int jj = j + NT * h;  // Ns = 512 > jj: no index expansion
+#pragma unroll
+        for (int r = 0; r < 4; r++) u[h][r] = buf[fpad(jj + 512 * r)];
+        const float t = -2.0f * (float)jj / 2048.0f;
+        const cf w1 = cexpi(t), w2 = cexpi(2.0f * t);
+        u[h][1] = cmul(u[h][1], w1);
+        u[h][2] = cmul(u[h][2], w2);
+        u[h][3] = cmul(u[h][3], cmul(w1, w2));
+        dft4(u[h]);
+    }
+    lds_barrier();
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) buf[fpad(j + NT * h + 512 * r)] = u[h][r];
+    lds_barrier();
+}
+
+// The even/odd split of a real transform.  From Z = DFT_M(z), the spectrum of x at k and
+// M - k (0 < k < M): E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i,
+// X[k] = E + W^k O, X[M-k] = conj(E - W^k O), W = e^{-2 pi i / FFT_N}.
+__device__ __forceinline__ void real_split(cf zk, cf zm, cf w, cf& xk, cf& xm) {
+    const cf e = cscale(cadd(zk, cconj(zm)), 0.5f);
+    const cf o = cscale(cmi(csub(zk, cconj(zm))), 0.5f);
+    const cf wo = cmul(w, o);
+    xk = cadd(e, wo);
+    xm = cconj(csub(e, wo));
+}
+
+// Its inverse for the half spectrum Y of a real sequence y: Z'[k] = (Y[k] + conj Y[M-k])
+// + i (Y[k] - conj Y[M-k]) W^-k has IDFT_M(Z')[n] = y[2n] + i y[2n+1]; stored conjugated,
+// so that the forward transform computes the inverse.
+__device__ __forceinline__ void real_join(cf yk, cf ym, cf w, cf& zk, cf& zm) {
+    const cf a = cadd(yk, cconj(ym));
+    const cf b = cmul(csub(yk, cconj(ym)), cconj(w));
+    zk = cconj(cadd(a, {-b.y, b.x}));                // a + i b
+    // the same at M - k: W^-(M-k) = -W^k
+    const cf a2 = cadd(ym, cconj(yk));
+    const cf b2 = cmul(csub(ym, cconj(yk)), {-w.x, -w.y});
+    zm = cconj(cadd(a2, {-b2.y, b2.x}));
+}
+
+// Frequencies of thread t in the split steps: k = t + NT i (i < 4) paired with M - k, and
+// k = M / 2 on thread 0; W^k = e^{-i pi k / M}.
+__device__ __forceinline__ cf split_twiddle(int t, int i) {
+    return cexpi(-(float)(t + NT * i) / (float)FFT_M);
+}
+
+// Spectrum A[k], k <= M, of the K taps h (zero-padded to FFT_N) into spec (global).
+__device__ void kernel_spectrum(const float* __restrict__ h, int K, cf* buf, cf* spec) {
+    float* xb = (float*)buf;
+    for (int i = threadIdx.x; i < FFT_N; i += NT) xb[i] = i < K ? h[i] : 0.0f;
+    fft_m<false>(buf);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int k = t + NT * i;
+        const cf zk = buf[fpad(k)], zm = buf[fpad((FFT_M - k) & (FFT_M - 1))];
+        if (k == 0) {
+            spec[0] = {zk.x + zk.y, 0.0f};
+            spec[FFT_M] = {zk.x - zk.y, 0.0f};
+        } else {
+            cf xk, xm;
+            real_split(zk, zm, split_twiddle(t, i), xk, xm);
+            spec[k] = xk;
+            spec[FFT_M - k] = xm;
+        }
+    }
+    if (t == 0) {
+        const cf z = buf[fpad(FFT_M / 2)];
+        cf xk, xm;
+        real_split(z, z, cexpi(-0.5f), xk, xm);
+        spec[FFT_M / 2] = xk;
+    }
+}
+
+// out[q] = sum_u h[u] x[off + q + K - 1 - u] (the rotation stage's "same" convolution of the
+// tile, x at logical index 0 = the first interpolated pixel) for q < nR, zero outside
+// [0, W) when mask.  xb: the tile as FFT_N floats (zero past the interpolated range), which
+// the transform overwrites; spec: the kernel's spectrum.
+__device__ void conv_fft(float* xb, const cf* __restrict__ spec, int K, int off, int nR,
+                         float* out, bool mask, long long x0, int W) {
+    cf* buf = (cf*)xb;
+    const int t = threadIdx.x;
+    // the spectrum's values for this thread's frequencies, read now and used after the
+    // forward transform (the LDS barriers do not wait for them)
+    cf ak[4], am[4], ah = {0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int k = t + NT * i;
+        ak[i] = spec[k];
+        am[i] = spec[FFT_M - k];
+    }
+    if (t == 0) ah = spec[FFT_M / 2];
+    fft_m<false>(buf);
+    const float s = 1.0f / (float)FFT_N;  // the inverse transform's scale, exact
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int k = t + NT * i;
+        const int km = (FFT_M - k) & (FFT_M - 1);
+        const cf zk = buf[fpad(k)], zm = buf[fpad(km)];
+        if (k == 0) {
+            // X[0] and X[M] are real: Re Z[0] +- Im Z[0]
+            const float y0 = (zk.x + zk.y) * ak[i].x * s, yM = (zk.x - zk.y) * am[i].x * s;
+            buf[fpad(0)] = cconj({y0 + yM, y0 - yM});
+        } else {
+            const cf w = split_twiddle(t, i);
+            cf xk, xm, zk2, zm2;
+            real_split(zk, zm, w, xk, xm);
+            real_join(cscale(cmul(xk, ak[i]), s), cscale(cmul(xm, am[i]), s), w, zk2, zm2);
+            buf[fpad(k)] = zk2;
+            buf[fpad(km)] = zm2;
+        }
+    }
+    if (t == 0) {
+        const cf z = buf[fpad(FFT_M / 2)];
+        const cf w = cexpi(-0.5f);
+        cf xk, xm, zk2, zm2;
+        real_split(z, z, w, xk, xm);
+        const cf y = cscale(cmul(xk, ah), s);
+        real_join(y, y, w, zk2, zm2);
+        buf[fpad(FFT_M / 2)] = zk2;
+    }
+    fft_m<true>(buf);
+    // IDFT_M(Z')[n] = conj(DFT_M(conj Z'))[n] = y[2n] + i y[2n+1]
+    const int base = off + K - 1;
+    for (int q = t; q < nR; q += NT) {
+        const int n = base + q;
+        const cf z = buf[fpad(n >> 1)];
+        const float y = (n & 1) ? -z.y : z.x;
+        const long long x = x0 + q;
+        out[q] = (mask && (x < 0 || x >= W)) ? 0.0f : y;
+    }
+}
+
+// the kernel spectra of the evaluations on the FFT path, into their tap slots
+__global__ __launch_bounds__(NT) void k_rot_spectrum(const DevEval* __restrict__ evals,
+                                                     float* __restrict__ rot) {
+    const DevEval& ev = evals[blockIdx.x];
+    if (ev.status != ST_OK || !ev.fft) return;
+    __shared__ __align__(16) cf buf[FFT_SLOTS];
+    float* h = rot + (size_t)blockIdx.x * ROT_STRIDE;
+    kernel_spectrum(h, ev.krot, buf, (cf*)h);
+}
+
+// ---------------------------------------------------------------------------
 // k_model: one workgroup per (work item, synthetic tile).  A work item is one
 // evaluation, or two or three evaluations that gather the same 64 neighbour rows: their
 // interpolation shares every 16-byte gather and u16 conversion (one conversion and one
@@ -522,7 +798,7 @@ struct TileJob {
     long long rlo;   // first pixel of the rotation output R (clo - hd)
     int nC, hd, K, nR;
     bool mask;
-    int kind;        // 0 no rotation (tile into Rb), 1 bf16 rotation, 2 f32 rotation
+    int kind;        // 0 no rotation (tile into Rb), 1 bf16 rotation, 2 f32 rotation, 3 FFT
     int inst;        // instrumental stage: DISP_* of the star, DISP_NONE when composed
     long long g0;    // first interpolated 8-pixel group (multiple of 8)
     int off, ng;     // logical index of rlo - K/2 (or rlo); groups interpolated
@@ -562,7 +838,7 @@ __device__ __forceinline__ bool tile_job(const DevGrid& G, const DevStar* stars,
     J.rlo = J.clo - J.hd;
     J.nR = J.nC + 2 * J.hd;
     J.mask = J.inst != DISP_NONE;
-    J.kind = J.K == 0 ? 0 : (J.K <= KB_MAX ? 1 : 2);
+    J.kind = J.K == 0 ? 0 : (ev.fft ? 3 : (J.K <= KB_MAX ? 1 : 2));
     const long long first = J.rlo - hr;
     J.g0 = first & ~7LL;
     J.off = (int)(first - J.g0);
@@ -588,7 +864,7 @@ __device__ __forceinline__ void store_group(const TileJob& J, int gl, const floa
         *(u16x8*)(d + ILOGB) = m;
         *(u16x8*)(d + 2 * ILOGB) = l;
     } else {
-        float* d = J.kind == 2 ? Ib + islot(8 * gl) : Rb + 8 * gl;
+        float* d = J.kind == 2 ? Ib + islot(8 * gl) : J.kind == 3 ? Ib + 8 * gl : Rb + 8 * gl;
         *(f4*)d = f4{acc[0], acc[1], acc[2], acc[3]};
         *(f4*)(d + 4) = f4{acc[4], acc[5], acc[6], acc[7]};
     }
@@ -689,6 +965,8 @@ __device__ __forceinline__ void load_taps(const TileJob& J, const float* rot, fl
             hrev[i] = (u >= 0 && u < K) ? h[K - 1 - u] : 0.0f;
         }
         if (tid < 48) Ib[islot(8 * J.ng + tid)] = 0.0f;  // the MFMA steps read past the tile
+    } else if (J.kind == 3) {
+        for (int i = 8 * J.ng + tid; i < FFT_N; i += NT) Ib[i] = 0.0f;  // the transform's input
     }
     if (J.inst == DISP_FIXED) {
         const int Kp = (S.Kd + 7) & ~7;
@@ -698,22 +976,26 @@ __device__ __forceinline__ void load_taps(const TileJob& J, const float* rot, fl
 
 // rotation, instrumental broadening and resample of a job whose interpolated tile and
 // taps are in LDS (stored but not yet synchronised)
+template <bool FFT = false>
 __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, const float* rot,
                                            float* model, int mode, float* Ib, float* Rb,
                                            float* hrev, unsigned int* tw, float* grev) {
     const int tid = threadIdx.x;
     const DevStar& S = *J.S;
-    __syncthreads();
+    lds_barrier();
     // rotational broadening, outputs p = off + q for R index q
     float* R = Rb;
     if (J.kind == 1)
         conv_rot_bf16((const unsigned short*)Ib, tw, J.K, J.off + J.nR, J.off, Rb, J.mask, J.rlo, G.W);
     else if (J.kind == 2)
         conv_rot_mfma(Ib, hrev, J.K, J.off + J.nR, J.off, Rb, J.mask, J.rlo, G.W);
+    else if (FFT && J.kind == 3)
+        conv_fft(Ib, (const cf*)(rot + (size_t)J.e * ROT_STRIDE), J.K, J.off, J.nR, Rb, J.mask,
+                 J.rlo, G.W);
     else
         R = Rb + J.off;
     if (tid < 16) R[J.nR + tid] = 0.0f;
-    __syncthreads();
+    lds_barrier();
 
     // instrumental broadening -> C (reuses Ib)
     const int nC = J.nC, hd = J.hd;
@@ -721,7 +1003,7 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
     if (J.inst == DISP_FIXED) {
         conv_tile(R, grev, S.Kd, nC, Ib, false, 0, 0);
         Cb = Ib;
-        __syncthreads();
+        lds_barrier();
     } else if (J.inst == DISP_VARIABLE) {
         const int Kd = S.Kd;
         for (int q = tid; q < nC; q += NT) {
@@ -738,7 +1020,7 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
             Ib[q] = c;
         }
         Cb = Ib;
-        __syncthreads();
+        lds_barrier();
     }
 
     // RV shift + resample (f32-quantised weight, convolve_rv.rs:569-573)
@@ -774,6 +1056,7 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
 }
 
 // evaluation e alone on this tile (its job exists; the LDS is free)
+template <bool FFT = false>
 __device__ __forceinline__ void run_solo(const DevGrid& G, const DevStar* stars, const DevEval* evals,
                                          const float* rot, int e, int tile, const int* tbound,
                                          int tb_stride, int mode, float* model, float* Ib, float* Rb,
@@ -786,7 +1069,7 @@ __device__ __forceinline__ void run_solo(const DevGrid& G, const DevStar* stars,
         gather_one(G, *B.ev, B.g0 + 8LL * gg, acc);
         store_group(B, gg, acc, Ib, Rb);
     }
-    finish_job(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
+    finish_job<FFT>(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
 }
 
 // A work item of GN (2 or 3) evaluations with the same 64 neighbour rows whose tiles span
@@ -801,7 +1084,7 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
                                           float* Rb, float* hrev, unsigned int* tw, float* grev) {
     // with three evaluations the registers go to the kept interpolations instead
     const DevEval& eb = evals[e1];
-    const bool btw = GN == 2 && eb.krot > 0 && eb.krot <= KB_MAX;
+    const bool btw = GN == 2 && eb.krot > 0 && eb.krot <= KB_MAX && !eb.fft;
     const int bq4 = btw ? tap_quads(eb.krot) : 0;
     u32x4 tkeep[3];
     if (GN == 2) {
@@ -849,7 +1132,7 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
     finish_job(G, A, rot, model, mode, Ib, Rb, hrev, tw, grev);
 #pragma unroll
     for (int g = 1; g < GN; g++) {
-        __syncthreads();
+        lds_barrier();
         TileJob B;
         tile_job(G, stars, evals, g == 1 ? e1 : e2, tile, tbound, tb_stride, mode, B);
         if (g == 1 && btw) {  // the second evaluation's split taps, read before A's rotation
@@ -872,6 +1155,7 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
     }
 }
 
+template <bool FFT>
 __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __restrict__ stars,
                                               const DevEval* __restrict__ evals,
                                               const float* __restrict__ rot,
@@ -918,6 +1202,19 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
         cg0 = T.g0;
         cng = T.ng;
     }
+    // A work item with an evaluation on the FFT path runs its evaluations one at a time
+    // through the one copy of the FFT stage (its registers would not fit beside a group's)
+    if (FFT && ((hasA && A.kind == 3) || (hasB && evals[ids[1]].fft) || (hasC && evals[ids[2]].fft))) {
+        bool first = true;
+        for (int g = 0; g < 3; g++) {
+            if (!(g == 0 ? hasA : g == 1 ? hasB : hasC)) continue;
+            if (!first) lds_barrier();
+            first = false;
+            run_solo<true>(G, stars, evals, rot, ids[g], tile, tbound, tb_stride, mode, model, Ib, Rb,
+                           hrev, tw, grev);
+        }
+        return;
+    }
     if (hasA) load_taps(A, rot, Ib, hrev, tw, grev);
     if (hasA && hasB) {
         long long pg0 = min(A.g0, bg0), pg1 = max(A.g0 + 8LL * A.ng, bg0 + 8LL * bng);
@@ -944,12 +1241,12 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
         finish_job(G, A, rot, model, mode, Ib, Rb, hrev, tw, grev);
     }
     if (hasB) {
-        if (hasA) __syncthreads();
+        if (hasA) lds_barrier();
         run_solo(G, stars, evals, rot, ids[1], tile, tbound, tb_stride, mode, model, Ib, Rb, hrev,
                  tw, grev);
     }
     if (hasC) {
-        if (hasA || hasB) __syncthreads();
+        if (hasA || hasB) lds_barrier();
         run_solo(G, stars, evals, rot, ids[2], tile, tbound, tb_stride, mode, model, Ib, Rb, hrev,
                  tw, grev);
     }
@@ -1764,11 +2061,15 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
                        int n_model,
                        const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
                        const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
-                       double* cpart, double* chi2, int* status, int mode,
+                       double* cpart, double* chi2, int* status, int mode, bool fft,
                        hipEvent_t ev_model0, hipEvent_t ev_model1, hipStream_t st) {
     if (n_evals <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64), 0, st, G, stars, labels, evals, rot, tbound,
-                       tb_stride, mode);
+    // without the FFT path the crossover is off for k_prep, so no evaluation selects it
+    DevGrid Gp = G;
+    if (!fft) Gp.fft_min_taps = 1 << 30;
+    hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64), 0, st, Gp, stars, labels, evals, rot,
+                       tbound, tb_stride, mode);
+    if (fft) hipLaunchKernelGGL(k_rot_spectrum, dim3(n_evals), dim3(NT), 0, st, evals, rot);
     const int nb = G.n_models_bins;
     hipLaunchKernelGGL(k_sort_zero, dim3((nb + 255) / 256), dim3(256), 0, st, sort_hist, nb);
     hipLaunchKernelGGL(k_sort_hist, dim3((n_evals + 255) / 256), dim3(256), 0, st, evals, n_evals,
@@ -1781,9 +2082,12 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
     hipLaunchKernelGGL(k_pair_items, dim3(1), dim3(1024), 0, st, perm, pair_eq, n_evals, items,
                        n_items);
     if (ev_model0) (void)hipEventRecord(ev_model0, st);
-    if (n_model > 0)
-        hipLaunchKernelGGL(k_model, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot, items,
-                           n_items, n_evals, tbound, tb_stride, model, mode);
+    if (n_model > 0 && fft)
+        hipLaunchKernelGGL(k_model<true>, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot,
+                           items, n_items, n_evals, tbound, tb_stride, model, mode);
+    else if (n_model > 0)
+        hipLaunchKernelGGL(k_model<false>, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot,
+                           items, n_items, n_evals, tbound, tb_stride, model, mode);
     if (ev_model1) (void)hipEventRecord(ev_model1, st);
     return launch_cont(stars, evals, n_evals, bmom, bchi2, cls, n_cls, solve_items, n_solve,
                        model, mpart, pf, cpart, chi2, status, st);

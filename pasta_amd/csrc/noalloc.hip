@@ -192,7 +192,7 @@ int pasta_eval_chi2_noalloc(pasta_star* s, const double* labels, int P, double* 
     HIPCHK(launch_eval(G, b.star, b.evals, P, b.labels, b.rot, b.tbound, D.n_tiles + 1, b.hist,
                        b.perm, b.pair_eq, b.items, b.nitems, D.n_tiles * P, b.bmom, b.bchi2, &cls, 1,
                        b.solve, L.n_solve, b.model, b.mpart, b.pf, b.cpart, b.chi2, b.status, 0,
-                       nullptr, nullptr, st));
+                       may_fft(s->grid, max_label_vsini(labels, P), composed_extra_taps(s)), nullptr, nullptr, st));
     HIPCHK(hipMemcpyAsync(chi2, b.chi2, sizeof(double) * (size_t)P, hipMemcpyDeviceToHost, st));
     if (status)
         HIPCHK(hipMemcpyAsync(status, b.status, sizeof(int) * (size_t)P, hipMemcpyDeviceToHost, st));
