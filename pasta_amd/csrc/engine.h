@@ -25,8 +25,8 @@ static_assert(TILE % 256 == 240 && TILE >= 752 && TILE <= 2032,
 constexpr int NT = 256;           // threads per workgroup (4 wavefronts)
 constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
 constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)
-// Rotation kernels of up to KB_MAX taps run on bf16 matrix cores with a three-way operand
-// split; their taps are kept as 3 split parts x 2 alignments x TW_MAX packed bf16 pairs.
+// Rotation kernels of up to KB_MAX taps run on f16 matrix cores with a two-way operand
+// split; their taps are kept as 2 split parts x 2 alignments x TW_MAX packed f16 pairs.
 constexpr int KB_MAX = 640;
 constexpr int TW_MAX = (32 * ((KB_MAX + 15 + 31) / 32) + 32) / 2;  // 352
 // per-evaluation rotation buffer: f32 taps, then the split pairs (or the kernel's spectrum,
@@ -138,6 +138,9 @@ struct DevEval {
                      // instrumental kernel, so k_model has no instrumental stage
     int fft;         // 1: the rotation runs as an FFT convolution; the tap slot holds the
                      // kernel's spectrum (k_rot_spectrum)
+    float rsc;       // f16 rotation: power of 2 the interpolated values are scaled by before
+                     // their split (|value| * rsc < 2^14)
+    float rinv;      // and 1 / (rsc * tap scale): undoes both scales on the rotation's output
     int pad1;
     double shift_log;
     long long m_off;  // offset of this evaluation's resampled model in the model buffer

@@ -29,7 +29,7 @@ namespace pasta {
 // in distinct LDS banks.  Logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KROT_MAX + 55.
 constexpr int ILOG = (7 + TILE + 1 + 2 * HD_MAX + KROT_MAX + 55 + 31) & ~31;  // 4416 at 2032
 constexpr int ILDS = ILOG + ILOG / 4;
-// For the bf16 rotation the tile is kept as three bf16 arrays of ILOGB elements
+// For the f16 rotation the tile is kept as two f16 arrays (hi, lo) of ILOGB elements
 // (logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KB_MAX + 71).
 constexpr int ILOGB = (7 + TILE + 1 + 2 * HD_MAX + KB_MAX + 71 + 7) & ~7;     // 3008 at 2032
 constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 8 + 16 + 8;
@@ -80,18 +80,24 @@ __device__ __forceinline__ int obs_lower_bound(const double* __restrict__ fidx, 
     return lo;
 }
 
-// Three-way bf16 split x = hi + mid + lo (each round-to-nearest; both differences are
-// exact in f32), so that products of two split values keep f32 accuracy when the
-// three smallest cross terms are dropped (relative error <= ~2^-23).
-__device__ __forceinline__ void split3(float x, unsigned short& h, unsigned short& m,
-                                       unsigned short& l) {
-    const __bf16 bh = (__bf16)x;
-    const float r = x - (float)bh;
-    const __bf16 bm = (__bf16)r;
-    const __bf16 bl = (__bf16)(r - (float)bm);
-    h = __builtin_bit_cast(unsigned short, bh);
-    m = __builtin_bit_cast(unsigned short, bm);
-    l = __builtin_bit_cast(unsigned short, bl);
+// Two-way f16 split x = hi + lo (each round-to-nearest; x - hi is exact in f32): for
+// 2^-3 <= |x| < 2^14 both parts are normal f16 and hi + lo is x to 2^-22 relative; below,
+// lo's absolute error is at most 2^-25.  Products of two split values keep ~2^-21 relative
+// accuracy when the smallest cross term lo*lo (<= 2^-22 relative) is dropped.  Callers scale
+// x by a power of 2 first (exact), so that their largest |x| is just below 2^14.
+__device__ __forceinline__ void split2(float x, unsigned short& h, unsigned short& l) {
+    const _Float16 fh = (_Float16)x;
+    const _Float16 fl = (_Float16)(x - (float)fh);
+    h = __builtin_bit_cast(unsigned short, fh);
+    l = __builtin_bit_cast(unsigned short, fl);
+}
+
+// 2^(14 - e) for a positive bound b in [2^(e-1), 2^e): b * scale < 2^14
+__device__ __forceinline__ float pow2_below_2_14(float b) {
+    if (!(b > 0.0f)) return 1.0f;
+    int e;
+    frexpf(b, &e);
+    return ldexpf(1.0f, 14 - e);
 }
 
 // ---------------------------------------------------------------------------
@@ -123,6 +129,13 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
         const float f = (ij >= 0 && G.factors) ? G.factors[ij] : 1.0f / 65535.0f;
         ev.fw[lane] = ij >= 0 ? f * ev.w[lane] : 0.0f;
         ev.rowp[lane] = G.spectra + (ij >= 0 ? (long long)ij * G.pitch : 0LL);
+    }
+    // scale of the f16-split interpolated values: |sum fw u| <= 65535 sum |fw|
+    float rsc = 1.0f;
+    if (active) {
+        float a = ij >= 0 ? fabsf(ev.fw[lane]) : 0.0f;
+        for (int o = 32; o; o >>= 1) a += __shfl_xor(a, o, 64);
+        rsc = pow2_below_2_14(65535.0f * a * 1.0001f);
     }
     unsigned int hsh = 2166136261u;  // FNV-1a over the neighbour rows: fast reject in pairing
     for (int j = 0; j < 64; j++) hsh = (hsh ^ (unsigned int)__builtin_amdgcn_readlane(ij, j)) * 16777619u;
@@ -188,6 +201,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
         ev.fft = fft;
         s_fft = fft;
         ev.rows_hash = (int)hsh;
+        ev.rsc = rsc;
         // storage row of the first present neighbour
         ev.key = (st == ST_OK && present) ? first : G.geom->n_models;
         s_K = (st == ST_OK) ? krot : 0;
@@ -249,21 +263,27 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
             if (lane == 0) ev.krot = Kt;
         }
         if (Kt <= KB_MAX && !s_fft) {
-            // reversed, zero-padded taps tpad[16 + u] = h[Kt - 1 - u], split into bf16
-            // hi/mid/lo, as pairs tw[part][parity][q] = (tpad[2q + parity], tpad[2q + parity + 1])
+            // reversed, zero-padded taps tpad[16 + u] = h[Kt - 1 - u], scaled by a power of
+            // 2 (largest tap just below 2^14) and split into f16 hi/lo, as pairs
+            // tw[part][parity][q] = (tpad[2q + parity], tpad[2q + parity + 1])
             __syncthreads();
+            float tmax = 0.0f;
+            for (int k = lane; k < Kt; k += 64) tmax = fmaxf(tmax, h[k]);
+            for (int o = 32; o; o >>= 1) tmax = fmaxf(tmax, __shfl_xor(tmax, o, 64));
+            const float tsc = pow2_below_2_14(tmax);
+            if (lane == 0) ev.rinv = 1.0f / (rsc * tsc);  // a power of 2: exact
             unsigned int* tw = (unsigned int*)(h + KROT_MAX);
             const int nq = 16 * ((Kt + 46) >> 5) + 8;  // every pair the MFMA groups read
             for (int i = lane; i < 2 * nq; i += 64) {
                 const int par = i >= nq, q = i - par * nq;
-                unsigned short v[2][3];
+                unsigned short v[2][2];
 #pragma unroll
                 for (int z = 0; z < 2; z++) {
                     const int u = 2 * q + par + z - 16;
-                    split3((u >= 0 && u < Kt) ? h[Kt - 1 - u] : 0.0f, v[z][0], v[z][1], v[z][2]);
+                    split2((u >= 0 && u < Kt) ? h[Kt - 1 - u] * tsc : 0.0f, v[z][0], v[z][1]);
                 }
 #pragma unroll
-                for (int part = 0; part < 3; part++)
+                for (int part = 0; part < 2; part++)
                     tw[(2 * part + par) * TW_MAX + q] =
                         (unsigned int)v[0][part] | ((unsigned int)v[1][part] << 16);
             }
@@ -437,49 +457,46 @@ __device__ __forceinline__ void conv_rot_mfma(const float* Ipad, const float* hp
     }
 }
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4b __attribute__((ext_vector_type(4)));
 
-// Rotational broadening for K <= KB_MAX on v_mfma_f32_16x16x32_bf16, the same Toeplitz
-// product as conv_rot_mfma with 32 contraction steps per instruction.  Each operand is
-// split into bf16 hi + mid + lo and the six products hi*hi, hi*mid, mid*hi, hi*lo,
-// lo*hi, mid*mid are accumulated in f32 (the dropped terms are below 2^-23 relative).
-// Operand layout (checked on gfx950): A lane l holds A[l%16][8(l/16)+t], B lane l holds
-// B[8(l/16)+t][l%16], t < 8, D as for the f32 instruction.  Lane (m, kk) of group c
-// takes j = 32c + 8kk + t: its A operands are I[16a + 32c + 8kk .. +7] (one 16-byte read
-// per split part), its B operands the padded taps tpad[32c + 8kk + 16 - n .. +7]
-// (tpad[16 + u] = tr[u]), read as four packed pairs from the copy whose pairs start at
-// the parity of that index: tw[part][p][q] = (tpad[2q + p], tpad[2q + p + 1]).
-__device__ __forceinline__ void load_a3(const unsigned short* p, bf16x8& h, bf16x8& m, bf16x8& l) {
-    h = __builtin_bit_cast(bf16x8, *(const u16x8*)p);
-    m = __builtin_bit_cast(bf16x8, *(const u16x8*)(p + ILOGB));
-    l = __builtin_bit_cast(bf16x8, *(const u16x8*)(p + 2 * ILOGB));
+// Rotational broadening for K <= KB_MAX on v_mfma_f32_16x16x32_f16, the same Toeplitz
+// product as conv_rot_mfma with 32 contraction steps per instruction.  Both operands are
+// scaled by powers of 2 (interpolated values by the evaluation's rsc, taps so that the
+// largest is just below 2^14) and split into f16 hi + lo; the three products hi*hi, hi*lo,
+// lo*hi are accumulated in f32 and the output is scaled back by rinv (exact).  Relative
+// error per product ~3 * 2^-22 (the representations and the dropped lo*lo).  Operand
+// layout (checked on gfx950 for the bf16 instruction of the same shape): A lane l holds
+// A[l%16][8(l/16)+t], B lane l holds B[8(l/16)+t][l%16], t < 8, D as for the f32
+// instruction.  Lane (m, kk) of group c takes j = 32c + 8kk + t: its A operands are
+// I[16a + 32c + 8kk .. +7] (one 16-byte read per split part), its B operands the padded
+// taps tpad[32c + 8kk + 16 - n .. +7] (tpad[16 + u] = tr[u]), read as four packed pairs
+// from the copy whose pairs start at the parity of that index: tw[part][p][q] =
+// (tpad[2q + p], tpad[2q + p + 1]).
+__device__ __forceinline__ void load_a2(const unsigned short* p, f16x8& h, f16x8& l) {
+    h = __builtin_bit_cast(f16x8, *(const u16x8*)p);
+    l = __builtin_bit_cast(f16x8, *(const u16x8*)(p + ILOGB));
 }
 
-__device__ __forceinline__ void load_b3(const unsigned int* q, bf16x8& h, bf16x8& m, bf16x8& l) {
+__device__ __forceinline__ void load_b2(const unsigned int* q, f16x8& h, f16x8& l) {
     const u32x4b vh = {q[0], q[1], q[2], q[3]};
-    const u32x4b vm = {q[2 * TW_MAX], q[2 * TW_MAX + 1], q[2 * TW_MAX + 2], q[2 * TW_MAX + 3]};
-    const u32x4b vl = {q[4 * TW_MAX], q[4 * TW_MAX + 1], q[4 * TW_MAX + 2], q[4 * TW_MAX + 3]};
-    h = __builtin_bit_cast(bf16x8, vh);
-    m = __builtin_bit_cast(bf16x8, vm);
-    l = __builtin_bit_cast(bf16x8, vl);
+    const u32x4b vl = {q[2 * TW_MAX], q[2 * TW_MAX + 1], q[2 * TW_MAX + 2], q[2 * TW_MAX + 3]};
+    h = __builtin_bit_cast(f16x8, vh);
+    l = __builtin_bit_cast(f16x8, vl);
 }
 
-// c += the six split products of one 32-step group (smallest terms first)
-__device__ __forceinline__ f4 mfma6(const bf16x8& ah, const bf16x8& am, const bf16x8& al,
-                                    const bf16x8& bh, const bf16x8& bm, const bf16x8& bl, f4 c) {
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+// c += the three split products of one 32-step group (smallest terms first)
+__device__ __forceinline__ f4 mfma3(const f16x8& ah, const f16x8& al, const f16x8& bh,
+                                    const f16x8& bl, f4 c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, c, 0, 0, 0);
     return c;
 }
 
-__device__ __forceinline__ void conv_rot_bf16(const unsigned short* Ib3, const unsigned int* tw,
-                                              int K, int np, int off, float* out, bool mask,
-                                              long long x0, int W) {
+__device__ __forceinline__ void conv_rot_f16(const unsigned short* Ib2, const unsigned int* tw,
+                                             int K, int np, int off, float* out, bool mask,
+                                             long long x0, int W, float rinv) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nblk = (np + 15) >> 4;
     const int tiles = (nblk + 15) >> 4;
@@ -491,24 +508,24 @@ __device__ __forceinline__ void conv_rot_bf16(const unsigned short* Ib3, const u
         const int t1 = t0 + 4;
         const bool has1 = t1 < tiles;
         const int a0 = min(16 * t0 + m, nblk - 1), a1 = min(16 * t1 + m, nblk - 1);
-        const unsigned short* ia0 = Ib3 + 16 * a0 + 8 * kk;
-        const unsigned short* ia1 = Ib3 + 16 * a1 + 8 * kk;
+        const unsigned short* ia0 = Ib2 + 16 * a0 + 8 * kk;
+        const unsigned short* ia1 = Ib2 + 16 * a1 + 8 * kk;
         f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
         if (has1) {
             for (int c = 0; c < C; c++) {
-                bf16x8 bh, bm, bl, ah, am, al, xh, xm, xl;
-                load_b3(twp + 16 * c, bh, bm, bl);
-                load_a3(ia0 + 32 * c, ah, am, al);
-                load_a3(ia1 + 32 * c, xh, xm, xl);
-                c0 = mfma6(ah, am, al, bh, bm, bl, c0);
-                c1 = mfma6(xh, xm, xl, bh, bm, bl, c1);
+                f16x8 bh, bl, ah, al, xh, xl;
+                load_b2(twp + 16 * c, bh, bl);
+                load_a2(ia0 + 32 * c, ah, al);
+                load_a2(ia1 + 32 * c, xh, xl);
+                c0 = mfma3(ah, al, bh, bl, c0);
+                c1 = mfma3(xh, xl, bh, bl, c1);
             }
         } else {
             for (int c = 0; c < C; c++) {
-                bf16x8 bh, bm, bl, ah, am, al;
-                load_b3(twp + 16 * c, bh, bm, bl);
-                load_a3(ia0 + 32 * c, ah, am, al);
-                c0 = mfma6(ah, am, al, bh, bm, bl, c0);
+                f16x8 bh, bl, ah, al;
+                load_b2(twp + 16 * c, bh, bl);
+                load_a2(ia0 + 32 * c, ah, al);
+                c0 = mfma3(ah, al, bh, bl, c0);
             }
         }
 #pragma unroll
@@ -516,12 +533,12 @@ __device__ __forceinline__ void conv_rot_bf16(const unsigned short* Ib3, const u
             int p = 16 * (16 * t0 + 4 * kk + i) + m;
             if (p >= off && p < np) {
                 const long long x = x0 + (p - off);
-                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c0[i];
+                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c0[i] * rinv;
             }
             p = 16 * (16 * t1 + 4 * kk + i) + m;
             if (has1 && p >= off && p < np) {
                 const long long x = x0 + (p - off);
-                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c1[i];
+                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c1[i] * rinv;
             }
         }
     }
@@ -798,7 +815,7 @@ struct TileJob {
     long long rlo;   // first pixel of the rotation output R (clo - hd)
     int nC, hd, K, nR;
     bool mask;
-    int kind;        // 0 no rotation (tile into Rb), 1 bf16 rotation, 2 f32 rotation, 3 FFT
+    int kind;        // 0 no rotation (tile into Rb), 1 f16 rotation, 2 f32 rotation, 3 FFT
     int inst;        // instrumental stage: DISP_* of the star, DISP_NONE when composed
     long long g0;    // first interpolated 8-pixel group (multiple of 8)
     int off, ng;     // logical index of rlo - K/2 (or rlo); groups interpolated
@@ -850,19 +867,18 @@ __device__ __forceinline__ bool tile_job(const DevGrid& G, const DevStar* stars,
 __device__ __forceinline__ void store_group(const TileJob& J, int gl, const float (&acc)[8],
                                             float* Ib, float* Rb) {
     if (J.kind == 1) {
-        u16x8 h, m, l;
+        const float sc = J.ev->rsc;
+        u16x8 h, l;
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            unsigned short a, b, c;
-            split3(acc[t], a, b, c);
+            unsigned short a, b;
+            split2(acc[t] * sc, a, b);
             h[t] = a;
-            m[t] = b;
-            l[t] = c;
+            l[t] = b;
         }
         unsigned short* d = (unsigned short*)Ib + 8 * gl;
         *(u16x8*)d = h;
-        *(u16x8*)(d + ILOGB) = m;
-        *(u16x8*)(d + 2 * ILOGB) = l;
+        *(u16x8*)(d + ILOGB) = l;
     } else {
         float* d = J.kind == 2 ? Ib + islot(8 * gl) : J.kind == 3 ? Ib + 8 * gl : Rb + 8 * gl;
         *(f4*)d = f4{acc[0], acc[1], acc[2], acc[3]};
@@ -925,9 +941,9 @@ __device__ __forceinline__ void gather_rows_g(const DevGrid& G, const DevEval* c
 
 // taps of a job into LDS (and the zero padding the MFMA steps read past its tile); issued
 // before the interpolation so that their latency overlaps it
-// 16-byte quads of each of the six split-tap arrays that the bf16 MFMA groups of a K-tap
+// 16-byte quads of each of the four split-tap arrays that the f16 MFMA groups of a K-tap
 // kernel read: words [0, 16 * ceil((K + 15) / 32) + 8), from the highest B operand index
-// 20 + 16 (C - 1) + 3 of conv_rot_bf16
+// 20 + 16 (C - 1) + 3 of conv_rot_f16
 __device__ __forceinline__ int tap_quads(int K) { return (16 * ((K + 46) >> 5) + 8) >> 2; }
 
 // the used quads of a K-tap kernel's split taps, as (array, quad) -> index in the arrays
@@ -945,16 +961,15 @@ __device__ __forceinline__ void load_taps(const TileJob& J, const float* rot, fl
         if (with_tw) {
             const u32x4* src = (const u32x4*)(rot + (size_t)J.e * ROT_STRIDE + KROT_MAX);
             const int nq4 = tap_quads(J.K);
-            for (int i = tid; i < 6 * nq4; i += NT) {
+            for (int i = tid; i < 4 * nq4; i += NT) {
                 const int q = tap_quad_index(i, nq4);
                 ((u32x4*)tw)[q] = src[q];
             }
         }
-        unsigned short* Ib3 = (unsigned short*)Ib;
+        unsigned short* Ib2 = (unsigned short*)Ib;
         if (tid < 64) {  // the MFMA groups read past the tile
-            Ib3[8 * J.ng + tid] = 0;
-            Ib3[ILOGB + 8 * J.ng + tid] = 0;
-            Ib3[2 * ILOGB + 8 * J.ng + tid] = 0;
+            Ib2[8 * J.ng + tid] = 0;
+            Ib2[ILOGB + 8 * J.ng + tid] = 0;
         }
     } else if (J.kind == 2) {
         const float* h = rot + (size_t)J.e * ROT_STRIDE;
@@ -986,7 +1001,8 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
     // rotational broadening, outputs p = off + q for R index q
     float* R = Rb;
     if (J.kind == 1)
-        conv_rot_bf16((const unsigned short*)Ib, tw, J.K, J.off + J.nR, J.off, Rb, J.mask, J.rlo, G.W);
+        conv_rot_f16((const unsigned short*)Ib, tw, J.K, J.off + J.nR, J.off, Rb, J.mask, J.rlo, G.W,
+                     J.ev->rinv);
     else if (J.kind == 2)
         conv_rot_mfma(Ib, hrev, J.K, J.off + J.nR, J.off, Rb, J.mask, J.rlo, G.W);
     else if (FFT && J.kind == 3)
@@ -1086,13 +1102,14 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
     const DevEval& eb = evals[e1];
     const bool btw = GN == 2 && eb.krot > 0 && eb.krot <= KB_MAX && !eb.fft;
     const int bq4 = btw ? tap_quads(eb.krot) : 0;
-    u32x4 tkeep[3];
+    u32x4 tkeep[2];
+    static_assert(4 * ((16 * ((KB_MAX + 46) >> 5) + 8) >> 2) <= 2 * NT, "split taps in 2 rounds");
     if (GN == 2) {
         const u32x4* src = (const u32x4*)(rot + (size_t)e1 * ROT_STRIDE + KROT_MAX);
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
+        for (int r = 0; r < 2; r++) {
             const int i = threadIdx.x + r * NT;
-            tkeep[r] = i < 6 * bq4 ? src[tap_quad_index(i, bq4)] : u32x4{0u, 0u, 0u, 0u};
+            tkeep[r] = i < 4 * bq4 ? src[tap_quad_index(i, bq4)] : u32x4{0u, 0u, 0u, 0u};
         }
     }
     const DevEval* ev[GN];
@@ -1137,9 +1154,9 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
         tile_job(G, stars, evals, g == 1 ? e1 : e2, tile, tbound, tb_stride, mode, B);
         if (g == 1 && btw) {  // the second evaluation's split taps, read before A's rotation
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
+            for (int r = 0; r < 2; r++) {
                 const int i = threadIdx.x + r * NT;
-                if (i < 6 * bq4) ((u32x4*)tw)[tap_quad_index(i, bq4)] = tkeep[r];
+                if (i < 4 * bq4) ((u32x4*)tw)[tap_quad_index(i, bq4)] = tkeep[r];
             }
             load_taps(B, rot, Ib, hrev, tw, grev, false);
         } else {
@@ -1164,15 +1181,15 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
                                               const int* __restrict__ tbound, int tb_stride,
                                               float* __restrict__ model, int mode) {
     // Rotation stage storage, shared by its two forms: f32 (padded tile Ib, reversed taps
-    // hrev: zero, then the taps from index 16, zeros after) or bf16 x 3 (tile arrays,
+    // hrev: zero, then the taps from index 16, zeros after) or f16 x 2 (tile arrays,
     // split tap pairs tw).  Ib also holds the instrumental-broadening output.
-    static_assert(3 * ILOGB * 2 + 6 * TW_MAX * 4 <= (ILDS + KROT_MAX + 48) * 4, "LDS union");
+    static_assert(2 * ILOGB * 2 + 4 * TW_MAX * 4 <= (ILDS + KROT_MAX + 48) * 4, "LDS union");
     __shared__ __align__(16) float lds_rot[ILDS + KROT_MAX + 48];
     __shared__ float Rb[RLDS];
     __shared__ float grev[2 * HD_MAX + 1 + 8];
     float* const Ib = lds_rot;
     float* const hrev = lds_rot + ILDS;
-    unsigned int* const tw = (unsigned int*)(lds_rot + (3 * ILOGB) / 2);
+    unsigned int* const tw = (unsigned int*)(lds_rot + ILOGB);
 
     // tile-major over the work items (in neighbour-row order): the workgroups resident on
     // one XCD work on one tile of evaluations that gather overlapping rows
