@@ -141,7 +141,8 @@ struct DevEval {
     float rsc;       // f16 rotation: power of 2 the interpolated values are scaled by before
                      // their split (|value| * rsc < 2^14)
     float rinv;      // and 1 / (rsc * tap scale): undoes both scales on the rotation's output
-    int pad1;
+    int nrow;        // present neighbours: rowp/fw hold them first, in the neighbour order,
+                     // then rows of weight 0 (row 0) up to 64
     double shift_log;
     long long m_off;  // offset of this evaluation's resampled model in the model buffer
     long long mp_off; // offset of its continuum-moment partials (doubles)
@@ -149,8 +150,8 @@ struct DevEval {
     long long pf_off; // offset of its continuum parameters (floats)
     int idx[64];
     float w[64];
-    float fw[64];     // factor * weight of each neighbour, 0 where the neighbour is absent
-    const uint16_t* rowp[64];  // its grid row (row 0 where absent)
+    float fw[64];     // factor * weight of the present neighbours (nrow), then 0
+    const uint16_t* rowp[64];  // their grid rows, then row 0
 };
 
 // What the host sets in an evaluation record (the rest is written by k_prep); expanded

@@ -125,15 +125,30 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     // the gemv multiplies by the weight (interpolate.rs:962,966); one product per neighbour
     // differs from that by an f32 rounding per term
     const int ij = ev.idx[lane];
+    float fwj = 0.0f;
     if (active) {
+        // the present neighbours' rows and weights first, in the neighbour order (an absent
+        // neighbour's weight is 0, so leaving it out of the sum changes nothing), then rows of
+        // weight 0: the interpolation gathers only the present rows, rounded up to 8
         const float f = (ij >= 0 && G.factors) ? G.factors[ij] : 1.0f / 65535.0f;
-        ev.fw[lane] = ij >= 0 ? f * ev.w[lane] : 0.0f;
-        ev.rowp[lane] = G.spectra + (ij >= 0 ? (long long)ij * G.pitch : 0LL);
+        fwj = ij >= 0 ? f * ev.w[lane] : 0.0f;
+        const unsigned long long pm = __ballot(ij >= 0);
+        const int np = __popcll(pm);
+        const int pos = __popcll(pm & ((1ull << lane) - 1ull));
+        if (ij >= 0) {
+            ev.fw[pos] = fwj;
+            ev.rowp[pos] = G.spectra + (long long)ij * G.pitch;
+        }
+        if (lane >= np) {
+            ev.fw[lane] = 0.0f;
+            ev.rowp[lane] = G.spectra;
+        }
+        if (lane == 0) ev.nrow = np;
     }
     // scale of the f16-split interpolated values: |sum fw u| <= 65535 sum |fw|
     float rsc = 1.0f;
     if (active) {
-        float a = ij >= 0 ? fabsf(ev.fw[lane]) : 0.0f;
+        float a = fabsf(fwj);
         for (int o = 32; o; o >>= 1) a += __shfl_xor(a, o, 64);
         rsc = pow2_below_2_14(65535.0f * a * 1.0001f);
     }
@@ -324,26 +339,34 @@ __device__ __forceinline__ void acc8(float (&acc)[8], const u32x4 v, float fw, l
     }
 }
 
+template <bool FAST, int RF>
+__device__ __forceinline__ void gather_batch(const DevGrid& G, const DevEval& ev, int j0, int voff,
+                                             long long x0, float (&acc)[8]) {
+    const int nrec = (int)(G.pitch * 2);
+    u32x4 v[RF];
+#pragma unroll
+    for (int jj = 0; jj < RF; jj++) {
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)ev.rowp[j0 + jj], 0, nrec, 0x00020000);
+        v[jj] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
+    }
+#pragma unroll
+    for (int jj = 0; jj < RF; jj++) {
+        const float fw = __builtin_bit_cast(
+            float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, ev.fw[j0 + jj])));
+        acc8<FAST>(acc, v[jj], fw, x0, G.Wq);
+    }
+}
+
+// the present rows (rounded up to 8) in batches of 16 rows in flight, an 8-row batch last
 template <bool FAST>
 __device__ __forceinline__ void gather_rows(const DevGrid& G, const DevEval& ev, int voff, long long x0,
                                             float (&acc)[8]) {
-    const int nrec = (int)(G.pitch * 2);
+    const int n8 = (__builtin_amdgcn_readfirstlane(ev.nrow) + 7) & ~7;
+    int j0 = 0;
 #pragma unroll 1
-    for (int j0 = 0; j0 < 64; j0 += 16) {
-        u32x4 v[16];
-#pragma unroll
-        for (int jj = 0; jj < 16; jj++) {
-            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                (void*)ev.rowp[j0 + jj], 0, nrec, 0x00020000);
-            v[jj] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
-        }
-#pragma unroll
-        for (int jj = 0; jj < 16; jj++) {
-            const float fw = __builtin_bit_cast(
-                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, ev.fw[j0 + jj])));
-            acc8<FAST>(acc, v[jj], fw, x0, G.Wq);
-        }
-    }
+    for (; j0 + 16 <= n8; j0 += 16) gather_batch<FAST, 16>(G, ev, j0, voff, x0, acc);
+    if (j0 < n8) gather_batch<FAST, 8>(G, ev, j0, voff, x0, acc);
 }
 
 // Workgroup barrier that orders LDS accesses only: k_model's threads share nothing through
@@ -910,9 +933,9 @@ __device__ __forceinline__ void gather_rows_g(const DevGrid& G, const DevEval* c
                                               int voff, long long x0, float (&acc)[GN][8]) {
     const int nrec = (int)(G.pitch * 2);
     constexpr int RF = 8;  // rows in flight (registers)
-    static_assert(64 % RF == 0, "whole batches of the 64 neighbour rows");
+    const int n8 = (__builtin_amdgcn_readfirstlane(ev[0]->nrow) + 7) & ~7;  // present rows
 #pragma unroll 1
-    for (int j0 = 0; j0 < 64; j0 += RF) {
+    for (int j0 = 0; j0 < n8; j0 += RF) {
         u32x4 v[RF];
 #pragma unroll
         for (int jj = 0; jj < RF; jj++) {
