@@ -857,6 +857,7 @@ int batch_build(Batch& B, pasta_grid* g, const std::vector<pasta_star*>& stars,
     CHK(B.d_perm.alloc(ne));
     CHK(B.d_pair_eq.alloc(ne));
     CHK(B.d_items.alloc(ne));
+    CHK(B.d_union.alloc(ne));
     CHK(B.d_nitems.alloc(1));
     CHK(B.d_bmom.upload(bmom));
     CHK(B.d_solve.upload(solve));
@@ -905,7 +906,7 @@ int batch_launch(Batch& B) {
     const bool fft = may_fft(B.grid, B.max_vsini, B.stars);
     HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p,
                        B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.d_pair_eq.p, B.d_items.p,
-                       B.d_nitems.p, B.n_bm, B.d_bmom.p,
+                       B.d_nitems.p, B.d_union.p, B.n_bm, B.d_bmom.p,
                        B.d_bchi2.p, B.cls.data(), (int)B.cls.size(), B.d_solve.p, B.n_solve,
                        B.d_model.p, B.d_mpart.p, B.d_pf.p, B.d_cpart.p, B.d_chi2.p,
                        B.d_status.p, B.mode, fft, B.eager_events ? B.ev[0] : nullptr,

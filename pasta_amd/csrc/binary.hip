@@ -127,7 +127,7 @@ static int binary_launch(pasta_binary* b, const double* labels, int P, bool chi2
     HIPCHK(hipEventRecord(b->join, b->side));
     HIPCHK(launch_eval(G, Bc.d_stars.p, Bc.d_evals.p, Bc.n_evals, Bc.d_labels.p, Bc.d_rot.p,
                        Bc.d_tbound.p, Bc.tb_stride, Bc.d_hist.p, Bc.d_perm.p, Bc.d_pair_eq.p,
-                       Bc.d_items.p, Bc.d_nitems.p, Bc.n_bm, Bc.d_bmom.p, Bc.d_bchi2.p,
+                       Bc.d_items.p, Bc.d_nitems.p, Bc.d_union.p, Bc.n_bm, Bc.d_bmom.p, Bc.d_bchi2.p,
                        Bc.cls.data(), 0, Bc.d_solve.p, 0, Bc.d_model.p, Bc.d_mpart.p, Bc.d_pf.p,
                        Bc.d_cpart.p, Bc.d_chi2.p, Bc.d_status.p, 1,
                        may_fft(b->g, HUGE_VAL, Bc.stars), nullptr, nullptr, st));

@@ -154,6 +154,21 @@ struct DevEval {
     const uint16_t* rowp[64];  // their grid rows, then row 0
 };
 
+// Rows of a k_model work item of two or three evaluations (k_union): the union of their
+// present rows in storage order (= each one's neighbour order), every evaluation's weight
+// on each (0 where the row is not its neighbour), rows of weight 0 (row 0) up to a multiple
+// of 8.  n < 0: the union exceeds UNION_MAX rows; the evaluations then run one at a time.
+constexpr int UNION_MAX = 128;
+// consecutive evaluations (in neighbour-row order) share a work item when their present rows
+// have at most this many distinct rows together: the same cell, or cells one node apart
+constexpr int LINK_MAX = 80;
+struct DevUnion {
+    int n;
+    int pad[3];
+    const uint16_t* rowp[UNION_MAX];
+    float fw[3][UNION_MAX];
+};
+
 // What the host sets in an evaluation record (the rest is written by k_prep); expanded
 // into DevEval on the device by k_eval_init, so a batch uploads 40 bytes per evaluation.
 struct EvalInit {

@@ -19,7 +19,7 @@ namespace pasta {
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
                        int* sort_hist, int* perm, int* pair_eq, int4* items, int* n_items,
-                       int n_model,
+                       DevUnion* unions, int n_model,
                        const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
                        const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int mode, bool fft,
@@ -194,6 +194,7 @@ struct Batch {
     DBuf<float> d_rot, d_model, d_pf;
     DBuf<int> d_status, d_hist, d_perm, d_tbound, d_pair_eq, d_nitems;
     DBuf<int4> d_items;
+    DBuf<DevUnion> d_union;
     int tb_stride = 1;
     DBuf<int4> d_bmom, d_bchi2;
     DBuf<int2> d_solve;

@@ -926,21 +926,22 @@ __device__ __forceinline__ void gather_one(const DevGrid& G, const DevEval& ev, 
     }
 }
 
-// GN evaluations with the same rows: one gather and conversion, GN weight sets; each
-// evaluation's sum runs over the rows in the same order as alone
+// GN evaluations over the union of their rows: one gather and conversion per union row, GN
+// weight sets (0 where a row is not an evaluation's neighbour: fmaf(q, 0, acc) == acc, as no
+// partial sum is -0); each evaluation's nonzero terms run in the same order as alone
 template <int GN, bool FAST>
-__device__ __forceinline__ void gather_rows_g(const DevGrid& G, const DevEval* const (&ev)[GN],
-                                              int voff, long long x0, float (&acc)[GN][8]) {
+__device__ __forceinline__ void gather_rows_g(const DevGrid& G, const DevUnion& U, int voff,
+                                              long long x0, float (&acc)[GN][8]) {
     const int nrec = (int)(G.pitch * 2);
     constexpr int RF = 8;  // rows in flight (registers)
-    const int n8 = (__builtin_amdgcn_readfirstlane(ev[0]->nrow) + 7) & ~7;  // present rows
+    const int n8 = __builtin_amdgcn_readfirstlane(U.n);  // a multiple of 8
 #pragma unroll 1
     for (int j0 = 0; j0 < n8; j0 += RF) {
         u32x4 v[RF];
 #pragma unroll
         for (int jj = 0; jj < RF; jj++) {
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                (void*)ev[0]->rowp[j0 + jj], 0, nrec, 0x00020000);
+                (void*)U.rowp[j0 + jj], 0, nrec, 0x00020000);
             v[jj] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
         }
 #pragma unroll
@@ -949,7 +950,7 @@ __device__ __forceinline__ void gather_rows_g(const DevGrid& G, const DevEval* c
 #pragma unroll
             for (int g = 0; g < GN; g++)
                 f[g] = __builtin_bit_cast(
-                    float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, ev[g]->fw[j0 + jj])));
+                    float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, U.fw[g][j0 + jj])));
             float u[8];
             unpack8(v[jj], u);
 #pragma unroll
@@ -1111,16 +1112,17 @@ __device__ __forceinline__ void run_solo(const DevGrid& G, const DevStar* stars,
     finish_job<FFT>(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
 }
 
-// A work item of GN (2 or 3) evaluations with the same 64 neighbour rows whose tiles span
-// at most 2 NT 8-pixel groups together: one shared interpolation, A's groups to LDS at once,
+// A work item of GN (2 or 3) evaluations whose rows are the union U (k_union) and whose tiles
+// span at most 2 NT 8-pixel groups together: one shared interpolation, A's groups to LDS at once,
 // the others' kept in registers (and for a pair the second one's split taps, read now), then each
 // evaluation's rotation, instrumental broadening and resample in turn.  A's taps are in LDS.
 template <int GN>
 __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars, const DevEval* evals,
                                           const float* rot, int tile, const int* tbound,
                                           int tb_stride, int mode, float* model, const TileJob& A,
-                                          int e1, int e2, long long pg0, int png, float* Ib,
-                                          float* Rb, float* hrev, unsigned int* tw, float* grev) {
+                                          int e1, int e2, const DevUnion& U, long long pg0, int png,
+                                          float* Ib, float* Rb, float* hrev, unsigned int* tw,
+                                          float* grev) {
     // with three evaluations the registers go to the kept interpolations instead
     const DevEval& eb = evals[e1];
     const bool btw = GN == 2 && eb.krot > 0 && eb.krot <= KB_MAX && !eb.fft;
@@ -1135,10 +1137,6 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
             tkeep[r] = i < 4 * bq4 ? src[tap_quad_index(i, bq4)] : u32x4{0u, 0u, 0u, 0u};
         }
     }
-    const DevEval* ev[GN];
-    ev[0] = A.ev;
-    ev[1] = evals + e1;
-    if (GN > 2) ev[GN - 1] = evals + e2;
     float keep[GN - 1][2][8];
 #pragma unroll
     for (int r = 0; r < 2; r++) {
@@ -1152,8 +1150,8 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
             const long long x0 = pg0 + 8LL * gg;
             const bool inside = x0 >= 0 && x0 < G.W;
             const int voff = inside ? (int)(2 * x0) : OOB_OFFSET;
-            if (__all(x0 + 7 < G.Wq)) gather_rows_g<GN, true>(G, ev, voff, x0, acc);
-            else gather_rows_g<GN, false>(G, ev, voff, x0, acc);
+            if (__all(x0 + 7 < G.Wq)) gather_rows_g<GN, true>(G, U, voff, x0, acc);
+            else gather_rows_g<GN, false>(G, U, voff, x0, acc);
             if (x0 + 8 > G.W) {
 #pragma unroll
                 for (int t = 0; t < 8; t++)
@@ -1200,7 +1198,8 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
                                               const DevEval* __restrict__ evals,
                                               const float* __restrict__ rot,
                                               const int4* __restrict__ items,
-                                              const int* __restrict__ n_items, int n_evals,
+                                              const int* __restrict__ n_items,
+                                              const DevUnion* __restrict__ unions, int n_evals,
                                               const int* __restrict__ tbound, int tb_stride,
                                               float* __restrict__ model, int mode) {
     // Rotation stage storage, shared by its two forms: f32 (padded tile Ib, reversed taps
@@ -1256,18 +1255,19 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
         return;
     }
     if (hasA) load_taps(A, rot, Ib, hrev, tw, grev);
-    if (hasA && hasB) {
+    const DevUnion& U = unions[k];
+    if (hasA && hasB && __builtin_amdgcn_readfirstlane(U.n) >= 0) {
         long long pg0 = min(A.g0, bg0), pg1 = max(A.g0 + 8LL * A.ng, bg0 + 8LL * bng);
         if (hasC) {
             const long long q0 = min(pg0, cg0), q1 = max(pg1, cg0 + 8LL * cng);
             if (((q1 - q0) >> 3) <= 2 * NT) {
                 run_group<3>(G, stars, evals, rot, tile, tbound, tb_stride, mode, model, A, ids[1],
-                             ids[2], q0, (int)((q1 - q0) >> 3), Ib, Rb, hrev, tw, grev);
+                             ids[2], U, q0, (int)((q1 - q0) >> 3), Ib, Rb, hrev, tw, grev);
                 return;
             }
         } else if (((pg1 - pg0) >> 3) <= 2 * NT) {
             run_group<2>(G, stars, evals, rot, tile, tbound, tb_stride, mode, model, A, ids[1], -1,
-                         pg0, (int)((pg1 - pg0) >> 3), Ib, Rb, hrev, tw, grev);
+                         U, pg0, (int)((pg1 - pg0) >> 3), Ib, Rb, hrev, tw, grev);
             return;
         }
     }
@@ -2010,6 +2010,8 @@ __global__ void k_sort_scatter(const DevEval* __restrict__ evals, int n, int* __
 // pairing: adjacent evaluations in neighbour-row order that gather the same 64 rows
 // for the same synthetic tiles become one work item of k_model
 // ---------------------------------------------------------------------------
+// link[i] = 1: evaluations perm[i] and perm[i + 1] (both valid, same synthetic tiles) have at
+// most LINK_MAX distinct present rows together (their row lists are ascending: a merge count)
 __global__ void k_pair_eq(const DevEval* __restrict__ evals, const int* __restrict__ perm, int n,
                           int* __restrict__ eq) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2018,14 +2020,62 @@ __global__ void k_pair_eq(const DevEval* __restrict__ evals, const int* __restri
     if (i + 1 < n) {
         const DevEval& a = evals[perm[i]];
         const DevEval& b = evals[perm[i + 1]];
-        if (a.status == ST_OK && b.status == ST_OK && a.tile_base == b.tile_base &&
-            a.key == b.key && a.rows_hash == b.rows_hash) {
-            r = 1;
-            for (int j = 0; j < 64; j++)
-                if (a.idx[j] != b.idx[j]) { r = 0; break; }
+        if (a.status == ST_OK && b.status == ST_OK && a.tile_base == b.tile_base) {
+            const int na = a.nrow, nb = b.nrow;
+            int ia = 0, ib = 0, u = 0;
+            while ((ia < na || ib < nb) && u <= LINK_MAX) {
+                const uint16_t* pa = ia < na ? a.rowp[ia] : nullptr;
+                const uint16_t* pb = ib < nb ? b.rowp[ib] : nullptr;
+                if (pb == nullptr || (pa != nullptr && pa < pb)) ia++;
+                else if (pa == nullptr || pb < pa) ib++;
+                else { ia++; ib++; }
+                u++;
+            }
+            r = u <= LINK_MAX;
         }
     }
     eq[i] = r;
+}
+
+// The union of the present rows of every work item of two or three evaluations (one thread
+// per item): a three-way merge of their ascending row lists, each evaluation's weight on
+// each union row, padded with rows of weight 0 to a multiple of 8.
+__global__ void k_union(const DevEval* __restrict__ evals, const int4* __restrict__ items,
+                        const int* __restrict__ n_items, int n, DevUnion* __restrict__ unions,
+                        const uint16_t* row0) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || k >= *n_items) return;
+    const int4 it = items[k];
+    if (it.y < 0) return;
+    const DevEval* e[3] = {evals + it.x, evals + it.y, it.z >= 0 ? evals + it.z : nullptr};
+    const int ng = e[2] ? 3 : 2;
+    int pos[3] = {0, 0, 0}, cnt[3];
+    for (int g = 0; g < 3; g++) cnt[g] = g < ng ? e[g]->nrow : 0;
+    DevUnion& U = unions[k];
+    int m = 0;
+    while (true) {
+        const uint16_t* best = nullptr;
+        for (int g = 0; g < ng; g++)
+            if (pos[g] < cnt[g] && (best == nullptr || e[g]->rowp[pos[g]] < best)) best = e[g]->rowp[pos[g]];
+        if (best == nullptr) break;
+        if (m == UNION_MAX) {
+            U.n = -1;
+            return;
+        }
+        U.rowp[m] = best;
+        for (int g = 0; g < 3; g++) {
+            float w = 0.0f;
+            if (g < ng && pos[g] < cnt[g] && e[g]->rowp[pos[g]] == best) w = e[g]->fw[pos[g]++];
+            U.fw[g][m] = w;
+        }
+        m++;
+    }
+    const int m8 = (m + 7) & ~7;
+    for (; m < m8; m++) {
+        U.rowp[m] = row0;
+        for (int g = 0; g < 3; g++) U.fw[g][m] = 0.0f;
+    }
+    U.n = m8;
 }
 
 // inclusive block scans over 1024 threads (op: max or sum)
@@ -2055,8 +2105,8 @@ __device__ __forceinline__ int block_scan(int v, int* wsum /* 16 */) {
     return r;
 }
 
-// Greedy grouping along each run of equal neighbour rows: positions 0-2, 3-5, ... of a
-// run share their gathers; items[k] = (first evaluation, second or -1, third or -1, 0), in
+// Greedy grouping along each run of linked evaluations (k_pair_eq): positions 0-2, 3-5, ... of
+// a run share their gathers; items[k] = (first evaluation, second or -1, third or -1, 0), in
 // neighbour-row order.
 __global__ __launch_bounds__(1024) void k_pair_items(const int* __restrict__ perm,
                                                      const int* __restrict__ eq, int n,
@@ -2098,7 +2148,7 @@ hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const 
 hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, int n_evals,
                        const double* labels, float* rot, int* tbound, int tb_stride,
                        int* sort_hist, int* perm, int* pair_eq, int4* items, int* n_items,
-                       int n_model,
+                       DevUnion* unions, int n_model,
                        const int4* bmom, const int4* bchi2, const ContClass* cls, int n_cls,
                        const int2* solve_items, int n_solve, float* model, double* mpart, float* pf,
                        double* cpart, double* chi2, int* status, int mode, bool fft,
@@ -2121,13 +2171,15 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
                        n_evals, pair_eq);
     hipLaunchKernelGGL(k_pair_items, dim3(1), dim3(1024), 0, st, perm, pair_eq, n_evals, items,
                        n_items);
+    hipLaunchKernelGGL(k_union, dim3((n_evals + 255) / 256), dim3(256), 0, st, evals, items, n_items,
+                       n_evals, unions, G.spectra);
     if (ev_model0) (void)hipEventRecord(ev_model0, st);
     if (n_model > 0 && fft)
         hipLaunchKernelGGL(k_model<true>, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot,
-                           items, n_items, n_evals, tbound, tb_stride, model, mode);
+                           items, n_items, unions, n_evals, tbound, tb_stride, model, mode);
     else if (n_model > 0)
         hipLaunchKernelGGL(k_model<false>, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot,
-                           items, n_items, n_evals, tbound, tb_stride, model, mode);
+                           items, n_items, unions, n_evals, tbound, tb_stride, model, mode);
     if (ev_model1) (void)hipEventRecord(ev_model1, st);
     return launch_cont(stars, evals, n_evals, bmom, bchi2, cls, n_cls, solve_items, n_solve,
                        model, mpart, pf, cpart, chi2, status, st);

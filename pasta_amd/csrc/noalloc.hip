@@ -128,6 +128,7 @@ struct Bufs {
     float *rot, *model, *pf;
     int *status, *hist, *perm, *tbound, *pair_eq, *nitems;
     int4 *items, *bmom, *bchi2;
+    DevUnion* unions;
     int2* solve;
 };
 
@@ -143,6 +144,7 @@ size_t carve(Carve& c, const Layout& L, const pasta_star* s, Bufs& b) {
     b.perm = c.take<int>(P);
     b.pair_eq = c.take<int>(P);
     b.items = c.take<int4>(P);
+    b.unions = c.take<DevUnion>(P);
     b.nitems = c.take<int>(1);
     b.model = c.take<float>(P * (size_t)L.m_stride);
     b.mpart = c.take<double>(P * (size_t)L.mp_stride);
@@ -190,7 +192,7 @@ int pasta_eval_chi2_noalloc(pasta_star* s, const double* labels, int P, double* 
     const ContClass cls{D.cont_kind, p1, 0, L.n_bmom, 0, L.n_bchi2};
     const DevGrid G = s->grid->dev();
     HIPCHK(launch_eval(G, b.star, b.evals, P, b.labels, b.rot, b.tbound, D.n_tiles + 1, b.hist,
-                       b.perm, b.pair_eq, b.items, b.nitems, D.n_tiles * P, b.bmom, b.bchi2, &cls, 1,
+                       b.perm, b.pair_eq, b.items, b.nitems, b.unions, D.n_tiles * P, b.bmom, b.bchi2, &cls, 1,
                        b.solve, L.n_solve, b.model, b.mpart, b.pf, b.cpart, b.chi2, b.status, 0,
                        may_fft(s->grid, max_label_vsini(labels, P), composed_extra_taps(s)), nullptr, nullptr, st));
     HIPCHK(hipMemcpyAsync(chi2, b.chi2, sizeof(double) * (size_t)P, hipMemcpyDeviceToHost, st));
