@@ -577,7 +577,7 @@ def main():
             "l2_to_cu_bytes_per_launch": pmc.get("l2_to_cu_bytes_per_launch"),
             "iteration_ms": round(iter_ms, 3),
             # PMC of the same iterations: matrix-core busy fraction and rate of the rotation
-            # (bf16 x 3) in k_model, VALU busy fraction of k_model
+            # (f16 x 2) in k_model, VALU busy fraction of k_model
             "pmc": {k: v for k, v in pmc.items() if k not in
                     ("traffic_bytes_per_launch", "l2_to_cu_bytes_per_launch")} or None,
         },

@@ -161,7 +161,10 @@ struct DevEval {
 constexpr int UNION_MAX = 128;
 // consecutive evaluations (in neighbour-row order) share a work item when their present rows
 // have at most this many distinct rows together: the same cell, or cells one node apart
-constexpr int LINK_MAX = 80;
+#ifndef PASTA_LINK_MAX
+#define PASTA_LINK_MAX 80
+#endif
+constexpr int LINK_MAX = PASTA_LINK_MAX;
 struct DevUnion {
     int n;
     int pad[3];
