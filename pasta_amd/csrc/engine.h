@@ -27,18 +27,18 @@ constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
 constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)
 // Rotation kernels of up to KB_MAX taps run on f16 matrix cores with a two-way operand
 // split; their taps are kept as 2 split parts x 2 alignments x TW_MAX packed f16 pairs.
-constexpr int KB_MAX = 640;
-constexpr int TW_MAX = (32 * ((KB_MAX + 15 + 31) / 32) + 32) / 2;  // 352
+constexpr int KB_MAX = 1536;
+constexpr int TW_MAX = (32 * ((KB_MAX + 15 + 31) / 32) + 32) / 2;  // 800
 // per-evaluation rotation buffer: f32 taps, then the split pairs (or the kernel's spectrum,
 // 2049 complex values, for the FFT path)
-constexpr int ROT_STRIDE = KROT_MAX + 6 * TW_MAX;
+constexpr int ROT_STRIDE = KROT_MAX + 4 * TW_MAX;
 // Default crossover of the rotation stage: kernels of at least this many taps are convolved
 // by FFT (if the tile and kernel fit one 4096-point transform), shorter ones directly on the
 // matrix cores.  Measured on the batched workload (128 stars x 64 particles, full grid): the
-// FFT stage costs about the same at any size, the bf16 x 3 products (up to KB_MAX taps) stay
-// cheaper, and the f32 products above KB_MAX are overtaken at ~677 taps.  Set per grid with
+// FFT stage costs about the same at any size, the f16 x 2 products stay cheaper up to
+// KB_MAX taps, and the f32 products above KB_MAX cost more than the FFT.  Set per grid with
 // pasta_grid_set_fft_taps.
-constexpr int FFT_MIN_TAPS = 677;
+constexpr int FFT_MIN_TAPS = KB_MAX + 1;
 constexpr int FFT_N = 4096;       // real points of the FFT path's transforms
 constexpr int FFT_M = FFT_N / 2;  // their complex half-length
 // A K-tap kernel whose tiles carry hd instrumental half-width pixels on each side fits one

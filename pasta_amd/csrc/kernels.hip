@@ -1125,10 +1125,12 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
                                           float* grev) {
     // with three evaluations the registers go to the kept interpolations instead
     const DevEval& eb = evals[e1];
-    const bool btw = GN == 2 && eb.krot > 0 && eb.krot <= KB_MAX && !eb.fft;
+    // (kernels of up to 640 taps: their split taps are 2 rounds of 16-byte loads)
+    constexpr int KB_PREFETCH = 640;
+    const bool btw = GN == 2 && eb.krot > 0 && eb.krot <= KB_PREFETCH && !eb.fft;
     const int bq4 = btw ? tap_quads(eb.krot) : 0;
     u32x4 tkeep[2];
-    static_assert(4 * ((16 * ((KB_MAX + 46) >> 5) + 8) >> 2) <= 2 * NT, "split taps in 2 rounds");
+    static_assert(4 * ((16 * ((KB_PREFETCH + 46) >> 5) + 8) >> 2) <= 2 * NT, "split taps in 2 rounds");
     if (GN == 2) {
         const u32x4* src = (const u32x4*)(rot + (size_t)e1 * ROT_STRIDE + KROT_MAX);
 #pragma unroll

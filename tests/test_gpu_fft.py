@@ -105,17 +105,32 @@ def test_fft_chi2_and_batch_identity(small, gpu_small):
     assert rel.max() < RTOL
 
 
-CROSSOVER = 677  # the default: composed taps (rotation + 6 instrumental) from which FFT runs
+CROSSOVER = 1537  # the default: composed taps (rotation + 6 instrumental) from which FFT runs
+
+
+def _vsini_for_taps(dvelo, target):
+    """Smallest vsini whose rotation kernel has at least `target` taps (bisection)."""
+    lo, hi = 1.0, 1500.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if len(O.rotation_kernel(mid, dvelo)) >= target:
+            hi = mid
+        else:
+            lo = mid
+    return hi
 
 
 def test_fft_default_crossover(small, gpu_small):
-    """With the default crossover, a kernel of 677 taps or more (the rotation kernel plus the
-    6 the composed R = 85,000 instrumental kernel adds) runs on the FFT path and a shorter one
-    directly: each result equals the forced path's bit for bit."""
+    """With the default crossover, a kernel of 1,537 taps or more (the rotation kernel plus
+    the 6 the composed R = 85,000 instrumental kernel adds; above the f16 path's 1,536) runs
+    on the FFT path and a shorter one directly: each result equals the forced path's bit for
+    bit."""
     pasta, wlg, interp = gpu_small
     d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
     dvelo = np.log(10.0) * small.spec.log10_step
-    labs = np.array([[6123.0, -0.3, 4.21, v, 5.0] for v in (150.0, 352.0, 365.0, 374.0, 600.0)])
+    vs = [150.0, _vsini_for_taps(dvelo, 1521), _vsini_for_taps(dvelo, 1531),
+          _vsini_for_taps(dvelo, 1541), 1000.0]
+    labs = np.array([[6123.0, -0.3, 4.21, v, 5.0] for v in vs])
     ks = [len(O.rotation_kernel(v, dvelo)) + 6 for v in labs[:, 3]]
     assert min(ks) < CROSSOVER <= max(ks) and sum(k >= CROSSOVER for k in ks) >= 2, ks
     default = interp.produce_model_bulk(d, labs)

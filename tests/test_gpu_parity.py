@@ -123,17 +123,23 @@ def test_interpolate_and_convolve_matches_oracle(small, gpu_small):
 
 
 def test_rotation_paths_match_oracle(small, gpu_small):
-    """Rotation kernels of up to 640 taps run on bf16 matrix cores (three-way split),
-    wider ones on f32 matrix cores: vsini values on both sides of the switch (K ~ 626,
-    648 taps on this grid) and far beyond it (K ~ 1100, 1840)."""
+    """Rotation kernels of up to 1,536 taps (composed with the instrumental kernel) run on
+    f16 matrix cores (two-way split), wider ones on f32 matrix cores (the FFT path off):
+    kernels on both sides of the switch (1,521 and 1,541 rotation taps, 1,527 and 1,547
+    composed) and well inside each (~626 and ~1,840 taps on this grid)."""
     pasta, wlg, interp = gpu_small
     d = pasta.FixedResolutionDispersion(small.wl, 85_000.0, wlg)
-    labs = np.array([[6123.0, -0.3, 4.21, v, 5.0] for v in (340.0, 352.0, 600.0, 1000.0)])
     dvelo = np.log(10.0) * small.spec.log10_step
+    vs = [340.0, _vsini_for_taps(dvelo, 1521), _vsini_for_taps(dvelo, 1541), 1000.0]
+    labs = np.array([[6123.0, -0.3, 4.21, v, 5.0] for v in vs])
     ks = [len(O.rotation_kernel(v, dvelo)) for v in labs[:, 3]]
-    assert ks[0] <= 640 < ks[1] and ks[3] > 1500, ks
+    assert ks[0] < 1521 and ks[1] == 1521 and ks[2] == 1541 and ks[3] > 1541, ks
+    interp.set_fft_taps(1 << 20)
     print(f"rotation paths: taps {ks}", flush=True)
-    out = interp.produce_model_bulk(d, labs)
+    try:
+        out = interp.produce_model_bulk(d, labs)
+    finally:
+        interp.set_fft_taps(0)
     for k, lab in enumerate(labs):
         ref = O.produce_model(small.og, small.odisp, lab)
         assert _rel(out[k], ref) < RTOL, (lab.tolist(), _rel(out[k], ref))
