@@ -2013,10 +2013,14 @@ __global__ void k_sort_scatter(const DevEval* __restrict__ evals, int n, int* __
 // for the same synthetic tiles become one work item of k_model
 // ---------------------------------------------------------------------------
 // link[i] = 1: evaluations perm[i] and perm[i + 1] (both valid, same synthetic tiles) have at
-// most LINK_MAX distinct present rows together (their row lists are ascending: a merge count)
-__global__ void k_pair_eq(const DevEval* __restrict__ evals, const int* __restrict__ perm, int n,
-                          int* __restrict__ eq) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// most LINK_MAX distinct present rows together.  One wavefront per pair: lane l holds the l-th
+// present row of each (ascending lists), finds its row of the first among the second's by a
+// binary search over the lanes, and |A u B| = |A| + |B| - |A n B|.
+__global__ __launch_bounds__(64) void k_pair_eq(const DevEval* __restrict__ evals,
+                                                const int* __restrict__ perm, int n,
+                                                int* __restrict__ eq) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
     if (i >= n) return;
     int r = 0;
     if (i + 1 < n) {
@@ -2024,60 +2028,98 @@ __global__ void k_pair_eq(const DevEval* __restrict__ evals, const int* __restri
         const DevEval& b = evals[perm[i + 1]];
         if (a.status == ST_OK && b.status == ST_OK && a.tile_base == b.tile_base) {
             const int na = a.nrow, nb = b.nrow;
-            int ia = 0, ib = 0, u = 0;
-            while ((ia < na || ib < nb) && u <= LINK_MAX) {
-                const uint16_t* pa = ia < na ? a.rowp[ia] : nullptr;
-                const uint16_t* pb = ib < nb ? b.rowp[ib] : nullptr;
-                if (pb == nullptr || (pa != nullptr && pa < pb)) ia++;
-                else if (pa == nullptr || pb < pa) ib++;
-                else { ia++; ib++; }
-                u++;
+            const unsigned long long pa = lane < na ? (unsigned long long)a.rowp[lane] : ~0ull;
+            const unsigned long long pb = lane < nb ? (unsigned long long)b.rowp[lane] : ~0ull;
+            // first lane of b whose row is >= pa (b's rows ascending, padded with ~0)
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step; step >>= 1) {
+                const unsigned long long v = __shfl(pb, lo + step - 1, 64);
+                if (v < pa) lo += step;
             }
-            r = u <= LINK_MAX;
+            const bool found = lane < na && lo < nb && __shfl(pb, lo, 64) == pa;
+            const int inter = __popcll(__ballot(found));
+            r = na + nb - inter <= LINK_MAX;
         }
     }
-    eq[i] = r;
+    if (lane == 0) eq[i] = r;
 }
 
-// The union of the present rows of every work item of two or three evaluations (one thread
-// per item): a three-way merge of their ascending row lists, each evaluation's weight on
-// each union row, padded with rows of weight 0 to a multiple of 8.
-__global__ void k_union(const DevEval* __restrict__ evals, const int4* __restrict__ items,
-                        const int* __restrict__ n_items, int n, DevUnion* __restrict__ unions,
-                        const uint16_t* row0) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n || k >= *n_items) return;
+// The union of the present rows of every work item of two or three evaluations, one wavefront
+// per item: the rows (storage indices) of all its evaluations set bits of a bitmap in LDS over
+// [first row, first row + 2048); a row's union position is the number of set bits below it
+// (prefix popcounts over the 64 words); every evaluation writes its weight at its rows'
+// positions, 0 elsewhere; rows of weight 0 pad the list to a multiple of 8.
+__global__ __launch_bounds__(64) void k_union(const DevEval* __restrict__ evals,
+                                              const int4* __restrict__ items,
+                                              const int* __restrict__ n_items,
+                                              DevUnion* __restrict__ unions,
+                                              const uint16_t* spectra, long long pitch) {
+    constexpr int SPAN = 2048;
+    __shared__ unsigned int bits[SPAN / 32];
+    const int k = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (k >= *n_items) return;
     const int4 it = items[k];
     if (it.y < 0) return;
-    const DevEval* e[3] = {evals + it.x, evals + it.y, it.z >= 0 ? evals + it.z : nullptr};
-    const int ng = e[2] ? 3 : 2;
-    int pos[3] = {0, 0, 0}, cnt[3];
-    for (int g = 0; g < 3; g++) cnt[g] = g < ng ? e[g]->nrow : 0;
+    const int ids[3] = {it.x, it.y, it.z};
+    const int ng = it.z >= 0 ? 3 : 2;
+    int row[3];
+    int lo = 0x7fffffff;
+#pragma unroll
+    for (int g = 0; g < 3; g++) {
+        row[g] = -1;
+        if (g < ng && lane < evals[ids[g]].nrow)
+            row[g] = (int)((evals[ids[g]].rowp[lane] - spectra) / pitch);
+        if (row[g] >= 0) lo = min(lo, row[g]);
+    }
+    for (int o = 32; o; o >>= 1) lo = min(lo, __shfl_xor(lo, o, 64));
+    bits[lane] = 0u;
+    __syncthreads();
+    bool wide = false;
+#pragma unroll
+    for (int g = 0; g < 3; g++)
+        if (row[g] >= 0) {
+            const int x = row[g] - lo;
+            if (x >= SPAN) wide = true;
+            else atomicOr(&bits[x >> 5], 1u << (x & 31));
+        }
     DevUnion& U = unions[k];
-    int m = 0;
-    while (true) {
-        const uint16_t* best = nullptr;
-        for (int g = 0; g < ng; g++)
-            if (pos[g] < cnt[g] && (best == nullptr || e[g]->rowp[pos[g]] < best)) best = e[g]->rowp[pos[g]];
-        if (best == nullptr) break;
-        if (m == UNION_MAX) {
-            U.n = -1;
-            return;
-        }
-        U.rowp[m] = best;
-        for (int g = 0; g < 3; g++) {
-            float w = 0.0f;
-            if (g < ng && pos[g] < cnt[g] && e[g]->rowp[pos[g]] == best) w = e[g]->fw[pos[g]++];
-            U.fw[g][m] = w;
-        }
-        m++;
+    if (__any(wide)) {  // rows too far apart for the bitmap: one evaluation at a time
+        if (lane == 0) U.n = -1;
+        return;
+    }
+    __syncthreads();
+    const unsigned int w = bits[lane];
+    int pre = __popc(w);  // inclusive prefix of the set bits per word
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(pre, o, 64);
+        if (lane >= o) pre += t;
+    }
+    const int m = __shfl(pre, 63, 64);
+    if (m > UNION_MAX) {
+        if (lane == 0) U.n = -1;
+        return;
     }
     const int m8 = (m + 7) & ~7;
-    for (; m < m8; m++) {
-        U.rowp[m] = row0;
-        for (int g = 0; g < 3; g++) U.fw[g][m] = 0.0f;
+    for (int q = lane; q < UNION_MAX; q += 64) {
+#pragma unroll
+        for (int g = 0; g < 3; g++) U.fw[g][q] = 0.0f;
+        if (q >= m) U.rowp[q] = spectra;
     }
-    U.n = m8;
+    __syncthreads();  // the zeros land before the weights (the barrier waits for the stores)
+    __shared__ int wpre[SPAN / 32];  // exclusive prefix per word
+    wpre[lane] = pre - __popc(w);
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < 3; g++)
+        if (row[g] >= 0) {
+            const int x = row[g] - lo;
+            const int pos = wpre[x >> 5] + __popc(bits[x >> 5] & ((1u << (x & 31)) - 1u));
+            U.rowp[pos] = evals[ids[g]].rowp[lane];
+            U.fw[g][pos] = evals[ids[g]].fw[lane];
+        }
+    if (lane == 0) U.n = m8;
 }
 
 // inclusive block scans over 1024 threads (op: max or sum)
@@ -2169,12 +2211,11 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, sort_hist, nb);
     hipLaunchKernelGGL(k_sort_scatter, dim3((n_evals + 255) / 256), dim3(256), 0, st, evals,
                        n_evals, sort_hist, perm);
-    hipLaunchKernelGGL(k_pair_eq, dim3((n_evals + 255) / 256), dim3(256), 0, st, evals, perm,
-                       n_evals, pair_eq);
+    hipLaunchKernelGGL(k_pair_eq, dim3(n_evals), dim3(64), 0, st, evals, perm, n_evals, pair_eq);
     hipLaunchKernelGGL(k_pair_items, dim3(1), dim3(1024), 0, st, perm, pair_eq, n_evals, items,
                        n_items);
-    hipLaunchKernelGGL(k_union, dim3((n_evals + 255) / 256), dim3(256), 0, st, evals, items, n_items,
-                       n_evals, unions, G.spectra);
+    hipLaunchKernelGGL(k_union, dim3(n_evals), dim3(64), 0, st, evals, items, n_items, unions,
+                       G.spectra, G.pitch);
     if (ev_model0) (void)hipEventRecord(ev_model0, st);
     if (n_model > 0 && fft)
         hipLaunchKernelGGL(k_model<true>, dim3(n_model), dim3(NT), 0, st, G, stars, evals, rot,
