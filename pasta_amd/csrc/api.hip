@@ -903,6 +903,7 @@ double max_label_vsini(const double* labels5, int n) {
 
 int batch_launch(Batch& B) {
     DevGrid G = B.grid->dev();
+    G.stream_models = B.model_len * (long long)sizeof(float) > MALL_BYTES;
     const bool fft = may_fft(B.grid, B.max_vsini, B.stars);
     HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p,
                        B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.d_pair_eq.p, B.d_items.p,

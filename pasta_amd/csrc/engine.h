@@ -93,7 +93,11 @@ struct DevGrid {
     double synth_first_wl, synth_last_wl;  // 10^first, 10^(first + step*W)
     int n_models_bins;        // n_models + 1 (sort bins; the last holds failed evaluations)
     int fft_min_taps;         // rotation kernels of at least this many taps use the FFT path
+    int stream_models;        // 1: k_model writes the resampled models past the caches (a
+                              // launch whose models exceed the MALL; per launch)
 };
+// the MI355X's last-level (MALL) cache: launches with more model bytes stream them
+constexpr long long MALL_BYTES = 256LL << 20;
 
 struct DevStar {
     int N, disp_kind, Kd, cont_kind;

@@ -1013,6 +1013,40 @@ __device__ __forceinline__ void load_taps(const TileJob& J, const float* rot, fl
     }
 }
 
+// RV shift + resample (f32-quantised weight, convolve_rv.rs:569-573) of the job's observed
+// pixels from the convolved tile Cb.  floor(idx) clamped at 0 (`as usize`) is an integer below
+// 2^31: exact in f64 and in a 32-bit int, and (float) of it rounds the same integer as (float)
+// of a 64-bit index.  Four float indices in flight per thread (global loads, not waited on by
+// LDS reads).
+template <bool STREAM>
+__device__ __forceinline__ void resample(const TileJob& J, const DevStar& S, const float* Cb,
+                                         float* out) {
+    const int tid = threadIdx.x;
+    const double sl = J.ev->shift_log;
+    const int clo = (int)J.clo, ihi = J.ihi;
+    const GlobalD* fx = (const GlobalD*)S.fidx;
+    for (int i0 = J.ilo + tid; i0 < ihi; i0 += 4 * NT) {
+        double idx[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) idx[u] = i0 + u * NT < ihi ? fx[i0 + u * NT] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * NT;
+            if (i < ihi) {
+                const double x = idx[u] + sl;
+                const double f = floor(x);
+                const double jf = f > 0.0 ? f : 0.0;
+                const int j = (int)jf - clo;
+                const float w = (float)x - (float)jf;
+                const float a = Cb[j], c = Cb[j + 1];
+                const float v = (1.0f - w) * a + w * c;
+                if (STREAM) __builtin_nontemporal_store(v, out + i);
+                else out[i] = v;
+            }
+        }
+    }
+}
+
 // rotation, instrumental broadening and resample of a job whose interpolated tile and
 // taps are in LDS (stored but not yet synchronised)
 template <bool FFT = false>
@@ -1069,30 +1103,10 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
         for (int q = tid; q < nC; q += NT) out[J.clo + q] = Cb[q];
         return;
     }
-    // floor(idx) clamped at 0 (`as usize`) is an integer below 2^31: exact in f64 and in a
-    // 32-bit int, and (float) of it rounds the same integer as (float) of a 64-bit index.
-    // Four float indices in flight per thread (global loads, not waited on by LDS reads).
-    const double sl = J.ev->shift_log;
-    const int clo = (int)J.clo, ihi = J.ihi;
-    const GlobalD* fx = (const GlobalD*)S.fidx;
-    for (int i0 = J.ilo + tid; i0 < ihi; i0 += 4 * NT) {
-        double idx[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) idx[u] = i0 + u * NT < ihi ? fx[i0 + u * NT] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int i = i0 + u * NT;
-            if (i < ihi) {
-                const double x = idx[u] + sl;
-                const double f = floor(x);
-                const double jf = f > 0.0 ? f : 0.0;
-                const int j = (int)jf - clo;
-                const float w = (float)x - (float)jf;
-                const float a = Cb[j], c = Cb[j + 1];
-                out[i] = (1.0f - w) * a + w * c;
-            }
-        }
-    }
+    // Models larger than the MALL are read back from HBM anyway: streamed past the caches
+    // (non-temporal stores), they do not evict the grid rows the interpolation re-reads.
+    if (G.stream_models) resample<true>(J, S, Cb, out);
+    else resample<false>(J, S, Cb, out);
 }
 
 // evaluation e alone on this tile (its job exists; the LDS is free)
