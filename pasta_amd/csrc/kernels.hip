@@ -40,6 +40,7 @@ constexpr double C_KMS = 299792.0;   // src/convolve_rv.rs:440,547
 constexpr double LN10 = 2.302585092994045684017991454684364208;
 
 typedef const __attribute__((address_space(1))) double GlobalD;
+typedef const __attribute__((address_space(1))) float GlobalF;
 
 __device__ __forceinline__ long long jfloor(double x) {
     double f = floor(x);
@@ -1412,14 +1413,18 @@ __device__ __forceinline__ float dot_na(const float (&a)[PM], const float* b, in
 // (continuum_fitting.rs:490-492): the f32 quotients f/m, added one after another in pixel
 // order (`target.iter().sum::<f32>()`, a sequential fold), divided by n in f32.  A
 // workgroup covers the E <= 16 evaluations of one star in block descriptor (first
-// evaluation, count, 0, 0).  Waves 1-3 form the quotients of 256-pixel chunks (IEEE
-// divisions; each thread issues all its flux and model reads of a chunk before its first
-// division, so ~22 reads are in flight per thread) into a double-buffered LDS tile, one row
-// per evaluation; lane e < E of wave 0 adds row e's 256 values in order while the next
-// chunk is being formed.  Past the last pixel the tile holds zeros: adding +0 leaves a sum
-// unchanged (but for the sign of a zero sum).
-constexpr int FOLD_CH = 256;      // pixels per chunk
-constexpr int FOLD_ROW = 260;     // LDS row stride: 16-byte rows, 260 = 4 mod 64 banks, so
+// evaluation, count, 0, 0).  The star's pixels go in chunks of 192, one per thread of
+// waves 1-3: the thread reads its pixel's flux and the E models' values there, forms the E
+// quotients (IEEE divisions) and stores them in a double-buffered LDS tile, one row per
+// evaluation; lane e < E of wave 0 adds row e's 192 values in order while the next chunk
+// is being formed.  A thread's reads of a chunk (1 + E) are issued four chunks before its
+// divisions, unconditionally (at a clamped pixel past the star's end), so no read waits on
+// another and the chain of additions sets the pace; the barriers order the LDS tile only.
+// Past the last pixel the tile holds zeros: adding +0 leaves a sum unchanged (but for the
+// sign of a zero sum).
+constexpr int FOLD_CH = NT - 64;  // pixels per chunk: one per producer thread
+constexpr int FOLD_SETS = 4;     // register sets of reads in flight per producer thread
+constexpr int FOLD_ROW = 196;     // LDS row stride: 16-byte rows, 196 = 4 mod 64 banks, so
                                   // 16 lanes' 16-byte reads cover the 64 banks once
 template <int E>
 __global__ __launch_bounds__(NT) void k_const_fold(const DevStar* __restrict__ stars,
@@ -1428,8 +1433,7 @@ __global__ __launch_bounds__(NT) void k_const_fold(const DevStar* __restrict__ s
                                                    const float* __restrict__ model,
                                                    float* __restrict__ pf) {
     static_assert(E <= 16, "one lane of wave 0 per evaluation, <= 16 for conflict-free rows");
-    constexpr int NP = NT - 64;                       // producer threads
-    constexpr int PER = (E * FOLD_CH + NP - 1) / NP;  // quotients per producer and chunk
+    static_assert(FOLD_CH % 4 == 0 && FOLD_ROW >= FOLD_CH, "float4 rows");
     __shared__ __align__(16) float q[2][E][FOLD_ROW];
     __shared__ long long s_off[E];
     __shared__ int s_ok[E];
@@ -1442,51 +1446,76 @@ __global__ __launch_bounds__(NT) void k_const_fold(const DevStar* __restrict__ s
         const int e = threadIdx.x;
         const bool ok = e < cnt && evals[bd.x + e].status == ST_OK;
         s_ok[e] = ok;
-        s_off[e] = ok ? evals[bd.x + e].m_off : 0;
+        s_off[e] = evals[bd.x + min(e, cnt - 1)].m_off;  // every evaluation owns N values
     }
     __syncthreads();
-    const float* __restrict__ flux = S.flux;
-    auto produce = [&](int c, int buf) {
-        const int t = threadIdx.x - 64;
-        float fv[PER], mv[PER];
+    GlobalF* flux = (GlobalF*)S.flux;
+    GlobalF* gmodel = (GlobalF*)model;
+    const int t = threadIdx.x - 64;  // producer thread: pixel t of each chunk
+    unsigned okm[E];                  // all ones for an evaluation whose quotients count
+    long long offv[E];
 #pragma unroll
-        for (int r = 0; r < PER; r++) {
-            const int k = t + r * NP;
-            const int e = k / FOLD_CH, i = k - e * FOLD_CH, px = c * FOLD_CH + i;
-            const bool in = k < E * FOLD_CH && px < N && s_ok[e < E ? e : 0];
-            fv[r] = in ? flux[px] : 0.0f;
-            mv[r] = in ? model[s_off[e] + px] : 1.0f;
-        }
+    for (int e = 0; e < E; e++) {
+        okm[e] = s_ok[e] ? ~0u : 0u;
+        offv[e] = s_off[e];
+    }
+    // chunk c's reads go to register set c % FOLD_SETS, issued FOLD_SETS - 1 steps before
+    // its quotients are formed (at most 3 x 17 newer reads in flight, below the 63 the wait
+    // counter tracks)
+    float f[FOLD_SETS], m[FOLD_SETS][E];
+    auto load = [&](int c, float& fv, float (&mv)[E]) {
+        const int pc = min(c * FOLD_CH + t, N - 1);
+        fv = flux[pc];
 #pragma unroll
-        for (int r = 0; r < PER; r++) {
-            const int k = t + r * NP;
-            if (k < E * FOLD_CH) {
-                const int e = k / FOLD_CH, i = k - e * FOLD_CH;
-                q[buf][e][i] = fv[r] / mv[r];  // 0 / 1 where there is no pixel
+        for (int e = 0; e < E; e++) mv[e] = gmodel[offv[e] + pc];
+    };
+    auto quot = [&](int c, float fv, const float (&mv)[E]) {
+        // every quotient is formed (a clamped read past the star's end divides too) and
+        // then masked, so no branch separates the divisions from their reads
+        const unsigned in = c * FOLD_CH + t < N ? ~0u : 0u;
+        float* row = &q[c & 1][0][t];
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            row[e * FOLD_ROW] = __uint_as_float(__float_as_uint(fv / mv[e]) & in & okm[e]);
+    };
+    auto fold = [&](int buf, float s) {
+        if (lane < cnt) {
+            const float4* r = (const float4*)&q[buf][lane][0];
+#pragma unroll 16
+            for (int i = 0; i < FOLD_CH / 4; i++) {
+                const float4 v = r[i];
+                s = s + v.x;
+                s = s + v.y;
+                s = s + v.z;
+                s = s + v.w;
             }
         }
+        return s;
     };
-    if (wv > 0) produce(0, 0);
+    const bool prod = wv > 0 && N > 0;
+    if (prod) {
+#pragma unroll
+        for (int j = 0; j < FOLD_SETS; j++) load(j, f[j], m[j]);
+        quot(0, f[0], m[0]);
+        load(FOLD_SETS, f[0], m[0]);
+    }
     __syncthreads();
     float s = -0.0f;
-    for (int c = 0; c < n_ch; c++) {
-        const int buf = c & 1;
-        if (wv == 0) {
-            if (lane < cnt) {
-                const float4* r = (const float4*)&q[buf][lane][0];
-#pragma unroll 16
-                for (int i = 0; i < FOLD_CH / 4; i++) {
-                    const float4 v = r[i];
-                    s = s + v.x;
-                    s = s + v.y;
-                    s = s + v.z;
-                    s = s + v.w;
-                }
+    for (int k0 = 0; k0 < n_ch; k0 += FOLD_SETS) {
+#pragma unroll
+        for (int j = 0; j < FOLD_SETS; j++) {
+            // step k: wave 0 adds chunk k; the producers form chunk k + 1 and issue the
+            // reads of chunk k + 1 + FOLD_SETS into the set it frees
+            const int k = k0 + j;
+            if (k >= n_ch) break;  // uniform over the workgroup
+            const int nx = (j + 1) % FOLD_SETS;
+            if (wv == 0) s = fold(k & 1, s);
+            else if (prod && k + 1 < n_ch) {
+                quot(k + 1, f[nx], m[nx]);
+                load(k + 1 + FOLD_SETS, f[nx], m[nx]);
             }
-        } else if (c + 1 < n_ch) {
-            produce(c + 1, buf ^ 1);
+            lds_barrier();
         }
-        __syncthreads();
     }
     if (wv == 0 && lane < cnt && s_ok[lane]) pf[evals[bd.x + lane].pf_off] = s / (float)N;
 }
