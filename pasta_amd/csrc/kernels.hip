@@ -1586,37 +1586,50 @@ __global__ __launch_bounds__(NT, 4) void k_moments_mfma(const DevStar* __restric
         drow2 = dbase + (size_t)min(16 + j, np - 1) * stride - b0;
     }
     const bool tile2 = NTL > 1 && (two || np > 16);  // uniform over the workgroup
-    auto round = [&](int a, const float (&y)[4]) {
-        const v4 d0 = *(gf4*)(drow + a);
+    auto mfma_round = [&](const v4& d0, const v4& d1, const float (&y)[4]) {
         acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.x, (double)y[0], acc[0][0], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.y, (double)y[1], acc[1][0], 0, 0, 0);
         acc[2][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.z, (double)y[2], acc[2][0], 0, 0, 0);
         acc[3][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d0.w, (double)y[3], acc[3][0], 0, 0, 0);
         if (NTL > 1 && tile2) {
-            const v4 d1 = *(gf4*)(drow2 + a);
             acc[0][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.x, (double)y[0], acc[0][NTL - 1], 0, 0, 0);
             acc[1][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.y, (double)y[1], acc[1][NTL - 1], 0, 0, 0);
             acc[2][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.z, (double)y[2], acc[2][NTL - 1], 0, 0, 0);
             acc[3][NTL - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)d1.w, (double)y[3], acc[3][NTL - 1], 0, 0, 0);
         }
     };
+    const v4 dz = {0.0f, 0.0f, 0.0f, 0.0f};
+    auto round = [&](int a, const float (&y)[4]) {
+        const v4 d0 = *(gf4*)(drow + a);
+        v4 d1 = dz;
+        if (NTL > 1 && tile2) d1 = *(gf4*)(drow2 + a);
+        mfma_round(d0, d1, y);
+    };
     // interior rounds: all 16 pixels inside the block, no per-pixel condition; the next
-    // round's flux and model quads are read before this round's products
+    // round's flux, model and design quads are read before this round's products
     const int s_in = (mb.y + 3) & ~3;
     int s0 = s_in;
     if (s0 + 16 <= mb.z) {
         v4 f = *(gf4*)(S.flux + s0 + 4 * g);
         v4 mv = *(gf4*)(m + s0 + 4 * g);
+        v4 d0 = *(gf4*)(drow + s0 + 4 * g);
+        v4 d1 = dz;
+        if (NTL > 1 && tile2) d1 = *(gf4*)(drow2 + s0 + 4 * g);
         for (; s0 + 16 <= mb.z; s0 += 16) {
             const int a = s0 + 4 * g;
             const bool more = s0 + 32 <= mb.z;
             const v4 fn = more ? *(gf4*)(S.flux + a + 16) : f;
             const v4 mn = more ? *(gf4*)(m + a + 16) : mv;
+            const v4 dn0 = more ? *(gf4*)(drow + a + 16) : d0;
+            v4 dn1 = d1;
+            if (more && NTL > 1 && tile2) dn1 = *(gf4*)(drow2 + a + 16);
             const float y[4] = {f.x * __builtin_amdgcn_rcpf(mv.x), f.y * __builtin_amdgcn_rcpf(mv.y),
                                 f.z * __builtin_amdgcn_rcpf(mv.z), f.w * __builtin_amdgcn_rcpf(mv.w)};
-            round(a, y);
+            mfma_round(d0, d1, y);
             f = fn;
             mv = mn;
+            d0 = dn0;
+            d1 = dn1;
         }
     }
     // the pixels before s_in (fewer than 4) and after the last whole round, masked
