@@ -74,6 +74,7 @@ __host__ __device__ constexpr KVariant chi2_variant(int kind, int p1) {
     return (kind == CONT_CONSTANT || kind == CONT_FIXED) ? KVariant{1, 8, 0}
            : (p1 == 5 || p1 == 9 || p1 == 11)           ? KVariant{16, CHI2_GROUP, p1}
            : p1 <= 16                                    ? KVariant{16, CHI2_GROUP, 0}
+           : kind == CONT_LINEAR                         ? KVariant{32, 16, 0}
                                                          : KVariant{32, 4, 0};
 }
 

@@ -1716,8 +1716,11 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
                                              const float* __restrict__ model,
                                              const float* __restrict__ pf,
                                              double* __restrict__ cpart) {
+    // the 30-column linear continuum's variant (16 evaluations per block, so each design
+    // column is read once per 16 evaluations) serves that continuum only
+    constexpr bool LIN = PM == 32 && E == 16;
     __shared__ double red[4 * E];
-    __shared__ float spf[E][2 * PM];
+    __shared__ __align__(16) float spf[E][2 * PM];
     const int4 bd = blocks[blockIdx.x];
     const DevStar& S = stars[evals[bd.x].star];
     const int4 cb = S.cblocks[bd.z];  // (kind, chunk, i0, i1)
@@ -1751,7 +1754,7 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
         }
         __syncthreads();
     }
-    if (S.cont_kind == CONT_CHUNK) {
+    if (!LIN && S.cont_kind == CONT_CHUNK) {
         const int c = cb.y;
         const double lo = S.chunk_lohi[2 * c], hi = S.chunk_lohi[2 * c + 1];
         const double plo = c > 0 ? S.chunk_lohi[2 * c - 2] : 0.0;
@@ -1796,16 +1799,26 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
         }
     } else if (S.cont_kind == CONT_LINEAR) {
         const size_t n4 = (size_t)(S.N + 3) & ~(size_t)3;  // design column stride
+        GlobalF* des = (GlobalF*)S.design;
+        GlobalF* flx = (GlobalF*)S.flux;
+        GlobalF* ivr = (GlobalF*)S.ivar;
         for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
             __asm__ volatile("" ::: "memory");
-            float d[PM];
+            float d[PM], mv[E];
 #pragma unroll
-            for (int k = 0; k < PM; k++) d[k] = k < p1 ? S.design[(size_t)k * n4 + i] : 0.0f;
-            const float fl = S.flux[i], iv = S.ivar[i];
+            for (int k = 0; k < PM; k++) d[k] = k < p1 ? des[(size_t)k * n4 + i] : 0.0f;
+            const float fl = flx[i], iv = ivr[i];
+#pragma unroll
+            for (int e = 0; e < E; e++) {  // every m[e] points at N values
+                const float x = ((GlobalF*)m[e])[i];
+                mv[e] = ok[e] ? x : 1.0f;
+            }
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 if (!ok[e]) continue;
-                const float bq = fl / m[e][i];
+                // one evaluation's parameters at a time (LDS broadcasts), not E * PM registers
+                if (LIN) __asm__ volatile("" ::: "memory");
+                const float bq = fl / mv[e];
                 float y = 0.0f;
 #pragma unroll
                 for (int k = 0; k < PM; k++) {
@@ -1818,7 +1831,7 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
                 acc[e] += (double)((r * iv) * r);
             }
         }
-    } else if (S.cont_kind == CONT_FIXED) {
+    } else if (!LIN && S.cont_kind == CONT_FIXED) {
         for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
             const float fl = S.flux[i], iv = S.ivar[i], fc = S.fixedc[i];
 #pragma unroll
@@ -1828,7 +1841,7 @@ __global__ __launch_bounds__(NT) void k_chi2(const DevStar* __restrict__ stars,
                 acc[e] += (double)((r * iv) * r);
             }
         }
-    } else {
+    } else if (!LIN) {
         for (int i = cb.z + threadIdx.x; i < cb.w; i += NT) {
             const float fl = S.flux[i], iv = S.ivar[i];
 #pragma unroll
@@ -2326,6 +2339,7 @@ hipError_t launch_cont(const DevStar* stars, DevEval* evals, int n_evals, const 
         PASTA_CHI2(16, CHI2_GROUP, 11)
         PASTA_CHI2(16, CHI2_GROUP, 0)
         PASTA_CHI2(32, 4, 0)
+        PASTA_CHI2(32, 16, 0)
 #undef PASTA_CHI2
     }
     hipLaunchKernelGGL(k_final, dim3((n_evals + 127) / 128), dim3(128), 0, st, stars, evals,
