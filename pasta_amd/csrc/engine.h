@@ -9,7 +9,7 @@ namespace pasta {
 // Tile and block shapes are build-time parameters (pasta_amd/tune.py rebuilds the library
 // with other values and times the batched iteration with each).
 #ifndef PASTA_TILE
-#define PASTA_TILE 2032
+#define PASTA_TILE 3056
 #endif
 #ifndef PASTA_CONT_BLOCK
 #define PASTA_CONT_BLOCK 4096
@@ -20,8 +20,8 @@ namespace pasta {
 constexpr int TILE = PASTA_TILE;  // synthetic pixels per model-kernel workgroup: the tile
                                   // and a 7-tap instrumental halo are whole 256-output MFMA
                                   // tiles of the rotation
-static_assert(TILE % 256 == 240 && TILE >= 752 && TILE <= 2032,
-              "tile = 256 k - 16 pixels, 3 <= k <= 8 (k_model's LDS fits 4 workgroups per CU)");
+static_assert(TILE % 256 == 240 && TILE >= 752 && TILE <= 3056,
+              "tile = 256 k - 16 pixels, 3 <= k <= 12 (k_model's LDS fits 4 workgroups per CU)");
 constexpr int NT = 256;           // threads per workgroup (4 wavefronts)
 constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
 constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)
@@ -41,10 +41,14 @@ constexpr int ROT_STRIDE = KROT_MAX + 4 * TW_MAX;
 constexpr int FFT_MIN_TAPS = KB_MAX + 1;
 constexpr int FFT_N = 4096;       // real points of the FFT path's transforms
 constexpr int FFT_M = FFT_N / 2;  // their complex half-length
-// A K-tap kernel whose tiles carry hd instrumental half-width pixels on each side fits one
-// transform: every tile's interpolated range (<= 7 + TILE + 1 + 2 hd + K - 1, rounded up to
-// 8) stays within FFT_N samples, so no output index of the circular convolution wraps.
-__host__ __device__ constexpr bool fft_fits(int K, int hd) { return K + 2 * hd <= FFT_N - TILE - 14; }
+// The FFT path convolves a tile's rotation output range (nR <= TILE + 1 + 2 hd values) in
+// one transform, or in two pieces of fft_piece(nR) values when the range and the kernel do
+// not fit one; a piece's interpolated input (<= 7 + piece + K - 1 samples) stays within
+// FFT_N samples, so no output index of the circular convolution wraps.
+__host__ __device__ constexpr int fft_piece(int nR) { return ((nR + 1) / 2 + 7) & ~7; }
+__host__ __device__ constexpr bool fft_fits(int K, int hd) {
+    return 7 + fft_piece(TILE + 1 + 2 * hd) + K - 1 <= FFT_N;
+}
 constexpr int CH_MAX = 32;        // max chunks
 constexpr int P1_MAX = 32;        // max continuum parameters per chunk / linear columns
 constexpr int BLK = PASTA_CONT_BLOCK;  // observed pixels per reduction block

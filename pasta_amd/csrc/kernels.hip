@@ -27,15 +27,23 @@ namespace pasta {
 // Interpolated tile for the rotation stage: logical index k lives in slot k + 4*(k/16)
 // (4 pad floats after every 16), so that the 16-float-strided MFMA operand rows fall
 // in distinct LDS banks.  Logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KROT_MAX + 55.
-constexpr int ILOG = (7 + TILE + 1 + 2 * HD_MAX + KROT_MAX + 55 + 31) & ~31;  // 4416 at 2032
+constexpr int ILOG = (7 + TILE + 1 + 2 * HD_MAX + KROT_MAX + 55 + 31) & ~31;  // 5440 at 3056
 constexpr int ILDS = ILOG + ILOG / 4;
 // For the f16 rotation the tile is kept as two f16 arrays (hi, lo) of ILOGB elements
 // (logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KB_MAX + 71).
-constexpr int ILOGB = (7 + TILE + 1 + 2 * HD_MAX + KB_MAX + 71 + 7) & ~7;     // 3008 at 2032
+constexpr int ILOGB = (7 + TILE + 1 + 2 * HD_MAX + KB_MAX + 71 + 7) & ~7;     // 4928 at 3056
+// The rotation output R (TILE + 2 hd + 2 values, 16 zeros after) and the instrumental
+// stage's output C (<= TILE + 1 values) share k_model's one LDS buffer with the rotation's
+// operands: R overwrites the interpolated tile at the buffer's start once every wavefront
+// has read its operands (for the FFT path, whose transform buffer that is, R follows the
+// transform buffer), and C follows R (for the FFT path, C takes the transform buffer).
 constexpr int RLDS = TILE + 1 + 2 * HD_MAX + 8 + 16 + 8;
+constexpr int LDS_ROT = ILDS + KROT_MAX + 48;  // floats of k_model's buffer
+constexpr int C_OFF = (RLDS + 3) & ~3;         // C after R
 // k_model's LDS: 4 workgroups per CU share its 160 KB
-static_assert((ILDS + KROT_MAX + 48 + RLDS + 2 * HD_MAX + 1 + 8) * 4 <= 40960,
+static_assert((LDS_ROT + 2 * HD_MAX + 1 + 8) * 4 <= 40960,
               "k_model LDS beyond a quarter of the CU's");
+static_assert(C_OFF + TILE + 8 <= LDS_ROT, "R and C within the buffer");
 constexpr double C_KMS = 299792.0;   // src/convolve_rv.rs:440,547
 constexpr double LN10 = 2.302585092994045684017991454684364208;
 
@@ -425,9 +433,45 @@ __device__ __forceinline__ void conv_tile(const float* in, const float* tr, int 
 // that lane (m, kk) of step r in group c takes j = 16c + 4kk + r: its four A operands
 // of a group are I[16(a+c) + 4kk .. +3], one 16-byte LDS read in the padded layout
 // (slot 20(a+c) + 4kk), conflict-free across the 16 rows.  hpad[16 + u] = tr[u] for
-// u < K and zero elsewhere in [0, 16C + 16).  Each wave owns 16-block tiles w and w+4;
-// rows past the last block read the last block (their results are not stored).
-// Outputs are stored to out[p - off] for off <= p < np, zero outside [0, W) when mask.
+// u < K and zero elsewhere in [0, 16C + 16).  Each wave owns 16-block tiles w and w+4
+// (and w + 8, w + 12); rows past the last block read the last block (their results are
+// not stored).  Outputs are stored to out[p - off] for off <= p < np, zero outside [0, W)
+// when mask; out may be the operand tile (the stores follow a barrier).
+// A wavefront's output tiles (16 x 16 blocks t and t + 4 per round, rounds 8 tiles apart)
+// are held in registers until every wavefront has read its operands, then stored over the
+// operand tile: ROT_ROUNDS rounds cover any tile and instrumental halo.
+constexpr int ROT_ROUNDS = 2;
+static_assert((((7 + TILE + 2 + 2 * HD_MAX + 15) >> 4) + 15) >> 4 <= 8 * ROT_ROUNDS,
+              "rotation outputs within two rounds of two 256-output tiles per wavefront");
+
+template <bool SCALE>
+__device__ __forceinline__ void store_rot(const f4 (&c0)[ROT_ROUNDS], const f4 (&c1)[ROT_ROUNDS],
+                                          int tiles, int np, int off, float* out, bool mask,
+                                          long long x0, int W, float sc) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m = lane & 15, kk = lane >> 4;
+    lds_barrier();  // every wavefront has read its operands
+#pragma unroll
+    for (int it = 0; it < ROT_ROUNDS; it++) {
+        const int t0 = wv + 8 * it, t1 = t0 + 4;
+        if (t0 >= tiles) break;
+        const bool has1 = t1 < tiles;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int p = 16 * (16 * t0 + 4 * kk + i) + m;
+            if (p >= off && p < np) {
+                const long long x = x0 + (p - off);
+                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : (SCALE ? c0[it][i] * sc : c0[it][i]);
+            }
+            p = 16 * (16 * t1 + 4 * kk + i) + m;
+            if (has1 && p >= off && p < np) {
+                const long long x = x0 + (p - off);
+                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : (SCALE ? c1[it][i] * sc : c1[it][i]);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void conv_rot_mfma(const float* Ipad, const float* hpad, int K, int np,
                                               int off, float* out, bool mask, long long x0, int W) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -436,8 +480,11 @@ __device__ __forceinline__ void conv_rot_mfma(const float* Ipad, const float* hp
     const int C = (K + 15 + 15) >> 4;
     const int m = lane & 15, kk = lane >> 4;
     const float* hb = hpad + 16 + 4 * kk - m;
-    for (int t0 = wv; t0 < tiles; t0 += 8) {
-        const int t1 = t0 + 4;
+    f4 k0[ROT_ROUNDS], k1[ROT_ROUNDS];
+#pragma unroll
+    for (int it = 0; it < ROT_ROUNDS; it++) {
+        const int t0 = wv + 8 * it, t1 = t0 + 4;
+        if (t0 >= tiles) break;
         const bool has1 = t1 < tiles;
         const int a0 = min(16 * t0 + m, nblk - 1), a1 = min(16 * t1 + m, nblk - 1);
         const f4* ia0 = (const f4*)(Ipad + 20 * a0 + 4 * kk);
@@ -465,20 +512,10 @@ __device__ __forceinline__ void conv_rot_mfma(const float* Ipad, const float* hp
                     c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0v[r], h[r], c0, 0, 0, 0);
             }
         }
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int p = 16 * (16 * t0 + 4 * kk + i) + m;
-            if (p >= off && p < np) {
-                const long long x = x0 + (p - off);
-                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c0[i];
-            }
-            p = 16 * (16 * t1 + 4 * kk + i) + m;
-            if (has1 && p >= off && p < np) {
-                const long long x = x0 + (p - off);
-                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c1[i];
-            }
-        }
+        k0[it] = c0;
+        k1[it] = c1;
     }
+    store_rot<false>(k0, k1, tiles, np, off, out, mask, x0, W, 1.0f);
 }
 
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
@@ -528,8 +565,11 @@ __device__ __forceinline__ void conv_rot_f16(const unsigned short* Ib2, const un
     const int m = lane & 15, kk = lane >> 4;
     const int s0 = 8 * kk + 16 - m;  // tpad index of this lane's first B operand, group 0
     const unsigned int* twp = tw + (s0 & 1) * TW_MAX + (s0 >> 1);
-    for (int t0 = wv; t0 < tiles; t0 += 8) {
-        const int t1 = t0 + 4;
+    f4 k0[ROT_ROUNDS], k1[ROT_ROUNDS];
+#pragma unroll
+    for (int it = 0; it < ROT_ROUNDS; it++) {
+        const int t0 = wv + 8 * it, t1 = t0 + 4;
+        if (t0 >= tiles) break;
         const bool has1 = t1 < tiles;
         const int a0 = min(16 * t0 + m, nblk - 1), a1 = min(16 * t1 + m, nblk - 1);
         const unsigned short* ia0 = Ib2 + 16 * a0 + 8 * kk;
@@ -552,20 +592,10 @@ __device__ __forceinline__ void conv_rot_f16(const unsigned short* Ib2, const un
                 c0 = mfma3(ah, al, bh, bl, c0);
             }
         }
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int p = 16 * (16 * t0 + 4 * kk + i) + m;
-            if (p >= off && p < np) {
-                const long long x = x0 + (p - off);
-                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c0[i] * rinv;
-            }
-            p = 16 * (16 * t1 + 4 * kk + i) + m;
-            if (has1 && p >= off && p < np) {
-                const long long x = x0 + (p - off);
-                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : c1[i] * rinv;
-            }
-        }
+        k0[it] = c0;
+        k1[it] = c1;
     }
+    store_rot<true>(k0, k1, tiles, np, off, out, mask, x0, W, rinv);
 }
 
 // ---------------------------------------------------------------------------
@@ -583,6 +613,8 @@ __device__ __forceinline__ void conv_rot_f16(const unsigned short* Ib2, const un
 // ---------------------------------------------------------------------------
 constexpr int FFT_SLOTS = FFT_M + FFT_M / 16;  // padded complex slots
 static_assert(2 * FFT_SLOTS <= ILDS, "FFT buffer within the rotation stage's LDS");
+static_assert(2 * FFT_SLOTS + RLDS <= LDS_ROT && TILE + 1 <= 2 * FFT_SLOTS,
+              "FFT path: R behind the transform buffer, C in it");
 static_assert(2 * (FFT_M + 1) <= ROT_STRIDE, "kernel spectrum within an evaluation's tap slot");
 static_assert(NT == 256 && FFT_M == 8 * NT, "one radix-8 butterfly per thread and pass");
 
@@ -839,7 +871,7 @@ struct TileJob {
     long long rlo;   // first pixel of the rotation output R (clo - hd)
     int nC, hd, K, nR;
     bool mask;
-    int kind;        // 0 no rotation (tile into Rb), 1 f16 rotation, 2 f32 rotation, 3 FFT
+    int kind;        // 0 no rotation (the tile is R), 1 f16 rotation, 2 f32 rotation, 3 FFT
     int inst;        // instrumental stage: DISP_* of the star, DISP_NONE when composed
     long long g0;    // first interpolated 8-pixel group (multiple of 8)
     int off, ng;     // logical index of rlo - K/2 (or rlo); groups interpolated
@@ -889,7 +921,7 @@ __device__ __forceinline__ bool tile_job(const DevGrid& G, const DevStar* stars,
 
 // store one interpolated 8-pixel group (local group index gl) in the job's tile layout
 __device__ __forceinline__ void store_group(const TileJob& J, int gl, const float (&acc)[8],
-                                            float* Ib, float* Rb) {
+                                            float* Ib) {
     if (J.kind == 1) {
         const float sc = J.ev->rsc;
         u16x8 h, l;
@@ -904,7 +936,7 @@ __device__ __forceinline__ void store_group(const TileJob& J, int gl, const floa
         *(u16x8*)d = h;
         *(u16x8*)(d + ILOGB) = l;
     } else {
-        float* d = J.kind == 2 ? Ib + islot(8 * gl) : J.kind == 3 ? Ib + 8 * gl : Rb + 8 * gl;
+        float* d = J.kind == 2 ? Ib + islot(8 * gl) : Ib + 8 * gl;
         *(f4*)d = f4{acc[0], acc[1], acc[2], acc[3]};
         *(f4*)(d + 4) = f4{acc[4], acc[5], acc[6], acc[7]};
     }
@@ -1005,8 +1037,6 @@ __device__ __forceinline__ void load_taps(const TileJob& J, const float* rot, fl
             hrev[i] = (u >= 0 && u < K) ? h[K - 1 - u] : 0.0f;
         }
         if (tid < 48) Ib[islot(8 * J.ng + tid)] = 0.0f;  // the MFMA steps read past the tile
-    } else if (J.kind == 3) {
-        for (int i = 8 * J.ng + tid; i < FFT_N; i += NT) Ib[i] = 0.0f;  // the transform's input
     }
     if (J.inst == DISP_FIXED) {
         const int Kp = (S.Kd + 7) & ~7;
@@ -1052,32 +1082,32 @@ __device__ __forceinline__ void resample(const TileJob& J, const DevStar& S, con
 // taps are in LDS (stored but not yet synchronised)
 template <bool FFT = false>
 __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, const float* rot,
-                                           float* model, int mode, float* Ib, float* Rb,
-                                           float* hrev, unsigned int* tw, float* grev) {
+                                           float* model, int mode, float* Ib, float* hrev,
+                                           unsigned int* tw, float* grev) {
     const int tid = threadIdx.x;
     const DevStar& S = *J.S;
     lds_barrier();
-    // rotational broadening, outputs p = off + q for R index q
-    float* R = Rb;
+    // rotational broadening, outputs p = off + q for R index q (stored over the operands)
+    float* R = Ib;
     if (J.kind == 1)
-        conv_rot_f16((const unsigned short*)Ib, tw, J.K, J.off + J.nR, J.off, Rb, J.mask, J.rlo, G.W,
+        conv_rot_f16((const unsigned short*)Ib, tw, J.K, J.off + J.nR, J.off, R, J.mask, J.rlo, G.W,
                      J.ev->rinv);
     else if (J.kind == 2)
-        conv_rot_mfma(Ib, hrev, J.K, J.off + J.nR, J.off, Rb, J.mask, J.rlo, G.W);
+        conv_rot_mfma(Ib, hrev, J.K, J.off + J.nR, J.off, R, J.mask, J.rlo, G.W);
     else if (FFT && J.kind == 3)
-        conv_fft(Ib, (const cf*)(rot + (size_t)J.e * ROT_STRIDE), J.K, J.off, J.nR, Rb, J.mask,
-                 J.rlo, G.W);
+        R = Ib + 2 * FFT_SLOTS;  // formed piece by piece (fft_pieces) behind the transform buffer
     else
-        R = Rb + J.off;
+        R = Ib + J.off;  // no rotation: the interpolated tile itself
     if (tid < 16) R[J.nR + tid] = 0.0f;
     lds_barrier();
 
-    // instrumental broadening -> C (reuses Ib)
+    // instrumental broadening -> C (behind R, or in the FFT path's transform buffer)
     const int nC = J.nC, hd = J.hd;
     const float* Cb = R;
+    float* const Cw = (FFT && J.kind == 3) ? Ib : Ib + C_OFF;
     if (J.inst == DISP_FIXED) {
-        conv_tile(R, grev, S.Kd, nC, Ib, false, 0, 0);
-        Cb = Ib;
+        conv_tile(R, grev, S.Kd, nC, Cw, false, 0, 0);
+        Cb = Cw;
         lds_barrier();
     } else if (J.inst == DISP_VARIABLE) {
         const int Kd = S.Kd;
@@ -1092,9 +1122,9 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
                 for (int u = 0; u < Kd; u++) a = fmaf(kr[u], R[q + u], a);
                 c = a;
             }
-            Ib[q] = c;
+            Cw[q] = c;
         }
-        Cb = Ib;
+        Cb = Cw;
         lds_barrier();
     }
 
@@ -1110,21 +1140,54 @@ __device__ __forceinline__ void finish_job(const DevGrid& G, const TileJob& J, c
     else resample<false>(J, S, Cb, out);
 }
 
+// The FFT path's rotation output R behind the transform buffer: the tile's output range in
+// one transform, or in two pieces (fft_piece) when the range and the kernel do not fit one.
+// A piece starting at output q0 (a multiple of 8) transforms the interpolated samples from
+// q0 on: shifting the input shifts the output, so its outputs are R[q0 ..].  Each piece
+// gathers its own input (an FFT evaluation's tile is interpolated once more where the
+// pieces overlap, by the kernel's width).
+__device__ __forceinline__ void fft_pieces(const DevGrid& G, const TileJob& J, const float* rot,
+                                           float* Ib) {
+    const int tid = threadIdx.x;
+    const int P = J.off + J.nR + J.K - 1 <= FFT_N ? J.nR : fft_piece(J.nR);
+    const cf* spec = (const cf*)(rot + (size_t)J.e * ROT_STRIDE);
+    float* R = Ib + 2 * FFT_SLOTS;
+    for (int q0 = 0; q0 < J.nR; q0 += P) {
+        const int nq = min(P, J.nR - q0);
+        const int gq = q0 >> 3;
+        const int ngq = min(J.ng - gq, (J.off + nq + J.K - 1 + 7) >> 3);
+        if (q0 > 0) lds_barrier();  // the previous piece's transform has been read
+        for (int gg = tid; gg < ngq; gg += NT) {
+            float acc[8];
+            gather_one(G, *J.ev, J.g0 + 8LL * (gq + gg), acc);
+            *(f4*)(Ib + 8 * gg) = f4{acc[0], acc[1], acc[2], acc[3]};
+            *(f4*)(Ib + 8 * gg + 4) = f4{acc[4], acc[5], acc[6], acc[7]};
+        }
+        for (int i = 8 * ngq + tid; i < FFT_N; i += NT) Ib[i] = 0.0f;  // the transform's input
+        lds_barrier();
+        conv_fft(Ib, spec, J.K, J.off, nq, R + q0, J.mask, J.rlo + q0, G.W);
+    }
+}
+
 // evaluation e alone on this tile (its job exists; the LDS is free)
 template <bool FFT = false>
 __device__ __forceinline__ void run_solo(const DevGrid& G, const DevStar* stars, const DevEval* evals,
                                          const float* rot, int e, int tile, const int* tbound,
-                                         int tb_stride, int mode, float* model, float* Ib, float* Rb,
+                                         int tb_stride, int mode, float* model, float* Ib,
                                          float* hrev, unsigned int* tw, float* grev) {
     TileJob B;
     tile_job(G, stars, evals, e, tile, tbound, tb_stride, mode, B);
     load_taps(B, rot, Ib, hrev, tw, grev);
-    for (int gg = threadIdx.x; gg < B.ng; gg += NT) {
-        float acc[8];
-        gather_one(G, *B.ev, B.g0 + 8LL * gg, acc);
-        store_group(B, gg, acc, Ib, Rb);
+    if (FFT && B.kind == 3) {
+        fft_pieces(G, B, rot, Ib);
+    } else {
+        for (int gg = threadIdx.x; gg < B.ng; gg += NT) {
+            float acc[8];
+            gather_one(G, *B.ev, B.g0 + 8LL * gg, acc);
+            store_group(B, gg, acc, Ib);
+        }
     }
-    finish_job<FFT>(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
+    finish_job<FFT>(G, B, rot, model, mode, Ib, hrev, tw, grev);
 }
 
 // A work item of GN (2 or 3) evaluations whose rows are the union U (k_union) and whose tiles
@@ -1136,7 +1199,7 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
                                           const float* rot, int tile, const int* tbound,
                                           int tb_stride, int mode, float* model, const TileJob& A,
                                           int e1, int e2, const DevUnion& U, long long pg0, int png,
-                                          float* Ib, float* Rb, float* hrev, unsigned int* tw,
+                                          float* Ib, float* hrev, unsigned int* tw,
                                           float* grev) {
     // with three evaluations the registers go to the kept interpolations instead
     const DevEval& eb = evals[e1];
@@ -1177,14 +1240,14 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
                         for (int g = 0; g < GN; g++) acc[g][t] = 0.0f;
             }
             const long long ga = (x0 - A.g0) >> 3;
-            if (ga >= 0 && ga < A.ng) store_group(A, (int)ga, acc[0], Ib, Rb);
+            if (ga >= 0 && ga < A.ng) store_group(A, (int)ga, acc[0], Ib);
         }
 #pragma unroll
         for (int g = 1; g < GN; g++)
 #pragma unroll
             for (int t = 0; t < 8; t++) keep[g - 1][r][t] = acc[g][t];
     }
-    finish_job(G, A, rot, model, mode, Ib, Rb, hrev, tw, grev);
+    finish_job(G, A, rot, model, mode, Ib, hrev, tw, grev);
 #pragma unroll
     for (int g = 1; g < GN; g++) {
         lds_barrier();
@@ -1204,9 +1267,9 @@ __device__ __forceinline__ void run_group(const DevGrid& G, const DevStar* stars
         for (int r = 0; r < 2; r++) {
             const int gg = threadIdx.x + r * NT;
             const long long gb = (pg0 + 8LL * gg - B.g0) >> 3;
-            if (gg < png && gb >= 0 && gb < B.ng) store_group(B, (int)gb, keep[g - 1][r], Ib, Rb);
+            if (gg < png && gb >= 0 && gb < B.ng) store_group(B, (int)gb, keep[g - 1][r], Ib);
         }
-        finish_job(G, B, rot, model, mode, Ib, Rb, hrev, tw, grev);
+        finish_job(G, B, rot, model, mode, Ib, hrev, tw, grev);
     }
 }
 
@@ -1223,8 +1286,7 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
     // hrev: zero, then the taps from index 16, zeros after) or f16 x 2 (tile arrays,
     // split tap pairs tw).  Ib also holds the instrumental-broadening output.
     static_assert(2 * ILOGB * 2 + 4 * TW_MAX * 4 <= (ILDS + KROT_MAX + 48) * 4, "LDS union");
-    __shared__ __align__(16) float lds_rot[ILDS + KROT_MAX + 48];
-    __shared__ float Rb[RLDS];
+    __shared__ __align__(16) float lds_rot[LDS_ROT];
     __shared__ float grev[2 * HD_MAX + 1 + 8];
     float* const Ib = lds_rot;
     float* const hrev = lds_rot + ILDS;
@@ -1266,7 +1328,7 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
             if (!(g == 0 ? hasA : g == 1 ? hasB : hasC)) continue;
             if (!first) lds_barrier();
             first = false;
-            run_solo<true>(G, stars, evals, rot, ids[g], tile, tbound, tb_stride, mode, model, Ib, Rb,
+            run_solo<true>(G, stars, evals, rot, ids[g], tile, tbound, tb_stride, mode, model, Ib,
                            hrev, tw, grev);
         }
         return;
@@ -1279,12 +1341,12 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
             const long long q0 = min(pg0, cg0), q1 = max(pg1, cg0 + 8LL * cng);
             if (((q1 - q0) >> 3) <= 2 * NT) {
                 run_group<3>(G, stars, evals, rot, tile, tbound, tb_stride, mode, model, A, ids[1],
-                             ids[2], U, q0, (int)((q1 - q0) >> 3), Ib, Rb, hrev, tw, grev);
+                             ids[2], U, q0, (int)((q1 - q0) >> 3), Ib, hrev, tw, grev);
                 return;
             }
         } else if (((pg1 - pg0) >> 3) <= 2 * NT) {
             run_group<2>(G, stars, evals, rot, tile, tbound, tb_stride, mode, model, A, ids[1], -1,
-                         U, pg0, (int)((pg1 - pg0) >> 3), Ib, Rb, hrev, tw, grev);
+                         U, pg0, (int)((pg1 - pg0) >> 3), Ib, hrev, tw, grev);
             return;
         }
     }
@@ -1293,18 +1355,18 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
         for (int gg = threadIdx.x; gg < A.ng; gg += NT) {
             float acc[8];
             gather_one(G, *A.ev, A.g0 + 8LL * gg, acc);
-            store_group(A, gg, acc, Ib, Rb);
+            store_group(A, gg, acc, Ib);
         }
-        finish_job(G, A, rot, model, mode, Ib, Rb, hrev, tw, grev);
+        finish_job(G, A, rot, model, mode, Ib, hrev, tw, grev);
     }
     if (hasB) {
         if (hasA) lds_barrier();
-        run_solo(G, stars, evals, rot, ids[1], tile, tbound, tb_stride, mode, model, Ib, Rb, hrev,
+        run_solo(G, stars, evals, rot, ids[1], tile, tbound, tb_stride, mode, model, Ib, hrev,
                  tw, grev);
     }
     if (hasC) {
         if (hasA || hasB) lds_barrier();
-        run_solo(G, stars, evals, rot, ids[2], tile, tbound, tb_stride, mode, model, Ib, Rb, hrev,
+        run_solo(G, stars, evals, rot, ids[2], tile, tbound, tb_stride, mode, model, Ib, hrev,
                  tw, grev);
     }
 }

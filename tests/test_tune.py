@@ -17,7 +17,7 @@ def _tune():
 
 
 def test_candidates_respect_the_shape_rules():
-    """Tiles are 256 k - 16 pixels with 3 <= k <= 8, blocks whole 1024-pixel units, chi2
+    """Tiles are 256 k - 16 pixels with 3 <= k <= 12, blocks whole 1024-pixel units, chi2
     groups 4, 8 or 16 (the static_asserts of csrc/engine.h), one parameter per candidate."""
     t = _tune()
     assert list(t.CANDIDATES)[0] == "default" and t.CANDIDATES["default"] == {}
@@ -25,7 +25,7 @@ def test_candidates_respect_the_shape_rules():
         assert len(d) <= 1, name
         for k, v in d.items():
             if k == "PASTA_TILE":
-                assert v % 256 == 240 and 752 <= v <= 2032, name
+                assert v % 256 == 240 and 752 <= v <= 3056, name
             elif k == "PASTA_CONT_BLOCK":
                 assert v % 1024 == 0 and v >= 1024, name
             elif k == "PASTA_CHI2_GROUP":
@@ -66,9 +66,9 @@ def test_build_stages_each_candidate(tmp_path, monkeypatch):
     monkeypatch.setattr(B, "build_lib", fake_build_lib)
     if not os.path.exists(B.SYNTH_LIB):
         pytest.skip("libpasta_synth.so is not built")
-    t.build(str(tmp_path), ["default", "tile1776"])
-    assert [c[0] for c in calls] == [{}, {"PASTA_TILE": 1776}]
-    for name in ("default", "tile1776"):
+    t.build(str(tmp_path), ["default", "tile2032"])
+    assert [c[0] for c in calls] == [{}, {"PASTA_TILE": 2032}]
+    for name in ("default", "tile2032"):
         d = tmp_path / name
         assert (d / "bench.py").exists()
         for f in t.PY_FILES + ["libpasta_synth.so", "libpasta_hip.so"]:
