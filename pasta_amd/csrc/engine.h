@@ -20,8 +20,8 @@ namespace pasta {
 constexpr int TILE = PASTA_TILE;  // synthetic pixels per model-kernel workgroup: the tile
                                   // and a 7-tap instrumental halo are whole 256-output MFMA
                                   // tiles of the rotation
-static_assert(TILE % 256 == 240 && TILE >= 752 && TILE <= 3056,
-              "tile = 256 k - 16 pixels, 3 <= k <= 12 (k_model's LDS fits 4 workgroups per CU)");
+static_assert(TILE % 256 == 240 && TILE >= 752 && TILE <= 3824,
+              "tile = 256 k - 16 pixels, 3 <= k <= 15 (k_model's LDS fits 4 workgroups per CU)");
 constexpr int NT = 256;           // threads per workgroup (4 wavefronts)
 constexpr int HD_MAX = 127;       // max half-width of the instrumental kernel
 constexpr int KROT_MAX = 2048;    // FFTSIZE of the reference (convolve_rv.rs:433)

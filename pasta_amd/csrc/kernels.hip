@@ -437,36 +437,35 @@ __device__ __forceinline__ void conv_tile(const float* in, const float* tr, int 
 // (and w + 8, w + 12); rows past the last block read the last block (their results are
 // not stored).  Outputs are stored to out[p - off] for off <= p < np, zero outside [0, W)
 // when mask; out may be the operand tile (the stores follow a barrier).
-// A wavefront's output tiles (16 x 16 blocks t and t + 4 per round, rounds 8 tiles apart)
-// are held in registers until every wavefront has read its operands, then stored over the
-// operand tile: ROT_ROUNDS rounds cover any tile and instrumental halo.
+// A wavefront's output tiles (NTR 16 x 16-block tiles per round, 4 apart: t, t + 4, ...;
+// rounds 4 NTR tiles apart) are held in registers until every wavefront has read its
+// operands, then stored over the operand tile: ROT_ROUNDS rounds cover any tile and
+// instrumental halo.
 constexpr int ROT_ROUNDS = 2;
-static_assert((((7 + TILE + 2 + 2 * HD_MAX + 15) >> 4) + 15) >> 4 <= 8 * ROT_ROUNDS,
-              "rotation outputs within two rounds of two 256-output tiles per wavefront");
+constexpr int ROT_TILES_MAX = (((7 + TILE + 2 + 2 * HD_MAX + 15) >> 4) + 15) >> 4;
+static_assert(ROT_TILES_MAX <= 8 * ROT_ROUNDS, "rotation outputs within two rounds per wavefront");
 
-template <bool SCALE>
-__device__ __forceinline__ void store_rot(const f4 (&c0)[ROT_ROUNDS], const f4 (&c1)[ROT_ROUNDS],
-                                          int tiles, int np, int off, float* out, bool mask,
-                                          long long x0, int W, float sc) {
+template <int NTR, bool SCALE>
+__device__ __forceinline__ void store_rot(const f4 (&c)[ROT_ROUNDS][NTR], int tiles, int np,
+                                          int off, float* out, bool mask, long long x0, int W,
+                                          float sc) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int m = lane & 15, kk = lane >> 4;
     lds_barrier();  // every wavefront has read its operands
 #pragma unroll
     for (int it = 0; it < ROT_ROUNDS; it++) {
-        const int t0 = wv + 8 * it, t1 = t0 + 4;
-        if (t0 >= tiles) break;
-        const bool has1 = t1 < tiles;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int p = 16 * (16 * t0 + 4 * kk + i) + m;
-            if (p >= off && p < np) {
-                const long long x = x0 + (p - off);
-                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : (SCALE ? c0[it][i] * sc : c0[it][i]);
-            }
-            p = 16 * (16 * t1 + 4 * kk + i) + m;
-            if (has1 && p >= off && p < np) {
-                const long long x = x0 + (p - off);
-                out[p - off] = (mask && (x < 0 || x >= W)) ? 0.0f : (SCALE ? c1[it][i] * sc : c1[it][i]);
+        for (int j = 0; j < NTR; j++) {
+            const int t = wv + 4 * j + 4 * NTR * it;
+            if (4 * j + 4 * NTR * it >= ROT_TILES_MAX || t >= tiles) continue;  // (no such tile)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int p = 16 * (16 * t + 4 * kk + i) + m;
+                if (p >= off && p < np) {
+                    const long long x = x0 + (p - off);
+                    out[p - off] =
+                        (mask && (x < 0 || x >= W)) ? 0.0f : (SCALE ? c[it][j][i] * sc : c[it][j][i]);
+                }
             }
         }
     }
@@ -480,12 +479,12 @@ __device__ __forceinline__ void conv_rot_mfma(const float* Ipad, const float* hp
     const int C = (K + 15 + 15) >> 4;
     const int m = lane & 15, kk = lane >> 4;
     const float* hb = hpad + 16 + 4 * kk - m;
-    f4 k0[ROT_ROUNDS], k1[ROT_ROUNDS];
+    f4 k[ROT_ROUNDS][2];
 #pragma unroll
     for (int it = 0; it < ROT_ROUNDS; it++) {
         const int t0 = wv + 8 * it, t1 = t0 + 4;
-        if (t0 >= tiles) break;
-        const bool has1 = t1 < tiles;
+        if (8 * it >= ROT_TILES_MAX || t0 >= tiles) break;
+        const bool has1 = 8 * it + 4 < ROT_TILES_MAX && t1 < tiles;
         const int a0 = min(16 * t0 + m, nblk - 1), a1 = min(16 * t1 + m, nblk - 1);
         const f4* ia0 = (const f4*)(Ipad + 20 * a0 + 4 * kk);
         const f4* ia1 = (const f4*)(Ipad + 20 * a1 + 4 * kk);
@@ -512,10 +511,10 @@ __device__ __forceinline__ void conv_rot_mfma(const float* Ipad, const float* hp
                     c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0v[r], h[r], c0, 0, 0, 0);
             }
         }
-        k0[it] = c0;
-        k1[it] = c1;
+        k[it][0] = c0;
+        k[it][1] = c1;
     }
-    store_rot<false>(k0, k1, tiles, np, off, out, mask, x0, W, 1.0f);
+    store_rot<2, false>(k, tiles, np, off, out, mask, x0, W, 1.0f);
 }
 
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
@@ -565,17 +564,33 @@ __device__ __forceinline__ void conv_rot_f16(const unsigned short* Ib2, const un
     const int m = lane & 15, kk = lane >> 4;
     const int s0 = 8 * kk + 16 - m;  // tpad index of this lane's first B operand, group 0
     const unsigned int* twp = tw + (s0 & 1) * TW_MAX + (s0 >> 1);
-    f4 k0[ROT_ROUNDS], k1[ROT_ROUNDS];
+    // three output tiles per round (t, t + 4, t + 8) share each group's tap operands
+    f4 k[ROT_ROUNDS][3];
 #pragma unroll
     for (int it = 0; it < ROT_ROUNDS; it++) {
-        const int t0 = wv + 8 * it, t1 = t0 + 4;
-        if (t0 >= tiles) break;
-        const bool has1 = t1 < tiles;
+        const int t0 = wv + 12 * it, t1 = t0 + 4, t2 = t0 + 8;
+        if (12 * it >= ROT_TILES_MAX || t0 >= tiles) break;
+        // (tiles no tile length reaches are left out at compile time)
+        const bool has1 = 12 * it + 4 < ROT_TILES_MAX && t1 < tiles;
+        const bool has2 = 12 * it + 8 < ROT_TILES_MAX && t2 < tiles;
         const int a0 = min(16 * t0 + m, nblk - 1), a1 = min(16 * t1 + m, nblk - 1);
+        const int a2 = min(16 * t2 + m, nblk - 1);
         const unsigned short* ia0 = Ib2 + 16 * a0 + 8 * kk;
         const unsigned short* ia1 = Ib2 + 16 * a1 + 8 * kk;
-        f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (has1) {
+        const unsigned short* ia2 = Ib2 + 16 * a2 + 8 * kk;
+        f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = c0, c2 = c0;
+        if (has2) {
+            for (int c = 0; c < C; c++) {
+                f16x8 bh, bl, ah, al, xh, xl, yh, yl;
+                load_b2(twp + 16 * c, bh, bl);
+                load_a2(ia0 + 32 * c, ah, al);
+                load_a2(ia1 + 32 * c, xh, xl);
+                load_a2(ia2 + 32 * c, yh, yl);
+                c0 = mfma3(ah, al, bh, bl, c0);
+                c1 = mfma3(xh, xl, bh, bl, c1);
+                c2 = mfma3(yh, yl, bh, bl, c2);
+            }
+        } else if (has1) {
             for (int c = 0; c < C; c++) {
                 f16x8 bh, bl, ah, al, xh, xl;
                 load_b2(twp + 16 * c, bh, bl);
@@ -592,10 +607,11 @@ __device__ __forceinline__ void conv_rot_f16(const unsigned short* Ib2, const un
                 c0 = mfma3(ah, al, bh, bl, c0);
             }
         }
-        k0[it] = c0;
-        k1[it] = c1;
+        k[it][0] = c0;
+        k[it][1] = c1;
+        k[it][2] = c2;
     }
-    store_rot<true>(k0, k1, tiles, np, off, out, mask, x0, W, rinv);
+    store_rot<3, true>(k, tiles, np, off, out, mask, x0, W, rinv);
 }
 
 // ---------------------------------------------------------------------------

@@ -2,7 +2,7 @@
 
 The kernels' tile and block shapes are compile-time parameters (csrc/engine.h):
 
-* PASTA_TILE        synthetic pixels per k_model workgroup (256 k - 16, k = 3..12);
+* PASTA_TILE        synthetic pixels per k_model workgroup (256 k - 16, k = 3..15);
 * PASTA_CONT_BLOCK  observed pixels per continuum reduction block;
 * PASTA_CHI2_GROUP  evaluations of one star per chi2 workgroup (4, 8 or 16).
 
@@ -29,8 +29,9 @@ ROOT = os.path.dirname(HERE)
 
 CANDIDATES = {
     "default": {},
+    "tile3568": {"PASTA_TILE": 3568},
+    "tile3312": {"PASTA_TILE": 3312},
     "tile2800": {"PASTA_TILE": 2800},
-    "tile2544": {"PASTA_TILE": 2544},
     "tile2032": {"PASTA_TILE": 2032},
     "block2048": {"PASTA_CONT_BLOCK": 2048},
     "block8192": {"PASTA_CONT_BLOCK": 8192},

@@ -17,7 +17,7 @@ def _tune():
 
 
 def test_candidates_respect_the_shape_rules():
-    """Tiles are 256 k - 16 pixels with 3 <= k <= 12, blocks whole 1024-pixel units, chi2
+    """Tiles are 256 k - 16 pixels with 3 <= k <= 15, blocks whole 1024-pixel units, chi2
     groups 4, 8 or 16 (the static_asserts of csrc/engine.h), one parameter per candidate."""
     t = _tune()
     assert list(t.CANDIDATES)[0] == "default" and t.CANDIDATES["default"] == {}
@@ -25,7 +25,7 @@ def test_candidates_respect_the_shape_rules():
         assert len(d) <= 1, name
         for k, v in d.items():
             if k == "PASTA_TILE":
-                assert v % 256 == 240 and 752 <= v <= 3056, name
+                assert v % 256 == 240 and 752 <= v <= 3824, name
             elif k == "PASTA_CONT_BLOCK":
                 assert v % 1024 == 0 and v >= 1024, name
             elif k == "PASTA_CHI2_GROUP":
