@@ -9,7 +9,7 @@ namespace pasta {
 // Tile and block shapes are build-time parameters (pasta_amd/tune.py rebuilds the library
 // with other values and times the batched iteration with each).
 #ifndef PASTA_TILE
-#define PASTA_TILE 3056
+#define PASTA_TILE 3312
 #endif
 #ifndef PASTA_CONT_BLOCK
 #define PASTA_CONT_BLOCK 4096

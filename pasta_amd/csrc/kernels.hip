@@ -27,11 +27,11 @@ namespace pasta {
 // Interpolated tile for the rotation stage: logical index k lives in slot k + 4*(k/16)
 // (4 pad floats after every 16), so that the 16-float-strided MFMA operand rows fall
 // in distinct LDS banks.  Logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KROT_MAX + 55.
-constexpr int ILOG = (7 + TILE + 1 + 2 * HD_MAX + KROT_MAX + 55 + 31) & ~31;  // 5440 at 3056
+constexpr int ILOG = (7 + TILE + 1 + 2 * HD_MAX + KROT_MAX + 55 + 31) & ~31;  // 5696 at 3312
 constexpr int ILDS = ILOG + ILOG / 4;
 // For the f16 rotation the tile is kept as two f16 arrays (hi, lo) of ILOGB elements
 // (logical extent <= 7 + (TILE + 1 + 2*HD_MAX) + KB_MAX + 71).
-constexpr int ILOGB = (7 + TILE + 1 + 2 * HD_MAX + KB_MAX + 71 + 7) & ~7;     // 4928 at 3056
+constexpr int ILOGB = (7 + TILE + 1 + 2 * HD_MAX + KB_MAX + 71 + 7) & ~7;     // 5184 at 3312
 // The rotation output R (TILE + 2 hd + 2 values, 16 zeros after) and the instrumental
 // stage's output C (<= TILE + 1 values) share k_model's one LDS buffer with the rotation's
 // operands: R overwrites the interpolated tile at the buffer's start once every wavefront

@@ -30,7 +30,7 @@ ROOT = os.path.dirname(HERE)
 CANDIDATES = {
     "default": {},
     "tile3568": {"PASTA_TILE": 3568},
-    "tile3312": {"PASTA_TILE": 3312},
+    "tile3056": {"PASTA_TILE": 3056},
     "tile2800": {"PASTA_TILE": 2800},
     "tile2032": {"PASTA_TILE": 2032},
     "block2048": {"PASTA_CONT_BLOCK": 2048},
