@@ -109,6 +109,10 @@ __device__ __forceinline__ float pow2_below_2_14(float b) {
     return ldexpf(1.0f, 14 - e);
 }
 
+// native 16-byte vector (HIP's uint4 struct is copied member-wise into 4 dword loads)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
 // ---------------------------------------------------------------------------
 // k_prep: one wavefront per evaluation
 // ---------------------------------------------------------------------------
@@ -127,6 +131,8 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     __shared__ int s_st;
     __shared__ int s_comp;
     __shared__ int s_fft;
+    __shared__ __align__(16) float s_taps[2][KROT_MAX + 4];  // rotation taps, composed taps
+    __shared__ float s_g[2 * HD_MAX + 1];                      // instrumental taps
     const bool active = ev.active;
     if (lane == 0) s_st = active ? local_weights(*G.geom, lab[0], lab[1], lab[2], ev.idx, ev.w) : -1;
     __syncthreads();
@@ -242,7 +248,10 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     }
     const int K = s_K;
     if (K > 0) {
+        // the taps are formed in LDS (the sequential sum and the composition read them back
+        // there, not from global memory), then stored to the evaluation's tap slot
         float* h = rot + (size_t)e * ROT_STRIDE;
+        float* hs = s_taps[0];
         const double eps = 0.6;
         const double vrot = vsini / C_KMS;
         const double mid = ((double)(K - 1) * dvelo) / 2.0;
@@ -254,45 +263,52 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
             if (y > 0.0)
                 k = (float)((2.0 * (1.0 - eps) * sqrt(y) + M_PI * eps / 2.0 * y) /
                             (M_PI * vrot * (1.0 - eps / 3.0)));
-            h[i] = k;
+            hs[i] = k;
         }
+        if (lane < 4) hs[K + lane] = 0.0f;  // the sum's last quad: + 0 leaves it unchanged
         __syncthreads();
         if (lane == 0) {  // f32 sequential sum, as kernel.iter().sum() (convolve_rv.rs:470)
             float s = 0.0f;
-            for (int i = 0; i < K; i++) s = s + h[i];
+            for (int i = 0; i < K; i += 4) {
+                const f4 v = *(const f4*)(hs + i);
+                s = s + v.x;
+                s = s + v.y;
+                s = s + v.z;
+                s = s + v.w;
+            }
             s_sum = s;
         }
         __syncthreads();
         const float s = s_sum;
-        for (int i = lane; i < K; i += 64) h[i] = h[i] / s;
+        for (int i = lane; i < K; i += 64) hs[i] = hs[i] / s;
         int Kt = K;
         if (s_comp) {
             // composed taps c[k] = sum_u g[u] h[k - u] of the normalised rotation taps h and
-            // the instrumental taps g, summed in f64 and rounded once to f32; staged in the
-            // split-tap area, which is written only after this
+            // the instrumental taps g, summed in f64 and rounded once to f32
             const DevStar& S = stars[ev.star];
             const int Kd = S.Kd;
             Kt = K + Kd - 1;
-            float* tmp = h + KROT_MAX;
+            float* cs = s_taps[1];
+            for (int u = lane; u < Kd; u += 64) s_g[u] = S.dkern[u];
             __syncthreads();
             for (int k = lane; k < Kt; k += 64) {
                 const int u0 = k - (K - 1) > 0 ? k - (K - 1) : 0;
                 const int u1 = k < Kd - 1 ? k : Kd - 1;
                 double a = 0.0;
-                for (int u = u0; u <= u1; u++) a += (double)S.dkern[u] * (double)h[k - u];
-                tmp[k] = (float)a;
+                for (int u = u0; u <= u1; u++) a += (double)s_g[u] * (double)hs[k - u];
+                cs[k] = (float)a;
             }
-            __syncthreads();
-            for (int k = lane; k < Kt; k += 64) h[k] = tmp[k];
+            hs = cs;
             if (lane == 0) ev.krot = Kt;
         }
+        __syncthreads();
+        for (int k = lane; k < Kt; k += 64) h[k] = hs[k];  // the taps in the evaluation's slot
         if (Kt <= KB_MAX && !s_fft) {
             // reversed, zero-padded taps tpad[16 + u] = h[Kt - 1 - u], scaled by a power of
             // 2 (largest tap just below 2^14) and split into f16 hi/lo, as pairs
             // tw[part][parity][q] = (tpad[2q + parity], tpad[2q + parity + 1])
-            __syncthreads();
             float tmax = 0.0f;
-            for (int k = lane; k < Kt; k += 64) tmax = fmaxf(tmax, h[k]);
+            for (int k = lane; k < Kt; k += 64) tmax = fmaxf(tmax, hs[k]);
             for (int o = 32; o; o >>= 1) tmax = fmaxf(tmax, __shfl_xor(tmax, o, 64));
             const float tsc = pow2_below_2_14(tmax);
             if (lane == 0) ev.rinv = 1.0f / (rsc * tsc);  // a power of 2: exact
@@ -304,7 +320,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
 #pragma unroll
                 for (int z = 0; z < 2; z++) {
                     const int u = 2 * q + par + z - 16;
-                    split2((u >= 0 && u < Kt) ? h[Kt - 1 - u] * tsc : 0.0f, v[z][0], v[z][1]);
+                    split2((u >= 0 && u < Kt) ? hs[Kt - 1 - u] * tsc : 0.0f, v[z][0], v[z][1]);
                 }
 #pragma unroll
                 for (int part = 0; part < 2; part++)
@@ -315,9 +331,6 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     }
 }
 
-// native 16-byte vector (HIP's uint4 struct is copied member-wise into 4 dword loads)
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void unpack8(const u32x4 v, float u[8]) {
     u[0] = (float)(v.x & 0xffffu); u[1] = (float)(v.x >> 16);
