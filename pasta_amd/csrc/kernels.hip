@@ -1550,9 +1550,9 @@ __global__ __launch_bounds__(NT) void k_const_fold(const DevStar* __restrict__ s
         okm[e] = s_ok[e] ? ~0u : 0u;
         offv[e] = s_off[e];
     }
-    // chunk c's reads go to register set c % FOLD_SETS, issued FOLD_SETS - 1 steps before
-    // its quotients are formed (at most 3 x 17 newer reads in flight, below the 63 the wait
-    // counter tracks)
+    // chunk c's reads go to register set c % FOLD_SETS, issued FOLD_SETS steps before its
+    // quotients are formed (at most (FOLD_SETS - 1) x 17 newer reads in flight, below the 63
+    // the wait counter tracks)
     float f[FOLD_SETS], m[FOLD_SETS][E];
     auto load = [&](int c, float& fv, float (&mv)[E]) {
         const int pc = min(c * FOLD_CH + t, N - 1);
