@@ -909,10 +909,12 @@ struct TileJob {
 __device__ __forceinline__ bool tile_job(const DevGrid& G, const DevStar* stars, const DevEval* evals,
                                          int e, int tile, const int* tbound, int tb_stride, int mode,
                                          TileJob& J) {
+    // branch-free: every read is issued before the job's validity is decided, so the reads of
+    // several jobs overlap (a failed evaluation's star index and tile-bound row exist; its
+    // bounds are clamped into the star's pixels)
     const DevEval& ev = evals[e];
-    if (ev.status != ST_OK) return false;
     const DevStar& S = stars[ev.star];
-    if (tile >= S.n_tiles) return false;
+    bool ok = ev.status == ST_OK && tile < S.n_tiles;
     J.ev = &ev;
     J.S = &S;
     J.e = e;
@@ -923,11 +925,14 @@ __device__ __forceinline__ bool tile_job(const DevGrid& G, const DevStar* stars,
         const int* tb = tbound + (size_t)e * tb_stride + tile;
         J.ilo = __builtin_amdgcn_readfirstlane(tb[0]);
         J.ihi = __builtin_amdgcn_readfirstlane(tb[1]);
-        if (J.ilo >= J.ihi) return false;
-        J.clo = jfloor(S.fidx[J.ilo] + sl);
-        chi = jfloor(S.fidx[J.ihi - 1] + sl) + 1;
+        const int nm = S.N - 1;
+        const int i0 = min(max(J.ilo, 0), nm), i1 = min(max(J.ihi - 1, 0), nm);
+        const double f0 = S.fidx[i0], f1 = S.fidx[i1];
+        ok = ok && J.ilo < J.ihi;
+        J.clo = jfloor(f0 + sl);
+        chi = jfloor(f1 + sl) + 1;
     } else {
-        if (t0 >= G.W) return false;
+        ok = ok && t0 < G.W;
         J.ilo = J.ihi = 0;
         J.clo = t0;
         chi = (t0 + TILE < G.W ? t0 + TILE : G.W) - 1;
@@ -945,7 +950,7 @@ __device__ __forceinline__ bool tile_job(const DevGrid& G, const DevStar* stars,
     J.g0 = first & ~7LL;
     J.off = (int)(first - J.g0);
     J.ng = (J.off + J.nR + 2 * hr + 7) >> 3;
-    return true;
+    return ok;
 }
 
 // store one interpolated 8-pixel group (local group index gl) in the job's tile layout
@@ -1337,17 +1342,17 @@ __global__ __launch_bounds__(NT, 4) void k_model(DevGrid G, const DevStar* __res
     bool hasB = false, hasC = false;
     long long bg0 = 0, cg0 = 0;
     int bng = 0, cng = 0;
-    if (ids[1] >= 0) {
+    {  // formed unconditionally (an absent evaluation repeats the first) so their reads overlap
         TileJob T;
-        hasB = tile_job(G, stars, evals, ids[1], tile, tbound, tb_stride, mode, T);
+        hasB = tile_job(G, stars, evals, ids[1] >= 0 ? ids[1] : ids[0], tile, tbound, tb_stride,
+                        mode, T) && ids[1] >= 0;
         bg0 = T.g0;
         bng = T.ng;
-    }
-    if (ids[2] >= 0) {
-        TileJob T;
-        hasC = tile_job(G, stars, evals, ids[2], tile, tbound, tb_stride, mode, T);
-        cg0 = T.g0;
-        cng = T.ng;
+        TileJob T2;
+        hasC = tile_job(G, stars, evals, ids[2] >= 0 ? ids[2] : ids[0], tile, tbound, tb_stride,
+                        mode, T2) && ids[2] >= 0;
+        cg0 = T2.g0;
+        cng = T2.ng;
     }
     // A work item with an evaluation on the FFT path runs its evaluations one at a time
     // through the one copy of the FFT stage (its registers would not fit beside a group's)
