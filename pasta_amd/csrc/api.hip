@@ -882,6 +882,19 @@ bool may_fft(const pasta_grid* g, double max_vsini, int extra_taps) {
     return (long long)v + 3 + extra_taps >= g->fft_min_taps;
 }
 
+// The most composed taps an evaluation of vsini <= max_vsini forms (the rotation kernel's
+// length as k_prep computes it, src/convolve_rv.rs:421-431, plus the composed instrumental
+// taps), with a margin; KROT_MAX when unbounded.  Sizes k_prep's LDS.
+int prep_taps_bound(const pasta_grid* g, double max_vsini, int extra_taps) {
+    if (!(max_vsini < 1e30)) return KROT_MAX;
+    if (!(max_vsini > 0.0)) return std::min(KROT_MAX, 16 + extra_taps);
+    const double eps = 0.6, dvelo = LN_10 * g->step, vrot = max_vsini / C_KMS;
+    const double a = 4.0 * (eps - 1.0) / (M_PI * eps);
+    const double v = std::ceil(2.0 * vrot / dvelo * std::sqrt(1.0 + a * a));
+    if (!(v < 1e6)) return KROT_MAX;
+    return (int)std::min<long long>(KROT_MAX, (long long)v + 3 + extra_taps + 16);
+}
+
 int composed_extra_taps(const pasta_star* s) {
     return s->dev.disp_kind == DISP_FIXED ? s->dev.Kd - 1 : 0;
 }
@@ -905,6 +918,11 @@ int batch_launch(Batch& B) {
     DevGrid G = B.grid->dev();
     G.stream_models = B.model_len * (long long)sizeof(float) > MALL_BYTES;
     const bool fft = may_fft(B.grid, B.max_vsini, B.stars);
+    {
+        int extra = 0;
+        for (const pasta_star* s : B.stars) extra = std::max(extra, composed_extra_taps(s));
+        G.prep_taps = prep_taps_bound(B.grid, B.max_vsini, extra);
+    }
     HIPCHK(launch_eval(G, B.d_stars.p, B.d_evals.p, B.n_evals, B.d_labels.p, B.d_rot.p,
                        B.d_tbound.p, B.tb_stride, B.d_hist.p, B.d_perm.p, B.d_pair_eq.p, B.d_items.p,
                        B.d_nitems.p, B.d_union.p, B.n_bm, B.d_bmom.p,

@@ -98,6 +98,8 @@ struct DevGrid {
     double synth_first_wl, synth_last_wl;  // 10^first, 10^(first + step*W)
     int n_models_bins;        // n_models + 1 (sort bins; the last holds failed evaluations)
     int fft_min_taps;         // rotation kernels of at least this many taps use the FFT path
+    int prep_taps;            // k_prep's LDS holds kernels of up to this many composed taps (a
+                              // bound from the launch's largest vsini; KROT_MAX when unknown)
     int stream_models;        // 1: k_model writes the resampled models past the caches (a
                               // launch whose models exceed the MALL; per launch)
 };

@@ -131,6 +131,7 @@ struct pasta_grid {
         g.synth_last_wl = last_wl;
         g.n_models_bins = geom.n_models + 1;
         g.fft_min_taps = fft_min_taps;
+        g.prep_taps = KROT_MAX;
         g.stream_models = 0;
         return g;
     }

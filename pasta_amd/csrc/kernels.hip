@@ -131,8 +131,12 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
     __shared__ int s_st;
     __shared__ int s_comp;
     __shared__ int s_fft;
-    __shared__ __align__(16) float s_taps[2][KROT_MAX + 4];  // rotation taps, composed taps
-    __shared__ float s_g[2 * HD_MAX + 1];                      // instrumental taps
+    // rotation taps and composed taps, G.prep_taps + 4 floats each (dynamic LDS: a launch
+    // whose kernels are short holds several times as many evaluations per CU)
+    extern __shared__ __align__(16) float s_dyn[];
+    const int cap = G.prep_taps;
+    float* const s_taps[2] = {s_dyn, s_dyn + cap + 4};
+    __shared__ float s_g[2 * HD_MAX + 1];  // instrumental taps
     const bool active = ev.active;
     if (lane == 0) s_st = active ? local_weights(*G.geom, lab[0], lab[1], lab[2], ev.idx, ev.w) : -1;
     __syncthreads();
@@ -181,6 +185,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
                 int K = rotation_len(vsini, dvelo);
                 if (vsini == 0.0 || K <= 2) krot = 0;
                 else if (K > KROT_MAX) st = ST_KERNEL;
+                else if (K > cap) st = ST_KERNEL;  // cap bounds every K of the launch: unreached
                 else krot = K;
             }
             if (!st && mode == 1) {  // synthetic-grid output: no RV shift / resample
@@ -212,7 +217,7 @@ __global__ __launch_bounds__(64) void k_prep(DevGrid G, const DevStar* __restric
                 // kernels have odd lengths, so the composed one is centred like its parts.)
                 const int hd = (S.Kd - 1) / 2;
                 comp = !st && S.disp_kind == DISP_FIXED && krot > 0 && (S.Kd & 1) &&
-                       krot + S.Kd - 1 <= KROT_MAX && j0 - hd >= 0 && jl + 1 + hd <= G.W - 1;
+                       krot + S.Kd - 1 <= cap && j0 - hd >= 0 && jl + 1 + hd <= G.W - 1;
             }
         }
         ev.status = st;
@@ -2366,7 +2371,9 @@ hipError_t launch_eval(const DevGrid& G, const DevStar* stars, DevEval* evals, i
     // without the FFT path the crossover is off for k_prep, so no evaluation selects it
     DevGrid Gp = G;
     if (!fft) Gp.fft_min_taps = 1 << 30;
-    hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64), 0, st, Gp, stars, labels, evals, rot,
+    if (Gp.prep_taps < 8 || Gp.prep_taps > KROT_MAX) Gp.prep_taps = KROT_MAX;
+    hipLaunchKernelGGL(k_prep, dim3(n_evals), dim3(64),
+                       2 * (Gp.prep_taps + 4) * sizeof(float), st, Gp, stars, labels, evals, rot,
                        tbound, tb_stride, mode);
     if (fft) hipLaunchKernelGGL(k_rot_spectrum, dim3(n_evals), dim3(NT), 0, st, evals, rot);
     const int nb = G.n_models_bins;
